@@ -1,0 +1,187 @@
+/*
+ * puffin_gpu.h -- C ABI of the MI355X (gfx950) implementation of Puffin's
+ * per-step integrator: the RK4 macroparticle push with tri-linear field
+ * interpolation and current deposition (rk4par) and the FFT diffraction step
+ * (DiffractionStep / diffractIM).
+ *
+ * Plain C: pointers, sizes and status codes only.  One context per GPU per
+ * process; a context is not re-entrant (calls on one context must not
+ * overlap).  All arrays use the reference's data layout:
+ *   particles : split SoA, one FP64 array per coordinate of length n
+ *               (sElX_G, sElY_G, sElZ2_G, sElPX_G, sElPY_G, sElGam_G and the
+ *               charge weights s_chi_bar_G; deriv_globals.f90:94,128-133);
+ *               PY holds -p_y (simple_electron_gen.f90:330).
+ *   field     : split real / imaginary FP64 arrays of length nx*ny*nz2,
+ *               x fastest, then y, then z2 (para_field.f90:1370-1381); the
+ *               imaginary part holds -A_y (rhs.f90:42-49).
+ *
+ * Status codes (the reference raises logical flags instead;
+ * system_interpolation.f90:60-80, derivative.f90:96-137):
+ *    0  ok
+ *    1  a macroparticle is outside the transverse mesh   (qInnerXYOK_G = .false.)
+ *    2  a macroparticle is beyond the z2 buffer           (qPArrOK_G   = .false.)
+ *    3  a macroparticle is beyond the temporal z2 mesh    (the reference STOPs)
+ *   <0  invalid argument / HIP error (see puff_last_error)
+ * When rk4par returns 1..3 the particle and field state is left unchanged,
+ * as the reference skips every RK4 update once a flag is raised
+ * (puffin_mpi_RK4.f90:187,213,225,249,257,296,305).
+ */
+#ifndef PUFFIN_GPU_H
+#define PUFFIN_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum puff_status {
+    PUFF_OK = 0,
+    PUFF_OUT_TRANSVERSE = 1,
+    PUFF_OUT_Z2_BUFFER = 2,
+    PUFF_OUT_Z2_MESH = 3,
+    PUFF_ERR_ARG = -1,
+    PUFF_ERR_HIP = -2,
+    PUFF_ERR_ALLOC = -3,
+    PUFF_ERR_STATE = -4
+};
+
+/* zUndType_G (setup_calcs.f90:757-777; bfields.f90:82,127,165,228) */
+enum puff_und_type {
+    PUFF_UND_ELLIPTICAL = 0, /* ''         : (fx, fy) free                       */
+    PUFF_UND_PLANEPOLE = 1,  /* 'planepole': y-focusing plane pole, sinh in bz   */
+    PUFF_UND_HELICAL = 2,    /* 'helical'  : fx = fy = 1                          */
+    PUFF_UND_CURVED = 3      /* 'curved'   : curved-pole, sinh/cosh in x and y    */
+};
+
+/* Arithmetic of the integrator. */
+enum puff_precision {
+    PUFF_FP64 = 0,  /* plain FP64, the reference's arithmetic                    */
+    PUFF_MIXED = 1  /* FP32 interpolation, deposition, field and FFT; FP64 z2,
+                       Gamma and RK4 accumulation of z2 and Gamma                 */
+};
+
+enum puff_mem_kind { PUFF_MEM_HOST = 0, PUFF_MEM_DEVICE = 1 };
+
+/*
+ * Setup constants of one undulator module: the module globals the reference's
+ * rk4par / DiffractionStep read (deriv_globals.f90:25-60,190-225,282-321,
+ * 377-415; rhs_vars.f90:23-44).  Filled once per module by the caller,
+ * exactly as setup / initUndulator leave them (acc_lattice.f90:848-895).
+ */
+typedef struct puff_params {
+    int32_t nx, ny, nz2;        /* NX_G, NY_G, NZ2_G mesh nodes                  */
+    int32_t periodic;           /* fieldMesh == iPeriodic (deriv_globals.f90:60) */
+    int32_t one_d;              /* qOneD_G: 1-D interpolation, bz = 0            */
+    int32_t und_type;           /* enum puff_und_type                            */
+    int32_t und_ends;           /* qUndEnds_G                                    */
+    int32_t focusing;           /* qFocussing_G                                  */
+    int32_t electrons_evolve;   /* qElectronsEvolve_G                            */
+    int32_t field_evolve;       /* qFieldEvolve_G                                */
+    int32_t coupling;           /* qElectronFieldCoupling_G                      */
+    int32_t filter;             /* qFilter (diffraction.f90:165)                 */
+    int32_t nbx, nby, nbz2;     /* absorbing boundary widths NBX_G, NBY_G, NBZ2_G */
+    int32_t reserved;
+    double dx, dy, dz2;         /* sLengthOfElmX_G, _Y_G, _Z2_G                  */
+    double rho, eta, kappa;     /* sRho_G, sEta_G, sKappa_G                      */
+    double gamma_r, aw;         /* sGammaR_G, sAw_G                              */
+    double fx, fy;              /* fx_G, fy_G (elliptical polarisation)          */
+    double szfs, szfe;          /* sZFS, sZFE: module-local zbar of the main section */
+    double kbeta_x, kbeta_y;    /* sKBetaXSF_G, sKBetaYSF_G (focusing)           */
+    double kx_und, ky_und;      /* kx_und_G, ky_und_G (curved pole)              */
+    double beta;                /* sBeta_G absorption coefficient                */
+    double filt_cutoff;         /* multiplyexp high-pass cutoff in kz2
+                                   (diffraction.f90:138-139)                     */
+    double pi_ref;              /* the reference's pi constant, a default-real
+                                   literal widened to FP64 (puffin_constants.f90:22) */
+} puff_params;
+
+typedef struct puff_ctx puff_ctx;
+
+/* Last error message of this thread (static storage, never NULL). */
+const char* puff_last_error(void);
+
+/* Number of visible HIP devices, or <0 on error.  Safe without a GPU. */
+int puff_device_count(void);
+
+/*
+ * Create a context on the current HIP device for n macroparticles.
+ * Allocates all device buffers for one undulator module
+ * (replaces allact_rk4_arrs, puffin_mpi_RK4.f90:374-418, and the FFTW plans
+ * of getTransformPlans_MultiD, transforms.f90:113-226).
+ */
+int puff_create(puff_ctx** ctx, const puff_params* params, int64_t n,
+                int32_t precision);
+int puff_destroy(puff_ctx* ctx);
+
+/* Run on the caller's HIP stream (hipStream_t cast to void*); NULL = own stream. */
+int puff_set_stream(puff_ctx* ctx, void* hip_stream);
+int puff_synchronize(puff_ctx* ctx);
+
+/* Particle and field exchange in the reference layout (see top of file).
+   mem_kind says whether the pointers are host or device memory.  Device
+   pointers must hold FP64 arrays of the reference layout as well. */
+int puff_upload_particles(puff_ctx* ctx, int32_t mem_kind, const double* x,
+                          const double* y, const double* z2, const double* px,
+                          const double* py, const double* gam,
+                          const double* chi);
+int puff_download_particles(puff_ctx* ctx, int32_t mem_kind, double* x,
+                            double* y, double* z2, double* px, double* py,
+                            double* gam);
+int puff_upload_field(puff_ctx* ctx, int32_t mem_kind, const double* ar,
+                      const double* ai);
+int puff_download_field(puff_ctx* ctx, int32_t mem_kind, double* ar,
+                        double* ai);
+
+/* Taper state n2col / n2col0 / undgrad (wiggler_taper.f90:16-53; the
+   reference mutates n2col0 once zbar passes sZFE). */
+int puff_set_taper(puff_ctx* ctx, double n2col, double n2col0, double undgrad);
+int puff_get_taper(puff_ctx* ctx, double* n2col, double* n2col0);
+
+/*
+ * Per-stage undulator strength override: when nvals > 0, stage zbar values
+ * are mapped to n2col = values[floor((zbar - z0) / period)] (clamped to the
+ * vector), replacing getAlpha's linear taper.  This is the per-period a_w
+ * model; it has no counterpart in the reference.  nvals = 0 restores the
+ * reference's linear taper.
+ */
+int puff_set_taper_table(puff_ctx* ctx, const double* values, int32_t nvals,
+                         double z0, double period);
+
+/*
+ * One RK4 step of particles and field together.
+ * Replaces rk4par(sZ, h, qD) (puffin_mpi_RK4.f90:38-366) with its four
+ * derivs / getrhs evaluations (derivative.f90:36-155, rhs.f90:60-330).
+ * sz is the module-local zbar (undulator.f90:199).  Returns a status code.
+ */
+int puff_rk4par(puff_ctx* ctx, double sz, double h);
+
+/*
+ * Free-space diffraction of the field over distance h.
+ * Replaces DiffractionStep(h, sAr, sAi, qOK) (diffraction.f90:209-348):
+ * forward 3-D DFT, multiplyexp, backward DFT, /ffact, AbsorptionStep.
+ */
+int puff_diffraction_step(puff_ctx* ctx, double h);
+
+/* Periodic z2 wrap applied when the reference recomputes its field layout
+   (para_field.f90:212-215).  No-op on a temporal mesh. */
+int puff_wrap_z2(puff_ctx* ctx);
+
+/* Power per z2 node: trapezoidal integral of |A|^2 over x and y
+   (gPowerP / fPower_3D, av_write.f90:39-202), or |A|^2 in 1-D. */
+int puff_power(puff_ctx* ctx, double* power_host);
+
+/* Slice bunching |b_h| for harmonic h over nslices slices of width
+   slice_len (getSliceTwiss, av_write.f90:405-773); b_host has nslices. */
+int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices,
+                  double slice_len, double* b_host);
+
+/* Device buffer of the current particle state (reference order is restored
+   by download); exposed for zero-copy callers.  which: 0..5 = x,y,z2,px,py,gam. */
+int puff_particle_count(puff_ctx* ctx, int64_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PUFFIN_GPU_H */

@@ -1,0 +1,1077 @@
+/*
+ * puffin_oracle.c -- CPU FP64 restatement of Puffin's per-step integrator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP implementation
+ * is compared against; only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load it.  The product never links or calls it.
+ *
+ * Every function cites the reference routine (UKFELs/Puffin, path:line) it
+ * restates.  Arithmetic is plain FP64 in the reference's order of operations
+ * where that order is visible in the source.  Periodic meshes use node indices
+ * taken modulo (NZ2-1): the reference keeps aliased tail planes that are
+ * summed into the head after every deposit and copied back
+ * (para_field.f90:924-1061, 1075-1138), which is the same arithmetic.
+ *
+ * OpenMP: the particle loops run in parallel; the deposit uses per-thread
+ * private grids reduced in thread order, so results do not depend on thread
+ * timing (they do depend on the thread count, at the rounding level).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/puffin_gpu.h"
+
+#define ORC_FLAG_TRANSVERSE 1
+#define ORC_FLAG_BUFFER 2
+#define ORC_FLAG_Z2MESH 4
+
+/* ------------------------------------------------------------------------ */
+/* p2 from Gamma and p_perp: getP2, gamma_to_p2.f90:84-85                    */
+/* ------------------------------------------------------------------------ */
+static inline double orc_p2(double gam, double pr, double pi, double eta,
+                            double gamma_r, double aw) {
+    return ((1.0 / sqrt(1.0 - 1.0 / (gamma_r * gamma_r * gam * gam) *
+                                  (1.0 + aw * aw * (pr * pr + pi * pi)))) -
+            1.0) / eta;
+}
+
+double orc_get_p2(double gam, double pr, double pi, double eta, double gamma_r,
+                  double aw) {
+    return orc_p2(gam, pr, pi, eta, gamma_r, aw);
+}
+
+/* getGamma, gamma_to_p2.f90:39-41 (inverse of getP2; pinned by a round trip) */
+double orc_get_gamma(double p2, double pr, double pi, double eta,
+                     double gamma_r, double aw) {
+    return sqrt((1.0 + (aw * aw * (pr * pr + pi * pi))) * pow(1.0 + eta * p2, 2.0) /
+                (eta * p2 * (eta * p2 + 2.0))) / gamma_r;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Taper: getAlpha, wiggler_taper.f90:39-45 (with its n2col0 side effect)    */
+/* ------------------------------------------------------------------------ */
+static double orc_alpha(const puff_params* p, double* taper /* n2col, n2col0, undgrad */,
+                        double sz) {
+    if (sz >= p->szfs && sz <= p->szfe) {
+        taper[0] = taper[1] + taper[2] * (sz - p->szfs);
+    } else if (sz > p->szfe) {
+        taper[1] = taper[0];
+    }
+    return taper[0];
+}
+
+/* adjUndPlace, puffin_equations.f90:218-254.  0 = start, 1 = main, 2 = end;
+   -1 for zbar < 0 (the reference STOPs). */
+static int orc_und_place(const puff_params* p, double sz) {
+    if (!p->und_ends) return 1;
+    if (sz < 0) return -1;
+    if (sz <= p->szfs) return 0;
+    if (sz >= p->szfe) return 2;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Undulator b-fields at one particle: getBXfield / getBYfield / getBZfield, */
+/* bfields.f90:59-309, 313-523, 528-881.                                      */
+/* ------------------------------------------------------------------------ */
+void orc_bfields(const puff_params* p, int place, double sz, double x, double y,
+                 double* bx, double* by, double* bz) {
+    const double rho = p->rho, pi = p->pi_ref, seta = sqrt(p->eta);
+    double szt = sz / 2.0 / rho;
+    double fx = p->fx, fy = p->fy;
+    double vx = 0, vy = 0, vz = 0;
+    int t = p->und_type;
+    if (t == PUFF_UND_HELICAL) { fx = 1.0; fy = 1.0; t = PUFF_UND_ELLIPTICAL; }
+
+    if (t == PUFF_UND_CURVED) {
+        const double kx = p->kx_und, ky = p->ky_und;
+        if (place == 0) {                                   /* bfields.f90:84-90,334-340,568-575 */
+            vx = kx / ky * sinh(kx * x) * sinh(ky * y) * szt / 4.0 / pi * sin(szt);
+            vy = cosh(kx * x) * cosh(ky * y) * szt / 4.0 / pi * sin(szt);
+            vz = seta / 2.0 / rho / ky * cosh(kx * x) * sinh(ky * y) * (szt / 4.0 / pi) * cos(szt);
+        } else if (place == 2) {                            /* :92-101,342-351,577-586 */
+            double s = (sz - p->szfe) / 2.0 / rho;
+            vx = kx / ky * sinh(kx * x) * sinh(ky * y) * (-s / 4.0 / pi + 1.0) * sin(s);
+            vy = cosh(kx * x) * cosh(ky * y) * (-s / 4.0 / pi + 1.0) * sin(s);
+            vz = seta / 2.0 / rho / kx * cosh(kx * x) * sinh(ky * y) * (-s / 4.0 / pi + 1.0) * cos(s);
+        } else {                                            /* :103-109,353-359,588-595 */
+            vx = kx / ky * sinh(kx * x) * sinh(ky * y) * sin(szt);
+            vy = cosh(kx * x) * cosh(ky * y) * sin(szt);
+            vz = seta / 2.0 / rho / kx * cosh(kx * x) * sinh(ky * y) * cos(szt);
+        }
+    } else if (t == PUFF_UND_PLANEPOLE) {
+        double a = seta / 2.0 / rho * y;
+        if (place == 0) {                                   /* :129-133,380-391,613-618 */
+            vx = 0.0;
+            vy = szt / 4.0 / pi * sin(szt);
+            vz = sinh(a) * (szt / 4.0 / pi) * cos(szt);
+        } else if (place == 2) {                            /* :135-142,395-407,620-628 */
+            double s = (sz - p->szfe) / 2.0 / rho;
+            vx = 0.0;
+            vy = (-s / 4.0 / pi + 1.0) * sin(s);
+            vz = sinh(a) * (-s / 4.0 / pi + 1.0) * cos(s);
+        } else {                                            /* :144-148,409-415,630-635 */
+            vx = 0.0;
+            vy = sin(szt);
+            vz = sinh(a) * cos(szt);
+        }
+    } else { /* elliptical (and helical with fx = fy = 1) */
+        if (place == 0) {                                   /* :239-257,481-485,763-787 */
+            vx = fx * (sz - pi * rho) / (6.0 * pi * rho) * cos(szt);
+            vz = -seta / 2.0 / rho * (sz - pi * rho) / (6.0 * pi * rho) * fx * x * sin(szt);
+            if (sz < pi * rho) {
+                vx = 0.0;
+                vz = 0.0;
+            } else if (sz > 7.0 * pi * rho) {
+                vx = fx * cos(szt);
+                vz = -seta / 2.0 / rho * fx * x * sin(szt);
+            }
+            vy = fy * szt / 4.0 / pi * sin(szt);
+            vz = vz + szt / 4.0 / pi * seta / 2.0 / rho * fy * y * cos(szt);
+        } else if (place == 2) {                            /* :259-278,487-494,803-838 */
+            double s = sz - p->szfe;
+            vx = -fx * (s - 7.0 * pi * rho) / (6.0 * pi * rho) * cos(s / 2.0 / rho);
+            vz = -seta / 2.0 / rho * (s - 7.0 * pi * rho) / (6.0 * pi * rho) * fx * x * sin(s / 2.0 / rho);
+            if (s < pi * rho) {
+                vx = fx * cos(s / 2.0 / rho);
+                vz = -seta / 2.0 / rho * fx * x * sin(s / 2.0 / rho);
+            } else if (s > 7.0 * pi * rho) {
+                vx = 0.0;
+                vz = 0.0;
+            }
+            double s2 = s / 2.0 / rho;
+            vy = fy * (-s2 / 4.0 / pi + 1.0) * sin(s2);
+            vz = vz + (-s / 8.0 / pi / rho + 1.0) * seta / 2.0 / rho * fy * y * cos(s / 2.0 / rho);
+        } else {                                            /* :280-284,496-500,862-867 */
+            vx = fx * cos(szt);
+            vy = fy * sin(szt);
+            vz = seta / 2.0 / rho * (-fx * x * sin(szt) + fy * y * cos(szt));
+        }
+    }
+    if (p->focusing) {                                      /* :299-306, 514-521 */
+        vx = seta * p->kbeta_y * p->kbeta_y / p->kappa * y + vx;
+        vy = -seta * p->kbeta_x * p->kbeta_x / p->kappa * x + vy;
+    }
+    if (p->one_d) vz = 0.0;                                 /* :544-551 */
+    *bx = vx;
+    *by = vy;
+    *bz = vz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interpolation weights and node of one particle.                          */
+/* 3-D: getInterps_3D, system_interpolation.f90:45-91 with the flat node     */
+/*      index of rhs.f90:178-182 (0-based here, whole mesh as the window,    */
+/*      halfx = ((NX-1)/2) dx as rhs.f90:395-396 with nspinDX = NX).         */
+/* 1-D: getInterps_1D, system_interpolation_1D.f90:41-59; rhs.f90:152.       */
+/* Returns flag bits; node[8] and w[8] are filled when no flag is raised.     */
+/* ------------------------------------------------------------------------ */
+static int orc_interp(const puff_params* p, double x, double y, double z2,
+                      int64_t* node, double* w) {
+    const int64_t nx = p->nx, ny = p->ny, ntr = (int64_t)p->nx * p->ny;
+    const int64_t nzp = p->periodic ? p->nz2 - 1 : p->nz2;
+    int flag = 0;
+    double zq = floor(z2 / p->dz2);
+    double locz2 = z2 - zq * p->dz2;      /* z2node-1 = zq */
+    double z2_in2 = locz2 / p->dz2;
+    double z2_in1 = 1.0 - z2_in2;
+    int64_t iz0, iz1;
+    if (p->periodic) {
+        if (!(zq == zq) || fabs(zq) > 4.0e15) return ORC_FLAG_Z2MESH;
+        int64_t q = (int64_t)zq;
+        iz0 = ((q % nzp) + nzp) % nzp;
+        iz1 = (iz0 + 1) % nzp;
+    } else {
+        /* temporal mesh: z2node >= NZ2 STOPs (system_interpolation.f90:70-75);
+           z2node < 1 would index before the mesh and is refused as well. */
+        if (!(zq >= 0.0 && zq <= (double)(nzp - 2))) return ORC_FLAG_Z2MESH;
+        iz0 = (int64_t)zq;
+        iz1 = iz0 + 1;
+    }
+    if (p->one_d) {
+        /* 1-D weights: lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1) */
+        w[0] = 1.0 - locz2 / p->dz2;
+        w[1] = 1.0 - w[0];
+        node[0] = iz0;
+        node[1] = iz1;
+        return 0;
+    }
+    double halfx = ((p->nx - 1) / 2.0) * p->dx;
+    double halfy = ((p->ny - 1) / 2.0) * p->dy;
+    double xq = floor((x + halfx) / p->dx);
+    double yq = floor((y + halfy) / p->dy);
+    /* xnode >= nspinDX .or. xnode < 1 raises both flags (:60-68) */
+    if (!(xq >= 0.0 && xq <= (double)(nx - 2))) flag |= ORC_FLAG_TRANSVERSE;
+    if (!(yq >= 0.0 && yq <= (double)(ny - 2))) flag |= ORC_FLAG_TRANSVERSE;
+    if (flag) return flag;
+    double locx = x + halfx - xq * p->dx;
+    double x_in2 = locx / p->dx, x_in1 = 1.0 - x_in2;
+    double locy = y + halfy - yq * p->dy;
+    double y_in2 = locy / p->dy, y_in1 = 1.0 - y_in2;
+    int64_t ix = (int64_t)xq, iy = (int64_t)yq;
+    int64_t b0 = iz0 * ntr + iy * nx + ix;
+    int64_t b1 = iz1 * ntr + iy * nx + ix;
+    w[0] = x_in1 * y_in1 * z2_in1;
+    w[1] = x_in2 * y_in1 * z2_in1;
+    w[2] = x_in1 * y_in2 * z2_in1;
+    w[3] = x_in2 * y_in2 * z2_in1;
+    w[4] = x_in1 * y_in1 * z2_in2;
+    w[5] = x_in2 * y_in1 * z2_in2;
+    w[6] = x_in1 * y_in2 * z2_in2;
+    w[7] = x_in2 * y_in2 * z2_in2;
+    node[0] = b0;
+    node[1] = b0 + 1;
+    node[2] = b0 + nx;
+    node[3] = b0 + nx + 1;
+    node[4] = b1;
+    node[5] = b1 + 1;
+    node[6] = b1 + nx;
+    node[7] = b1 + nx + 1;
+    return 0;
+}
+
+/* Exposed for tests: weights / nodes of one particle. */
+int orc_interp_one(const puff_params* p, double x, double y, double z2,
+                   int64_t* node, double* w) {
+    return orc_interp(p, x, y, z2, node, w);
+}
+
+typedef struct {
+    const puff_params* p;
+    double* taper;
+    int64_t n;
+    int nthreads;
+    int64_t nmesh;
+    /* private deposit grids, nthreads x (2 * nmesh) */
+    double* priv;
+    int64_t* nodes;   /* n x 8 */
+    double* wts;      /* n x 8 */
+    double* p2;       /* n */
+} orc_work;
+
+/* ------------------------------------------------------------------------ */
+/* One derivative evaluation: derivs / getrhs, derivative.f90:36-155,        */
+/* rhs.f90:60-330.  Inputs: field (ar, ai), trial particles; outputs: the six */
+/* particle derivatives and the field source dA (overwritten, not summed:    */
+/* rk4par always passes zeroed dadz arrays, puffin_mpi_RK4.f90:95-100,284).  */
+/* Returns flag bits.                                                        */
+/* ------------------------------------------------------------------------ */
+static int orc_derivs(orc_work* wk, double sz, const double* ar, const double* ai,
+                      const double* sx, const double* sy, const double* sz2,
+                      const double* spr, const double* spi, const double* sgam,
+                      const double* chi, double* dx, double* dy, double* dz2,
+                      double* dpr, double* dpi, double* dgam, double* dar,
+                      double* dai) {
+    const puff_params* p = wk->p;
+    const int64_t n = wk->n, nmesh = wk->nmesh;
+    const int nnb = p->one_d ? 2 : 8;
+    const double dV3 = p->dx * p->dy * p->dz2;        /* rhs.f90:361 */
+    const double sInv2rho = 1.0 / (2.0 * p->rho);     /* rhs.f90:366 */
+    const double n2col = orc_alpha(p, wk->taper, sz); /* rhs.f90:130 */
+    const int place = orc_und_place(p, sz);           /* rhs.f90:131 */
+    int flags = 0;
+
+    /* getP2 and getInterps over all particles first (rhs.f90:143,202) */
+#pragma omp parallel for reduction(| : flags) num_threads(wk->nthreads) schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        wk->p2[i] = orc_p2(sgam[i], spr[i], spi[i], p->eta, p->gamma_r, p->aw);
+        flags |= orc_interp(p, sx[i], sy[i], sz2[i], wk->nodes + 8 * i, wk->wts + 8 * i);
+    }
+    if (place < 0) flags |= ORC_FLAG_Z2MESH;
+
+    memset(dar, 0, sizeof(double) * nmesh);
+    memset(dai, 0, sizeof(double) * nmesh);
+    const int do_fields = (flags == 0);
+
+    /* getFFelecs + getSource (+ equations) */
+#pragma omp parallel num_threads(wk->nthreads)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double* pr_ = wk->priv + (int64_t)t * 2 * nmesh;
+        double* pi_ = pr_ + nmesh;
+        if (do_fields && wk->nthreads > 1) {
+            memset(pr_, 0, sizeof(double) * 2 * nmesh);
+        }
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; i++) {
+            double fr = 0.0, fi = 0.0;
+            const double p2 = wk->p2[i];
+            const int64_t* nd = wk->nodes + 8 * i;
+            const double* w = wk->wts + 8 * i;
+            if (do_fields) {
+                /* getFFelecs_3D: F = sum_k w_k A(node_k) (system_interpolation.f90:121-137) */
+                for (int k = 0; k < nnb; k++) {
+                    fr = w[k] * ar[nd[k]] + fr;
+                    fi = w[k] * ai[nd[k]] + fi;
+                }
+                /* getSource_3D: s = chi/dV3 (1 + eta p2) p / Gamma (:171-172, :208-209) */
+                double sr = ((chi[i] / dV3) * (1 + p->eta * p2) * spr[i] / sgam[i]);
+                double si = ((chi[i] / dV3) * (1 + p->eta * p2) * spi[i] / sgam[i]);
+                double* tr = (wk->nthreads > 1) ? pr_ : dar;
+                double* ti = (wk->nthreads > 1) ? pi_ : dai;
+                for (int k = 0; k < nnb; k++) {
+                    tr[nd[k]] = w[k] * sr + tr[nd[k]];
+                    ti[nd[k]] = w[k] * si + ti[nd[k]];
+                }
+            }
+            if (!p->coupling) { fr = 0.0; fi = 0.0; }     /* rhs.f90:221-224 */
+            if (p->electrons_evolve) {
+                double bx, by, bz;
+                orc_bfields(p, place, sz, sx[i], sy[i], &bx, &by, &bz);
+                const double eta = p->eta, kap = p->kappa, g = sgam[i];
+                /* puffin_equations.f90:172, 115-117, 144-146, 31-35, 61-65, 86-87 */
+                dz2[i] = p2;
+                dx[i] = 2 * p->rho * kap / sqrt(eta) * (1 + eta * p2) / g * spr[i];
+                dy[i] = -2 * p->rho * kap / sqrt(eta) * (1 + eta * p2) / g * spi[i];
+                dpr[i] = sInv2rho * (n2col * by - eta * p2 / (kap * kap) * fr) +
+                         kap * spi[i] / g * (1 + eta * p2) * n2col * bz;
+                dpi[i] = sInv2rho * (n2col * bx - eta * p2 / (kap * kap) * fi) -
+                         kap * spr[i] / g * (1 + eta * p2) * n2col * bz;
+                dgam[i] = -p->rho * (1 + eta * p2) / g * 2.0 * (spr[i] * fr + spi[i] * fi);
+            } else {                                      /* rhs.f90:301-308 */
+                dz2[i] = dx[i] = dy[i] = dpr[i] = dpi[i] = dgam[i] = 0.0;
+            }
+        }
+    }
+    if (do_fields && wk->nthreads > 1) {
+        /* reduce private grids in thread order (deterministic for a given count) */
+#pragma omp parallel for num_threads(wk->nthreads) schedule(static)
+        for (int64_t j = 0; j < nmesh; j++) {
+            double sr = 0.0, si = 0.0;
+            for (int t = 0; t < wk->nthreads; t++) {
+                sr += wk->priv[(int64_t)t * 2 * nmesh + j];
+                si += wk->priv[(int64_t)t * 2 * nmesh + nmesh + j];
+            }
+            dar[j] = sr;
+            dai[j] = si;
+        }
+    }
+    if (!p->field_evolve) {                               /* rhs.f90:294-297 */
+        memset(dar, 0, sizeof(double) * nmesh);
+        memset(dai, 0, sizeof(double) * nmesh);
+    }
+    return flags;
+}
+
+/* Periodic mesh: keep the aliased last plane equal to the first
+   (redistbackFFT tail copy, para_field.f90:3197-3244). */
+static void orc_periodic_alias(const puff_params* p, double* ar, double* ai) {
+    if (!p->periodic) return;
+    int64_t ntr = (int64_t)p->nx * p->ny;
+    memcpy(ar + (int64_t)(p->nz2 - 1) * ntr, ar, sizeof(double) * ntr);
+    memcpy(ai + (int64_t)(p->nz2 - 1) * ntr, ai, sizeof(double) * ntr);
+}
+
+static int orc_status(int flags) {
+    if (flags & ORC_FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
+    if (flags & ORC_FLAG_TRANSVERSE) return PUFF_OUT_TRANSVERSE;
+    if (flags & ORC_FLAG_BUFFER) return PUFF_OUT_Z2_BUFFER;
+    return PUFF_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rk4par, puffin_mpi_RK4.f90:38-366.                                        */
+/* taper = {n2col, n2col0, undgrad}, updated in place.                       */
+/* ------------------------------------------------------------------------ */
+int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
+               double* y, double* z2, double* pr, double* pi, double* gam,
+               const double* chi, double* ar, double* ai, double sz, double h,
+               int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    const int64_t nmesh = (int64_t)p->nx * p->ny * p->nz2;
+    orc_work wk;
+    wk.p = p;
+    wk.taper = taper;
+    wk.n = n;
+    wk.nthreads = nthreads;
+    wk.nmesh = nmesh;
+    wk.priv = (nthreads > 1) ? (double*)malloc(sizeof(double) * 2 * nmesh * nthreads) : NULL;
+    wk.nodes = (int64_t*)malloc(sizeof(int64_t) * 8 * (n > 0 ? n : 1));
+    wk.wts = (double*)malloc(sizeof(double) * 8 * (n > 0 ? n : 1));
+    wk.p2 = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+
+    /* work arrays of allact_rk4_arrs (puffin_mpi_RK4.f90:380-412) */
+    double* buf = (double*)calloc((size_t)(24 * n + 1), sizeof(double));
+    double *dxdx = buf, *dydx = buf + n, *dz2dx = buf + 2 * n, *dpxdx = buf + 3 * n,
+           *dpydx = buf + 4 * n, *dpz2dx = buf + 5 * n;
+    double *dxt = buf + 6 * n, *dyt = buf + 7 * n, *dz2t = buf + 8 * n,
+           *dpxt = buf + 9 * n, *dpyt = buf + 10 * n, *dpz2t = buf + 11 * n;
+    double *dxm = buf + 12 * n, *dym = buf + 13 * n, *dz2m = buf + 14 * n,
+           *dpxm = buf + 15 * n, *dpym = buf + 16 * n, *dpz2m = buf + 17 * n;
+    double *xt = buf + 18 * n, *yt = buf + 19 * n, *z2t = buf + 20 * n,
+           *pxt = buf + 21 * n, *pyt = buf + 22 * n, *pz2t = buf + 23 * n;
+    double* fbuf = (double*)calloc((size_t)(10 * nmesh), sizeof(double));
+    double *dar0 = fbuf, *dai0 = fbuf + nmesh, *dar1 = fbuf + 2 * nmesh,
+           *dai1 = fbuf + 3 * nmesh, *dar2 = fbuf + 4 * nmesh, *dai2 = fbuf + 5 * nmesh;
+    double *atr = fbuf + 6 * nmesh, *ati = fbuf + 7 * nmesh;
+    double *dar3 = fbuf + 8 * nmesh, *dai3 = fbuf + 9 * nmesh;
+
+    const double hh = h * 0.5, h6 = h / 6.0;
+    double szh = sz + hh;
+    int flags = 0, status = PUFF_OK;
+
+    /* first stage: derivs at (sz, A0, y0) (:174-177) */
+    flags = orc_derivs(&wk, sz, ar, ai, x, y, z2, pr, pi, gam, chi, dxdx, dydx,
+                       dz2dx, dpxdx, dpydx, dpz2dx, dar0, dai0);
+    if (flags) goto done;
+    for (int64_t i = 0; i < n; i++) {                  /* :189-198 */
+        xt[i] = x[i] + hh * dxdx[i];
+        yt[i] = y[i] + hh * dydx[i];
+        z2t[i] = z2[i] + hh * dz2dx[i];
+        pxt[i] = pr[i] + hh * dpxdx[i];
+        pyt[i] = pi[i] + hh * dpydx[i];
+        pz2t[i] = gam[i] + hh * dpz2dx[i];
+    }
+    for (int64_t j = 0; j < nmesh; j++) {
+        atr[j] = ar[j] + hh * dar0[j];
+        ati[j] = ai[j] + hh * dai0[j];
+    }
+    /* second stage (:213-217) */
+    flags = orc_derivs(&wk, szh, atr, ati, xt, yt, z2t, pxt, pyt, pz2t, chi, dxt,
+                       dyt, dz2t, dpxt, dpyt, dpz2t, dar1, dai1);
+    if (flags) goto done;
+    for (int64_t i = 0; i < n; i++) {                  /* :226-235 */
+        xt[i] = x[i] + hh * dxt[i];
+        yt[i] = y[i] + hh * dyt[i];
+        z2t[i] = z2[i] + hh * dz2t[i];
+        pxt[i] = pr[i] + hh * dpxt[i];
+        pyt[i] = pi[i] + hh * dpyt[i];
+        pz2t[i] = gam[i] + hh * dpz2t[i];
+    }
+    for (int64_t j = 0; j < nmesh; j++) {
+        atr[j] = ar[j] + hh * dar1[j];
+        ati[j] = ai[j] + hh * dai1[j];
+    }
+    /* third stage (:249-253) */
+    flags = orc_derivs(&wk, szh, atr, ati, xt, yt, z2t, pxt, pyt, pz2t, chi, dxm,
+                       dym, dz2m, dpxm, dpym, dpz2m, dar2, dai2);
+    if (flags) goto done;
+    for (int64_t i = 0; i < n; i++) {                  /* :258-279 */
+        xt[i] = x[i] + h * dxm[i];
+        yt[i] = y[i] + h * dym[i];
+        z2t[i] = z2[i] + h * dz2m[i];
+        pxt[i] = pr[i] + h * dpxm[i];
+        pyt[i] = pi[i] + h * dpym[i];
+        pz2t[i] = gam[i] + h * dpz2m[i];
+        dxm[i] = dxt[i] + dxm[i];
+        dym[i] = dyt[i] + dym[i];
+        dz2m[i] = dz2t[i] + dz2m[i];
+        dpxm[i] = dpxt[i] + dpxm[i];
+        dpym[i] = dpyt[i] + dpym[i];
+        dpz2m[i] = dpz2t[i] + dpz2m[i];
+    }
+    for (int64_t j = 0; j < nmesh; j++) {              /* :266-285 */
+        atr[j] = ar[j] + h * dar2[j];
+        ati[j] = ai[j] + h * dai2[j];
+        dar2[j] = dar1[j] + dar2[j];
+        dai2[j] = dai1[j] + dai2[j];
+    }
+    /* fourth stage at sz + h (:291-300) */
+    szh = sz + h;
+    flags = orc_derivs(&wk, szh, atr, ati, xt, yt, z2t, pxt, pyt, pz2t, chi, dxt,
+                       dyt, dz2t, dpxt, dpyt, dpz2t, dar3, dai3);
+    if (flags) goto done;
+    for (int64_t i = 0; i < n; i++) {                  /* :307-312 */
+        x[i] = x[i] + h6 * (dxdx[i] + dxt[i] + 2.0 * dxm[i]);
+        y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
+        z2[i] = z2[i] + h6 * (dz2dx[i] + dz2t[i] + 2.0 * dz2m[i]);
+        pr[i] = pr[i] + h6 * (dpxdx[i] + dpxt[i] + 2.0 * dpxm[i]);
+        pi[i] = pi[i] + h6 * (dpydx[i] + dpyt[i] + 2.0 * dpym[i]);
+        gam[i] = gam[i] + h6 * (dpz2dx[i] + dpz2t[i] + 2.0 * dpz2m[i]);
+    }
+    for (int64_t j = 0; j < nmesh; j++) {              /* :314-315 */
+        ar[j] = ar[j] + h6 * (dar0[j] + dar3[j] + 2.0 * dar2[j]);
+        ai[j] = ai[j] + h6 * (dai0[j] + dai3[j] + 2.0 * dai2[j]);
+    }
+    orc_periodic_alias(p, ar, ai);
+
+done:
+    status = orc_status(flags);
+    free(wk.priv);
+    free(wk.nodes);
+    free(wk.wts);
+    free(wk.p2);
+    free(buf);
+    free(fbuf);
+    return status;
+}
+
+/* Source-only evaluation (tests): dA of one getSource_3D call for the given
+   particles and the interpolated field F at each particle. */
+int orc_interp_deposit(const puff_params* p, int64_t n, const double* x,
+                       const double* y, const double* z2, const double* pr,
+                       const double* pi, const double* gam, const double* chi,
+                       const double* ar, const double* ai, double* fr,
+                       double* fi, double* dar, double* dai) {
+    const int64_t nmesh = (int64_t)p->nx * p->ny * p->nz2;
+    const int nnb = p->one_d ? 2 : 8;
+    const double dV3 = p->dx * p->dy * p->dz2;
+    int flags = 0;
+    memset(dar, 0, sizeof(double) * nmesh);
+    memset(dai, 0, sizeof(double) * nmesh);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t nd[8];
+        double w[8];
+        int f = orc_interp(p, x[i], y[i], z2[i], nd, w);
+        flags |= f;
+        fr[i] = fi[i] = 0.0;
+        if (f) continue;
+        double p2 = orc_p2(gam[i], pr[i], pi[i], p->eta, p->gamma_r, p->aw);
+        double sr = ((chi[i] / dV3) * (1 + p->eta * p2) * pr[i] / gam[i]);
+        double si = ((chi[i] / dV3) * (1 + p->eta * p2) * pi[i] / gam[i]);
+        for (int k = 0; k < nnb; k++) {
+            fr[i] = w[k] * ar[nd[k]] + fr[i];
+            fi[i] = w[k] * ai[nd[k]] + fi[i];
+            dar[nd[k]] += w[k] * sr;
+            dai[nd[k]] += w[k] * si;
+        }
+    }
+    return orc_status(flags);
+}
+
+/* Periodic wrap of z2 (para_field.f90:212-215). */
+void orc_wrap_z2(const puff_params* p, int64_t n, double* z2) {
+    if (!p->periodic) return;
+    double lenz2 = p->dz2 * (double)(p->nz2 - 1);
+    for (int64_t i = 0; i < n; i++) z2[i] = z2[i] - (floor(z2[i] / lenz2) * lenz2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mixed-radix complex DFT (FFTW convention: sign -1 forward, +1 backward,    */
+/* unnormalised), standing in for fftw_mpi_plan_dft_3d / execute             */
+/* (transforms.f90:186-211, 423).                                            */
+/* ------------------------------------------------------------------------ */
+static int orc_smallest_factor(int n) {
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (int f = 3; (int64_t)f * f <= n; f += 2)
+        if (n % f == 0) return f;
+    return n;
+}
+
+/* y[k] = sum_j x[j*xs] exp(sign 2 pi i j k / n); y contiguous. */
+static void orc_fft_rec(int n, const double* x, int64_t xs, double* y, int sign,
+                        double* tmp) {
+    if (n == 1) {
+        y[0] = x[0];
+        y[1] = x[1];
+        return;
+    }
+    int r = orc_smallest_factor(n);
+    int m = n / r;
+    if (m == 1) {
+        for (int k = 0; k < n; k++) {
+            double sr = 0.0, si = 0.0;
+            for (int j = 0; j < n; j++) {
+                int64_t e = ((int64_t)j * k) % n;
+                double a = sign * 2.0 * M_PI * (double)e / (double)n;
+                double c = cos(a), s = sin(a);
+                double xr = x[2 * j * xs], xi = x[2 * j * xs + 1];
+                sr += xr * c - xi * s;
+                si += xr * s + xi * c;
+            }
+            y[2 * k] = sr;
+            y[2 * k + 1] = si;
+        }
+        return;
+    }
+    /* decimation in time: r sub-transforms of length m */
+    for (int q = 0; q < r; q++) orc_fft_rec(m, x + 2 * q * xs, xs * r, y + 2 * q * m, sign, tmp);
+    for (int k = 0; k < m; k++) {
+        for (int s = 0; s < r; s++) {
+            int64_t kk = k + (int64_t)s * m;
+            double sr = 0.0, si = 0.0;
+            for (int q = 0; q < r; q++) {
+                int64_t e = ((int64_t)q * kk) % n;
+                double a = sign * 2.0 * M_PI * (double)e / (double)n;
+                double c = cos(a), sn = sin(a);
+                double yr = y[2 * (q * m + k)], yi = y[2 * (q * m + k) + 1];
+                sr += yr * c - yi * sn;
+                si += yr * sn + yi * c;
+            }
+            tmp[2 * kk] = sr;
+            tmp[2 * kk + 1] = si;
+        }
+    }
+    memcpy(y, tmp, sizeof(double) * 2 * n);
+}
+
+/* 1-D transform of n complex values (interleaved) in place. */
+void orc_fft1d(double* data, int n, int sign) {
+    double* y = (double*)malloc(sizeof(double) * 2 * n);
+    double* in = (double*)malloc(sizeof(double) * 2 * n);
+    double* tmp = (double*)malloc(sizeof(double) * 2 * n);
+    memcpy(in, data, sizeof(double) * 2 * n);
+    orc_fft_rec(n, in, 1, y, sign, tmp);
+    memcpy(data, y, sizeof(double) * 2 * n);
+    free(y);
+    free(in);
+    free(tmp);
+}
+
+/* 3-D transform of a (n3, n2, n1) row-major complex array (n1 fastest). */
+void orc_fft3d(double* a, int n1, int n2, int n3, int sign, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    int nmax = n1 > n2 ? n1 : n2;
+    if (n3 > nmax) nmax = n3;
+#pragma omp parallel num_threads(nthreads)
+    {
+        double* line = (double*)malloc(sizeof(double) * 2 * nmax);
+        double* y = (double*)malloc(sizeof(double) * 2 * nmax);
+        double* tmp = (double*)malloc(sizeof(double) * 2 * nmax);
+        /* axis 1 (x) */
+#pragma omp for schedule(static)
+        for (int64_t l = 0; l < (int64_t)n2 * n3; l++) {
+            double* base = a + 2 * l * n1;
+            orc_fft_rec(n1, base, 1, y, sign, tmp);
+            memcpy(base, y, sizeof(double) * 2 * n1);
+        }
+        /* axis 2 (y) */
+#pragma omp for schedule(static)
+        for (int64_t l = 0; l < (int64_t)n1 * n3; l++) {
+            int64_t i1 = l % n1, i3 = l / n1;
+            double* base = a + 2 * (i3 * (int64_t)n2 * n1 + i1);
+            for (int j = 0; j < n2; j++) {
+                line[2 * j] = base[2 * (int64_t)j * n1];
+                line[2 * j + 1] = base[2 * (int64_t)j * n1 + 1];
+            }
+            orc_fft_rec(n2, line, 1, y, sign, tmp);
+            for (int j = 0; j < n2; j++) {
+                base[2 * (int64_t)j * n1] = y[2 * j];
+                base[2 * (int64_t)j * n1 + 1] = y[2 * j + 1];
+            }
+        }
+        /* axis 3 (z2) */
+#pragma omp for schedule(static)
+        for (int64_t l = 0; l < (int64_t)n1 * n2; l++) {
+            double* base = a + 2 * l;
+            int64_t st = (int64_t)n1 * n2;
+            for (int j = 0; j < n3; j++) {
+                line[2 * j] = base[2 * j * st];
+                line[2 * j + 1] = base[2 * j * st + 1];
+            }
+            orc_fft_rec(n3, line, 1, y, sign, tmp);
+            for (int j = 0; j < n3; j++) {
+                base[2 * j * st] = y[2 * j];
+                base[2 * j * st + 1] = y[2 * j + 1];
+            }
+        }
+        free(line);
+        free(y);
+        free(tmp);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* k-vectors: GetKValues, transforms.f90:494-573 (exact pi = 4 atan(1)).     */
+/* ------------------------------------------------------------------------ */
+void orc_kvalues(const puff_params* p, double* kx, double* ky, double* kz2) {
+    const double pi = 4.0 * atan(1.0);
+    int nzp = p->periodic ? p->nz2 - 1 : p->nz2;
+    double lx = (double)p->nx * p->dx, ly = (double)p->ny * p->dy;
+    double lz = (double)nzp * p->dz2;
+    int maxx = (int)ceil((double)p->nx / 2.0), maxy = (int)ceil((double)p->ny / 2.0);
+    int maxz = (int)ceil((double)nzp / 2.0);
+    for (int i = 0; i < p->nx; i++) kx[i] = 2.0 * pi * (double)(i >= maxx ? i - p->nx : i) / lx;
+    for (int i = 0; i < p->ny; i++) ky[i] = 2.0 * pi * (double)(i >= maxy ? i - p->ny : i) / ly;
+    for (int i = 0; i < nzp; i++) kz2[i] = 2.0 * pi * (double)(i >= maxz ? i - nzp : i) / lz;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Absorbing masks, masks.f90.                                              */
+/* ------------------------------------------------------------------------ */
+/* linspace, base_functions.f90:67-97 */
+static void orc_linspace(double a, double b, int n, double* out) {
+    if (n > 1) {
+        double d = (b - a) / (double)(n - 1);
+        for (int i = 0; i < n; i++) out[i] = a + i * d;
+    } else if (n == 1) {
+        out[0] = a;
+    }
+}
+
+/* Mask1D2, masks.f90:266-302 (1-based Fortran slices restated 0-based) */
+static void orc_mask1d2(const double* x, double dx, int nx, int nb, double pi,
+                        double* m) {
+    double lb = (nb - 1) * dx;
+    double x01 = x[nx - 1], x02 = x[0];
+    for (int i = nx - nb; i < nx; i++) m[i] = pow(cos(pi * (x[i] - x01) / (2.0 * lb)), 2.0);
+    for (int i = 0; i < nb; i++) m[i] = pow(cos(pi * (x[i] - x02) / (2.0 * lb)), 2.0);
+    for (int i = nb; i < nx - nb; i++) m[i] = 0.0;
+}
+
+/* Maskf4, masks.f90:329-361 */
+static void orc_maskf4(const double* x, double dx, int nx, int nb, double pi,
+                       double* m) {
+    double lb = (nb - 1) * dx;
+    double x01 = x[nx - nb], x02 = x[nb - 1];
+    for (int i = nx - nb; i < nx; i++) m[i] = pow(cos(pi * (x[i] - x01) / (2.0 * lb)), 2.0);
+    for (int i = 0; i < nb; i++) m[i] = pow(cos(pi * (x[i] - x02) / (2.0 * lb)), 2.0);
+    for (int i = nb; i < nx - nb; i++) m[i] = 1.0;
+}
+
+/* getMask + Mask2D, masks.f90:50-87, 136-189: mask(ix,iy) =
+   Maskf3_y(iy) Mask1D2_x(ix) + Maskf4_x(ix) Mask1D2_y(iy), Maskf3 = 1. */
+void orc_mask_xy(const puff_params* p, double* mask) {
+    int nx = p->nx, ny = p->ny;
+    double* x = (double*)malloc(sizeof(double) * nx);
+    double* y = (double*)malloc(sizeof(double) * ny);
+    double* mx = (double*)malloc(sizeof(double) * nx);
+    double* my = (double*)malloc(sizeof(double) * ny);
+    double* mx2 = (double*)malloc(sizeof(double) * nx);
+    double lenx = (nx - 1) * p->dx, leny = (ny - 1) * p->dy;
+    orc_linspace(-lenx / 2, lenx / 2, nx, x);
+    orc_linspace(-leny / 2, leny / 2, ny, y);
+    orc_mask1d2(x, p->dx, nx, p->nbx, p->pi_ref, mx);
+    orc_mask1d2(y, p->dy, ny, p->nby, p->pi_ref, my);
+    orc_maskf4(x, p->dx, nx, p->nbx, p->pi_ref, mx2);
+    for (int iy = 0; iy < ny; iy++)
+        for (int ix = 0; ix < nx; ix++) mask[iy * nx + ix] = 1.0 * mx[ix] + mx2[ix] * my[iy];
+    free(x);
+    free(y);
+    free(mx);
+    free(my);
+    free(mx2);
+}
+
+/* getZ2Mask, masks.f90:366-430, including its Lf = (nx - 1*dx) expression
+   (evaluates to nx - dx); zero on a periodic mesh (diffraction.f90:447). */
+void orc_mask_z2(const puff_params* p, double* mz) {
+    int nzp = p->periodic ? p->nz2 - 1 : p->nz2;
+    if (p->periodic) {
+        for (int i = 0; i < nzp; i++) mz[i] = 0.0;
+        return;
+    }
+    double dx = p->dz2;
+    int nx = p->nz2, nb = p->nbz2, loc_start = 0, loc_nx = nzp;
+    double* x = (double*)malloc(sizeof(double) * loc_nx);
+    orc_linspace((double)loc_start * dx, (double)(loc_start + loc_nx - 1) * dx, loc_nx, x);
+    double lb = (nb - 1) * dx;
+    double lf = (nx - 1 * dx);
+    double x0 = lf - lb / 2.0;
+    int fbn = nx - nb + 1;
+    for (int ind = 1; ind <= loc_nx; ind++) {
+        mz[ind - 1] = 0.0;
+        if (loc_start + ind >= fbn) mz[ind - 1] = pow(cos(p->pi_ref * (x[ind - 1] - x0) / lb), 2.0);
+    }
+    free(x);
+}
+
+/* ------------------------------------------------------------------------ */
+/* DiffractionStep, diffraction.f90:209-348, with multiplyexp (:111-184) and */
+/* AbsorptionStep (:367-523).  ar / ai: full mesh, nz2 planes.               */
+/* ------------------------------------------------------------------------ */
+int orc_diffraction_step(const puff_params* p, double* ar, double* ai, double h,
+                         int nthreads) {
+    const int nx = p->nx, ny = p->ny;
+    const int nzp = p->periodic ? p->nz2 - 1 : p->nz2;
+    const int64_t ntr = (int64_t)nx * ny, nn = ntr * nzp;
+    const double ffact = (double)nx * (double)ny * (double)nzp;   /* setup.f90:334-346 */
+    double* a = (double*)malloc(sizeof(double) * 2 * nn);
+    double* anb = (double*)malloc(sizeof(double) * 2 * nn);
+    double* kx = (double*)malloc(sizeof(double) * nx);
+    double* ky = (double*)malloc(sizeof(double) * ny);
+    double* kz = (double*)malloc(sizeof(double) * nzp);
+    double* mxy = (double*)malloc(sizeof(double) * ntr);
+    double* mz = (double*)malloc(sizeof(double) * nzp);
+    orc_kvalues(p, kx, ky, kz);
+    orc_mask_xy(p, mxy);
+    orc_mask_z2(p, mz);
+
+    for (int64_t j = 0; j < nn; j++) {                   /* pack, :255-264 */
+        a[2 * j] = ar[j];
+        a[2 * j + 1] = ai[j];
+    }
+    orc_fft3d(a, nx, ny, nzp, -1, nthreads);              /* :272 */
+    const double cutoff = p->filt_cutoff;                /* multiplyexp :138-139 */
+    for (int iz = 0; iz < nzp; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                int64_t j = ((int64_t)iz * ny + iy) * nx + ix;
+                if ((kz[iz] > cutoff) || (kz[iz] < -cutoff)) {
+                    if (kz[iz] != 0.0) {
+                        double th = h * (kx[ix] * kx[ix] + ky[iy] * ky[iy]) / (2.0 * kz[iz]);
+                        double c = cos(th), s = sin(th);
+                        double re = a[2 * j], im = a[2 * j + 1];
+                        a[2 * j] = c * re - s * im;
+                        a[2 * j + 1] = s * re + c * im;
+                    }
+                } else if (p->filter) {
+                    a[2 * j] = 0.0;
+                    a[2 * j + 1] = 0.0;
+                }
+            }
+    orc_fft3d(a, nx, ny, nzp, +1, nthreads);              /* :297 */
+    for (int64_t j = 0; j < 2 * nn; j++) a[j] = a[j] / ffact; /* :310 */
+
+    /* AbsorptionStep: split (:454-470) */
+    for (int iz = 0; iz < nzp; iz++)
+        for (int64_t t = 0; t < ntr; t++) {
+            int64_t j = (int64_t)iz * ntr + t;
+            double m = mxy[t] + (mz[iz] * (1.0 - mxy[t]));
+            double keep = (1.0 - m);
+            anb[2 * j] = keep * a[2 * j];
+            anb[2 * j + 1] = keep * a[2 * j + 1];
+            a[2 * j] = m * a[2 * j];
+            a[2 * j + 1] = m * a[2 * j + 1];
+        }
+    orc_fft3d(a, nx, ny, nzp, -1, nthreads);              /* :476 */
+    for (int iz = 0; iz < nzp; iz++)                     /* :480-501 */
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                if (kz[iz] != 0.0) {
+                    int64_t j = ((int64_t)iz * ny + iy) * nx + ix;
+                    double f = exp(-h * p->beta * (fabs(kx[ix]) + fabs(ky[iy])) /
+                                   (sqrt(fabs(2.0 * kz[iz]))));
+                    a[2 * j] *= f;
+                    a[2 * j + 1] *= f;
+                }
+            }
+    orc_fft3d(a, nx, ny, nzp, +1, nthreads);              /* :505 */
+    for (int64_t j = 0; j < 2 * nn; j++) a[j] = a[j] / ffact + anb[j]; /* :513-517 */
+
+    for (int64_t j = 0; j < nn; j++) {                   /* unpack, :327-336 */
+        ar[j] = a[2 * j];
+        ai[j] = a[2 * j + 1];
+    }
+    orc_periodic_alias(p, ar, ai);
+    free(a);
+    free(anb);
+    free(kx);
+    free(ky);
+    free(kz);
+    free(mxy);
+    free(mz);
+    return PUFF_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Diagnostics                                                               */
+/* ------------------------------------------------------------------------ */
+/* Power per z2 node (gPower / fPower_1D / fPower_3D, av_write.f90:108-202):
+   trapezoidal integral over x and y on the linspace axes of initPowerCalc
+   (:79-98).  f32_accumulate = 1 reproduces m_trapz / m_trapz2D being default
+   REAL functions (:217-276): the running sum is rounded to single precision
+   after every term. */
+void orc_power(const puff_params* p, const double* ar, const double* ai,
+               int f32_accumulate, double* power) {
+    int nx = p->nx, ny = p->ny;
+    int64_t ntr = (int64_t)nx * ny;
+    if (nx == 1 && ny == 1) {
+        for (int iz = 0; iz < p->nz2; iz++) power[iz] = pow(fabs(ar[iz]), 2.0) + pow(fabs(ai[iz]), 2.0);
+        return;
+    }
+    double* xa = (double*)malloc(sizeof(double) * nx);
+    double* ya = (double*)malloc(sizeof(double) * ny);
+    double* cul = (double*)malloc(sizeof(double) * ny);
+    double lx = p->dx * (nx - 1), ly = p->dy * (ny - 1);
+    orc_linspace(-lx / 2.0, lx / 2.0, nx, xa);
+    orc_linspace(-ly / 2.0, ly / 2.0, ny, ya);
+    for (int iz = 0; iz < p->nz2; iz++) {
+        const double* r = ar + iz * ntr;
+        const double* im = ai + iz * ntr;
+        for (int iy = 0; iy < ny; iy++) {
+            double s = 0.0;
+            float sf = 0.0f;
+            for (int ix = 0; ix < nx - 1; ix++) {
+                int64_t a0 = iy * nx + ix, a1 = a0 + 1;
+                double f0 = pow(fabs(r[a0]), 2.0) + pow(fabs(im[a0]), 2.0);
+                double f1 = pow(fabs(r[a1]), 2.0) + pow(fabs(im[a1]), 2.0);
+                if (f32_accumulate)
+                    sf = (float)((double)sf + (xa[ix + 1] - xa[ix]) * (f1 + f0) / 2.0);
+                else
+                    s = s + (xa[ix + 1] - xa[ix]) * (f1 + f0) / 2.0;
+            }
+            cul[iy] = f32_accumulate ? (double)sf : s;
+        }
+        double s = 0.0;
+        float sf = 0.0f;
+        for (int iy = 0; iy < ny - 1; iy++) {
+            if (f32_accumulate)
+                sf = (float)((double)sf + (ya[iy + 1] - ya[iy]) * (cul[iy + 1] + cul[iy]) / 2.0);
+            else
+                s = s + (ya[iy + 1] - ya[iy]) * (cul[iy + 1] + cul[iy]) / 2.0;
+        }
+        power[iz] = f32_accumulate ? (double)sf : s;
+    }
+    free(xa);
+    free(ya);
+    free(cul);
+}
+
+/* Slice bunching of one harmonic, getSliceTwiss (av_write.f90:498-650,
+   725-729).  Linear weights split each particle between slices is and is+1
+   on a temporal mesh; the reference writes past the array for is = nslices,
+   which is skipped here.  A slice holding no particle of its own reports 0
+   (sdata(is) = 0 branch, av_write.f90:732-749). */
+int orc_bunching(const puff_params* p, int64_t n, const double* z2,
+                 const double* chi, int harmonic, int nslices, double slice,
+                 double* b) {
+    double* br = (double*)calloc(nslices + 1, sizeof(double));
+    double* bi = (double*)calloc(nslices + 1, sizeof(double));
+    double* sdata = (double*)calloc(nslices + 1, sizeof(double));
+    int status = PUFF_OK;
+    for (int64_t i = 0; i < n; i++) {
+        int is = (int)ceil(z2[i] / slice);
+        if (p->periodic) {
+            if (is >= nslices) is = is - ((int)floor((double)is / (double)nslices) * nslices) + 1;
+            if (is < 1) { status = PUFF_ERR_ARG; continue; }
+        } else {
+            if ((is > nslices) || (is < 1)) { status = PUFF_ERR_ARG; continue; }
+        }
+        sdata[is - 1] += chi[i];
+        double ph = (double)harmonic * z2[i] / (2.0 * p->rho);
+        double c = cos(ph), s = sin(ph);
+        double w1 = ((is * slice - z2[i]) / slice);
+        br[is - 1] += chi[i] * c * w1;
+        bi[is - 1] += chi[i] * s * w1;
+        if (!p->periodic && is < nslices) {
+            double w2 = (-((is - 1) * slice - z2[i]) / slice);
+            br[is] += chi[i] * c * w2;
+            bi[is] += chi[i] * s * w2;
+        }
+    }
+    for (int is = 0; is < nslices; is++)
+        b[is] = (sdata[is] > 0.0) ? sqrt(br[is] * br[is] + bi[is] * bi[is]) / slice : 0.0;
+    free(br);
+    free(bi);
+    free(sdata);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Undulator entry / exit matching and the exit corrector, used to rebuild    */
+/* the reference's own 1-D test run.                                         */
+/* ------------------------------------------------------------------------ */
+/* pxOffset / pyOffset (init_conds.f90:188-223) */
+static double orc_px_offset(double z, double rho, double uy, double n2col) {
+    return -uy * n2col * cos(z / (2.0 * rho));
+}
+static double orc_py_offset(double z, double rho, double ux, double n2col) {
+    return -ux * n2col * sin(z / (2.0 * rho));
+}
+
+/* matchIn (sign = +1, acc_lattice.f90:743-828) and matchOut (sign = -1,
+   :638-725) for the elliptical / helical undulator, with xOffSet_Array and
+   yOffSet_Array (init_conds.f90:65-86, 129-152). */
+void orc_match(const puff_params* p, double n2col, int64_t n, double* x,
+               double* y, double* px, double* py, const double* gam, double sz,
+               int sign) {
+    double spx0 = orc_px_offset(sz, p->rho, p->fy, n2col);
+    double spy0 = -1.0 * orc_py_offset(sz, p->rho, p->fx, n2col);
+    /* matchIn passes -spy0_offset as py, matchOut passes spy0_offset */
+    double pyarg = (sign > 0) ? -spy0 : spy0;
+    double ncx = p->aw * p->aw;
+    double ncy = 2.0 * p->aw * p->aw / (p->fx * p->fx + p->fy * p->fy);
+    for (int64_t i = 0; i < n; i++) {
+        double gj = p->gamma_r * gam[i];
+        double xo = -p->fy * 4.0 * p->kappa * pow(p->rho, 2.0) / sqrt(p->eta) *
+                    (p->gamma_r / sqrt(gj * gj - (1.0 + ncx * (spx0 * spx0 + pyarg * pyarg)))) *
+                    n2col * sin(sz / (2.0 * p->rho));
+        double yo = p->fx * 4.0 * p->kappa * pow(p->rho, 2.0) / sqrt(p->eta) *
+                    (p->gamma_r / sqrt(gj * gj - (1.0 + ncy * (spx0 * spx0 + pyarg * pyarg)))) *
+                    n2col * cos(sz / (2.0 * p->rho));
+        if (sign > 0) {
+            x[i] = x[i] + xo;
+            y[i] = y[i] + yo;
+            px[i] = px[i] + spx0;
+            py[i] = py[i] + spy0;
+        } else {
+            x[i] = x[i] - xo;
+            y[i] = y[i] - yo;
+            px[i] = px[i] - spx0;
+            py[i] = py[i] - spy0;
+        }
+    }
+}
+
+/* correctTrans, acc_lattice.f90:601-623 with arr_mean_para_weighted
+   (base_functions.f90:552-573). */
+void orc_correct_trans(int64_t n, double* x, double* y, double* px, double* py,
+                       const double* chi) {
+    double ws = 0.0, spx = 0.0, spy = 0.0, sx = 0.0, sy = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        ws += chi[i];
+        spx += chi[i] * px[i];
+        spy += chi[i] * py[i];
+        sx += chi[i] * x[i];
+        sy += chi[i] * y[i];
+    }
+    spx /= ws;
+    spy /= ws;
+    sx /= ws;
+    sy /= ws;
+    for (int64_t i = 0; i < n; i++) {
+        px[i] -= spx;
+        py[i] -= spy;
+        x[i] -= sx;
+        y[i] -= sy;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* UndSection's split-step schedule, undulator.f90:103-333 (no resume, no    */
+/* retries: a raised flag ends the section with its status).                 */
+/* write_mask[i] != 0 marks a write after step i+1: power(z2) of the written */
+/* field is stored to powers[w * nz2] for the w-th write.                    */
+/* taper = {n2col, n2col0, undgrad}; *sz_local is updated.                   */
+/* ------------------------------------------------------------------------ */
+int orc_und_section(const puff_params* p, double* taper, int64_t n, double* x,
+                    double* y, double* z2, double* pr, double* pi, double* gam,
+                    const double* chi, double* ar, double* ai, int nsteps,
+                    double h, int diffraction, int steps4diff, int redist_every,
+                    int64_t* csteps, const uint8_t* write_mask, double* powers,
+                    double* sz_local, int nthreads) {
+    int status = PUFF_OK;
+    double szl = *sz_local;
+    double dzds = 0.0, dzdf, dzd;
+    int nw = 0;
+    orc_wrap_z2(p, n, z2);                                 /* getLocalFieldIndices, :103 */
+    if (diffraction) {                                     /* first half step, :156-164 */
+        dzds = (double)steps4diff * h / 2.0;
+        orc_diffraction_step(p, ar, ai, dzds, nthreads);
+    }
+    for (int istep = 1; istep <= nsteps; istep++) {
+        (*csteps)++;
+        status = orc_rk4par(p, taper, n, x, y, z2, pr, pi, gam, chi, ar, ai, szl, h, nthreads);
+        if (status != PUFF_OK) break;
+        szl = szl + h;                                     /* :222 */
+        int wr = write_mask ? write_mask[istep - 1] : 0;
+        if (diffraction && ((istep % steps4diff == 0) || (istep == nsteps))) {
+            dzdf = dzds;                                   /* :237 */
+            if (istep == nsteps) {
+                dzds = 0.0;
+            } else if ((istep + steps4diff) <= nsteps) {
+                dzds = (double)steps4diff * h / 2;
+            } else {
+                int left = nsteps - istep;
+                dzds = (double)left * h / 2;
+            }
+            if (!wr) {
+                dzd = dzdf + dzds;                         /* :269-271 */
+                orc_diffraction_step(p, ar, ai, dzd, nthreads);
+            } else {
+                orc_diffraction_step(p, ar, ai, dzdf, nthreads);  /* :279 */
+                if (powers) orc_power(p, ar, ai, 0, powers + (int64_t)nw * p->nz2);
+                nw++;
+                if (dzds > 0.0) orc_diffraction_step(p, ar, ai, dzds, nthreads);
+                wr = 0;                                    /* written already */
+            }
+        }
+        if (wr) {
+            if (powers) orc_power(p, ar, ai, 0, powers + (int64_t)nw * p->nz2);
+            nw++;
+        }
+        if (redist_every > 0 && (*csteps % redist_every) == 0) orc_wrap_z2(p, n, z2);
+    }
+    *sz_local = szl;
+    return status;
+}
