@@ -163,6 +163,11 @@ int puff_rk4par(puff_ctx* ctx, double sz, double h);
  */
 int puff_diffraction_step(puff_ctx* ctx, double h);
 
+/* In-place unnormalised 3-D DFT of the field (the first NZ2' planes; sign -1
+   forward, +1 backward).  Replaces Transform / Transform_MultiD with the
+   plans of getTransformPlans_MultiD (transforms.f90:343-435, 186-211). */
+int puff_transform(puff_ctx* ctx, int32_t sign);
+
 /* Periodic z2 wrap applied when the reference recomputes its field layout
    (para_field.f90:212-215).  No-op on a temporal mesh. */
 int puff_wrap_z2(puff_ctx* ctx);
@@ -176,9 +181,25 @@ int puff_power(puff_ctx* ctx, double* power_host);
 int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices,
                   double slice_len, double* b_host);
 
-/* Device buffer of the current particle state (reference order is restored
-   by download); exposed for zero-copy callers.  which: 0..5 = x,y,z2,px,py,gam. */
+/* Number of macroparticles of the context. */
 int puff_particle_count(puff_ctx* ctx, int64_t* n);
+
+/*
+ * Kernel timing with HIP events on the context's stream, replacing the
+ * reference's wall-clock prints per step and per diffraction phase
+ * (Get_time, undulator.f90:316-321; diffraction.f90:234-320).
+ * Kinds: 0..3 RK4 stage kernels 1..4, 4 field combination, 5 diffraction
+ * (all passes), 6 step start.  ms[k] is the summed duration, counts[k] the
+ * number of launches since the last reset; nkinds <= PUFF_PROF_KINDS.
+ */
+#define PUFF_PROF_KINDS 7
+int puff_profile_enable(puff_ctx* ctx, int32_t on);
+int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds,
+                      int32_t reset);
+
+/* Active box of the last step in cells (x0, x1, y0, y1, z0, z1), the region
+   whose nodes the deposit touched. */
+int puff_active_box(puff_ctx* ctx, int32_t* box6);
 
 #ifdef __cplusplus
 }

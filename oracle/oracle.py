@@ -61,6 +61,7 @@ def lib():
                                    ctypes.c_double, _dp]
         l.orc_match.argtypes = [P, ctypes.c_double, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp,
                                 ctypes.c_double, ctypes.c_int]
+        l.orc_set_deposit_atomic.argtypes = [ctypes.c_int]
         l.orc_correct_trans.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
         l.orc_und_section.restype = ctypes.c_int
         l.orc_und_section.argtypes = ([P, _dp, ctypes.c_int64] + [_dp] * 9 +
@@ -220,6 +221,11 @@ def match(mp: ModuleParams, n2col: float, parts: dict, sz: float, sign: int):
     n = parts["x"].size
     lib().orc_match(ctypes.byref(p), n2col, n, _ptr(parts["x"]), _ptr(parts["y"]),
                     _ptr(parts["px"]), _ptr(parts["py"]), _ptr(parts["gam"]), sz, sign)
+
+
+def set_deposit_atomic(on: bool):
+    """Shared-grid atomic deposit (timing runs) instead of private grids."""
+    lib().orc_set_deposit_atomic(int(on))
 
 
 def correct_trans(parts: dict):

@@ -241,6 +241,13 @@ int orc_interp_one(const puff_params* p, double x, double y, double z2,
     return orc_interp(p, x, y, z2, node, w);
 }
 
+/* Deposit mode of the threaded oracle: 0 = per-thread private grids reduced
+   in thread order (deterministic, the checker's default); 1 = shared grid
+   with atomic adds, as the reference's !$OMP ATOMIC scatter
+   (system_interpolation.f90:174-241), used when timing the CPU baseline. */
+static int g_deposit_atomic = 0;
+void orc_set_deposit_atomic(int on) { g_deposit_atomic = on; }
+
 typedef struct {
     const puff_params* p;
     double* taper;
@@ -284,9 +291,14 @@ static int orc_derivs(orc_work* wk, double sz, const double* ar, const double* a
     }
     if (place < 0) flags |= ORC_FLAG_Z2MESH;
 
-    memset(dar, 0, sizeof(double) * nmesh);
-    memset(dai, 0, sizeof(double) * nmesh);
+#pragma omp parallel for num_threads(wk->nthreads) schedule(static)
+    for (int64_t j = 0; j < nmesh; j++) {
+        dar[j] = 0.0;
+        dai[j] = 0.0;
+    }
     const int do_fields = (flags == 0);
+    const int atomic_dep = g_deposit_atomic && wk->nthreads > 1;
+    const int private_dep = wk->nthreads > 1 && !atomic_dep;
 
     /* getFFelecs + getSource (+ equations) */
 #pragma omp parallel num_threads(wk->nthreads)
@@ -297,7 +309,7 @@ static int orc_derivs(orc_work* wk, double sz, const double* ar, const double* a
 #endif
         double* pr_ = wk->priv + (int64_t)t * 2 * nmesh;
         double* pi_ = pr_ + nmesh;
-        if (do_fields && wk->nthreads > 1) {
+        if (do_fields && private_dep) {
             memset(pr_, 0, sizeof(double) * 2 * nmesh);
         }
 #pragma omp for schedule(static)
@@ -315,11 +327,20 @@ static int orc_derivs(orc_work* wk, double sz, const double* ar, const double* a
                 /* getSource_3D: s = chi/dV3 (1 + eta p2) p / Gamma (:171-172, :208-209) */
                 double sr = ((chi[i] / dV3) * (1 + p->eta * p2) * spr[i] / sgam[i]);
                 double si = ((chi[i] / dV3) * (1 + p->eta * p2) * spi[i] / sgam[i]);
-                double* tr = (wk->nthreads > 1) ? pr_ : dar;
-                double* ti = (wk->nthreads > 1) ? pi_ : dai;
-                for (int k = 0; k < nnb; k++) {
-                    tr[nd[k]] = w[k] * sr + tr[nd[k]];
-                    ti[nd[k]] = w[k] * si + ti[nd[k]];
+                if (atomic_dep) {
+                    for (int k = 0; k < nnb; k++) {
+#pragma omp atomic
+                        dar[nd[k]] += w[k] * sr;
+#pragma omp atomic
+                        dai[nd[k]] += w[k] * si;
+                    }
+                } else {
+                    double* tr = private_dep ? pr_ : dar;
+                    double* ti = private_dep ? pi_ : dai;
+                    for (int k = 0; k < nnb; k++) {
+                        tr[nd[k]] = w[k] * sr + tr[nd[k]];
+                        ti[nd[k]] = w[k] * si + ti[nd[k]];
+                    }
                 }
             }
             if (!p->coupling) { fr = 0.0; fi = 0.0; }     /* rhs.f90:221-224 */
@@ -341,7 +362,7 @@ static int orc_derivs(orc_work* wk, double sz, const double* ar, const double* a
             }
         }
     }
-    if (do_fields && wk->nthreads > 1) {
+    if (do_fields && private_dep) {
         /* reduce private grids in thread order (deterministic for a given count) */
 #pragma omp parallel for num_threads(wk->nthreads) schedule(static)
         for (int64_t j = 0; j < nmesh; j++) {
@@ -393,7 +414,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
     wk.n = n;
     wk.nthreads = nthreads;
     wk.nmesh = nmesh;
-    wk.priv = (nthreads > 1) ? (double*)malloc(sizeof(double) * 2 * nmesh * nthreads) : NULL;
+    wk.priv = (nthreads > 1 && !g_deposit_atomic) ? (double*)malloc(sizeof(double) * 2 * nmesh * nthreads) : NULL;
     wk.nodes = (int64_t*)malloc(sizeof(int64_t) * 8 * (n > 0 ? n : 1));
     wk.wts = (double*)malloc(sizeof(double) * 8 * (n > 0 ? n : 1));
     wk.p2 = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
@@ -422,6 +443,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
     flags = orc_derivs(&wk, sz, ar, ai, x, y, z2, pr, pi, gam, chi, dxdx, dydx,
                        dz2dx, dpxdx, dpydx, dpz2dx, dar0, dai0);
     if (flags) goto done;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t i = 0; i < n; i++) {                  /* :189-198 */
         xt[i] = x[i] + hh * dxdx[i];
         yt[i] = y[i] + hh * dydx[i];
@@ -430,6 +452,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
         pyt[i] = pi[i] + hh * dpydx[i];
         pz2t[i] = gam[i] + hh * dpz2dx[i];
     }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t j = 0; j < nmesh; j++) {
         atr[j] = ar[j] + hh * dar0[j];
         ati[j] = ai[j] + hh * dai0[j];
@@ -438,6 +461,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
     flags = orc_derivs(&wk, szh, atr, ati, xt, yt, z2t, pxt, pyt, pz2t, chi, dxt,
                        dyt, dz2t, dpxt, dpyt, dpz2t, dar1, dai1);
     if (flags) goto done;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t i = 0; i < n; i++) {                  /* :226-235 */
         xt[i] = x[i] + hh * dxt[i];
         yt[i] = y[i] + hh * dyt[i];
@@ -446,6 +470,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
         pyt[i] = pi[i] + hh * dpyt[i];
         pz2t[i] = gam[i] + hh * dpz2t[i];
     }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t j = 0; j < nmesh; j++) {
         atr[j] = ar[j] + hh * dar1[j];
         ati[j] = ai[j] + hh * dai1[j];
@@ -454,6 +479,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
     flags = orc_derivs(&wk, szh, atr, ati, xt, yt, z2t, pxt, pyt, pz2t, chi, dxm,
                        dym, dz2m, dpxm, dpym, dpz2m, dar2, dai2);
     if (flags) goto done;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t i = 0; i < n; i++) {                  /* :258-279 */
         xt[i] = x[i] + h * dxm[i];
         yt[i] = y[i] + h * dym[i];
@@ -468,6 +494,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
         dpym[i] = dpyt[i] + dpym[i];
         dpz2m[i] = dpz2t[i] + dpz2m[i];
     }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t j = 0; j < nmesh; j++) {              /* :266-285 */
         atr[j] = ar[j] + h * dar2[j];
         ati[j] = ai[j] + h * dai2[j];
@@ -479,6 +506,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
     flags = orc_derivs(&wk, szh, atr, ati, xt, yt, z2t, pxt, pyt, pz2t, chi, dxt,
                        dyt, dz2t, dpxt, dpyt, dpz2t, dar3, dai3);
     if (flags) goto done;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t i = 0; i < n; i++) {                  /* :307-312 */
         x[i] = x[i] + h6 * (dxdx[i] + dxt[i] + 2.0 * dxm[i]);
         y[i] = y[i] + h6 * (dydx[i] + dyt[i] + 2.0 * dym[i]);
@@ -487,6 +515,7 @@ int orc_rk4par(const puff_params* p, double* taper, int64_t n, double* x,
         pi[i] = pi[i] + h6 * (dpydx[i] + dpyt[i] + 2.0 * dpym[i]);
         gam[i] = gam[i] + h6 * (dpz2dx[i] + dpz2t[i] + 2.0 * dpz2m[i]);
     }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t j = 0; j < nmesh; j++) {              /* :314-315 */
         ar[j] = ar[j] + h6 * (dar0[j] + dar3[j] + 2.0 * dar2[j]);
         ai[j] = ai[j] + h6 * (dai0[j] + dai3[j] + 2.0 * dai2[j]);

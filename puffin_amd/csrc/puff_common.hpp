@@ -1,0 +1,85 @@
+// Shared definitions of the gfx950 integrator: error handling, the context,
+// complex helpers and wavefront reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/puffin_gpu.h"
+
+namespace puff {
+
+void set_error(const std::string& msg);
+
+#define PUFF_HIP(call)                                                        \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::puff::set_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+            return PUFF_ERR_HIP;                                              \
+        }                                                                     \
+    } while (0)
+
+template <typename R>
+struct cplx {
+    R x, y;
+};
+
+template <typename R>
+__device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) {
+    return {a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> csub(cplx<R> a, cplx<R> b) {
+    return {a.x - b.x, a.y - b.y};
+}
+
+// Flag bits raised by the stage kernels (status codes are derived on the host).
+constexpr int FLAG_TRANSVERSE = 1;
+constexpr int FLAG_BUFFER = 2;
+constexpr int FLAG_Z2MESH = 4;
+
+// Active box of one RK4 step: min/max cell index along x, y, z2.
+struct Box {
+    int lo[3];
+    int hi[3];
+};
+
+// Wavefront (64-lane) reductions.
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Per-stage scalars computed on the host: taper strength and the b-field
+// coefficients of the stage's zbar (see puff_bfield in puff_push.hip).
+struct StageScalars {
+    double n2col;
+    double c[8];
+    int und_type;
+};
+
+// 1-D FFT plan of one axis: radices (product N), twiddle table on device.
+struct FftAxis {
+    int n;
+    int npass;
+    int radix[16];
+    const void* tw;   // cplx<R>[n]: exp(-2 pi i e / n)
+};
+
+}  // namespace puff

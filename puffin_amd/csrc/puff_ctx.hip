@@ -1,0 +1,1009 @@
+// Context, host-side setup and the C ABI (include/puffin_gpu.h).
+//
+// Host-side restatements used to set up the device:
+//   stage b-field coefficients   undulator/bfields.f90:36-881
+//   taper                        undulator/wiggler_taper.f90:16-53
+//   undulator place              interaction/puffin_equations.f90:218-254
+//   k-vectors                    diffraction/transforms.f90:472-590
+//   masks                        diffraction/masks.f90:50-430
+//   ffact                        setup/setup.f90:334-346
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "puff_common.hpp"
+#include "puff_kernels.hpp"
+
+namespace puff {
+
+static thread_local std::string g_err = "";
+void set_error(const std::string& m) { g_err = m; }
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+template <typename R>
+struct Particles {
+    PartPtrs<R> y;
+    PartPtrs<R> k[3];
+    R* chi = nullptr;
+};
+
+struct Ctx {
+    puff_params p{};
+    int precision = PUFF_FP64;
+    int64_t n = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::vector<void*> allocs;
+    // particles (typed views; R chosen by precision)
+    Particles<double> pd;
+    Particles<float> pf;
+    // field
+    void* A = nullptr;
+    void* dA[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* anb = nullptr;
+    double* kx = nullptr;
+    double* ky = nullptr;
+    double* kz = nullptr;
+    double* mxy = nullptr;
+    double* mz = nullptr;
+    FftAxis ax{}, ay{}, az{};
+    int* flags = nullptr;
+    Box* box = nullptr;
+    int* h_flags = nullptr;   // pinned
+    double* stage_buf = nullptr;   // FP64 staging (2 * max(n, nmesh))
+    int64_t stage_len = 0;
+    double* diag = nullptr;
+    int64_t diag_len = 0;
+    // taper state (wiggler_taper.f90)
+    double n2col = 1.0, n2col0 = 1.0, undgrad = 0.0;
+    std::vector<double> taper_table;
+    double taper_z0 = 0.0, taper_period = 1.0;
+    MeshGeom mesh{};
+    Physics phys{};
+    // kernel timing (puff_profile_*)
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    struct Pend {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Pend> pending;
+    double prof_ms[PUFF_PROF_KINDS] = {};
+    int64_t prof_n[PUFF_PROF_KINDS] = {};
+
+    int64_t nmesh() const { return (int64_t)p.nx * p.ny * p.nz2; }
+    int nzp() const { return p.periodic ? p.nz2 - 1 : p.nz2; }
+};
+
+template <typename R>
+static Particles<R>& parts(Ctx* c);
+template <>
+Particles<double>& parts<double>(Ctx* c) { return c->pd; }
+template <>
+Particles<float>& parts<float>(Ctx* c) { return c->pf; }
+
+static int dalloc(Ctx* c, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return PUFF_ERR_ALLOC;
+    }
+    c->allocs.push_back(*p);
+    e = hipMemset(*p, 0, bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMemset: ") + hipGetErrorString(e));
+        return PUFF_ERR_HIP;
+    }
+    return PUFF_OK;
+}
+
+#define PUFF_TRY(x)                   \
+    do {                              \
+        int r_ = (x);                 \
+        if (r_ != PUFF_OK) return r_; \
+    } while (0)
+
+// ---- FFT plans ----------------------------------------------------------------
+
+static void factorize(int n, FftAxis& ax) {
+    ax.n = n;
+    ax.npass = 0;
+    int m = n;
+    while (m % 4 == 0) { ax.radix[ax.npass++] = 4; m /= 4; }
+    while (m % 2 == 0) { ax.radix[ax.npass++] = 2; m /= 2; }
+    while (m % 3 == 0) { ax.radix[ax.npass++] = 3; m /= 3; }
+    for (int f = 5; m > 1 && (int64_t)f * f <= m; f += 2)
+        while (m % f == 0) { ax.radix[ax.npass++] = f; m /= f; }
+    if (m > 1) ax.radix[ax.npass++] = m;
+}
+
+template <typename R>
+static int make_axis(Ctx* c, int n, FftAxis& ax) {
+    factorize(n, ax);
+    std::vector<cplx<R>> tw(n);
+    for (int e = 0; e < n; ++e) {
+        // exp(-2 pi i e / n), argument reduced to the nearer quadrant
+        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
+        tw[e] = {(R)std::cos(a), (R)std::sin(a)};
+    }
+    void* d = nullptr;
+    PUFF_TRY(dalloc(c, &d, sizeof(cplx<R>) * n));
+    PUFF_HIP(hipMemcpy(d, tw.data(), sizeof(cplx<R>) * n, hipMemcpyHostToDevice));
+    ax.tw = d;
+    return PUFF_OK;
+}
+
+// ---- k-vectors and masks (host restatement) -----------------------------------
+
+static void kvalues(const puff_params& p, std::vector<double>& kx, std::vector<double>& ky,
+                    std::vector<double>& kz) {
+    const double pi = 4.0 * std::atan(1.0);   // GetKValues' own pi (transforms.f90:494)
+    const int nzp = p.periodic ? p.nz2 - 1 : p.nz2;
+    const double lx = (double)p.nx * p.dx, ly = (double)p.ny * p.dy, lz = (double)nzp * p.dz2;
+    const int mx = (int)std::ceil(p.nx / 2.0), my = (int)std::ceil(p.ny / 2.0), mz = (int)std::ceil(nzp / 2.0);
+    kx.resize(p.nx);
+    ky.resize(p.ny);
+    kz.resize(nzp);
+    for (int i = 0; i < p.nx; ++i) kx[i] = 2.0 * pi * (double)(i >= mx ? i - p.nx : i) / lx;
+    for (int i = 0; i < p.ny; ++i) ky[i] = 2.0 * pi * (double)(i >= my ? i - p.ny : i) / ly;
+    for (int i = 0; i < nzp; ++i) kz[i] = 2.0 * pi * (double)(i >= mz ? i - nzp : i) / lz;
+}
+
+static std::vector<double> linspace(double a, double b, int n) {
+    std::vector<double> v(n);
+    if (n > 1) {
+        const double d = (b - a) / (double)(n - 1);
+        for (int i = 0; i < n; ++i) v[i] = a + i * d;
+    } else if (n == 1) {
+        v[0] = a;
+    }
+    return v;
+}
+
+// Mask1D2 (masks.f90:266-302) when edge = true, Maskf4 (:329-361) otherwise.
+static std::vector<double> mask_1d(const std::vector<double>& x, double dx, int nb, double pi, bool edge) {
+    const int n = (int)x.size();
+    std::vector<double> m(n, 0.0);
+    const double lb = (nb - 1) * dx;
+    const double x01 = edge ? x[n - 1] : x[n - nb];
+    const double x02 = edge ? x[0] : x[nb - 1];
+    for (int i = n - nb; i < n; ++i) m[i] = std::pow(std::cos(pi * (x[i] - x01) / (2.0 * lb)), 2.0);
+    for (int i = 0; i < nb; ++i) m[i] = std::pow(std::cos(pi * (x[i] - x02) / (2.0 * lb)), 2.0);
+    for (int i = nb; i < n - nb; ++i) m[i] = edge ? 0.0 : 1.0;
+    return m;
+}
+
+static void masks(const puff_params& p, std::vector<double>& mxy, std::vector<double>& mz) {
+    const int nx = p.nx, ny = p.ny, nzp = p.periodic ? p.nz2 - 1 : p.nz2;
+    mxy.assign((size_t)nx * ny, 0.0);
+    mz.assign(nzp, 0.0);
+    if (nx > 1 && ny > 1 && p.nbx > 1 && p.nby > 1 && nx >= p.nbx && ny >= p.nby) {
+        // getMask / Mask2D: Maskf3_y Mask1D2_x + Maskf4_x Mask1D2_y, Maskf3 = 1
+        const double lx = (nx - 1) * p.dx, ly = (ny - 1) * p.dy;
+        auto x = linspace(-lx / 2, lx / 2, nx);
+        auto y = linspace(-ly / 2, ly / 2, ny);
+        auto mx = mask_1d(x, p.dx, p.nbx, p.pi_ref, true);
+        auto my = mask_1d(y, p.dy, p.nby, p.pi_ref, true);
+        auto mx2 = mask_1d(x, p.dx, p.nbx, p.pi_ref, false);
+        for (int iy = 0; iy < ny; ++iy)
+            for (int ix = 0; ix < nx; ++ix) mxy[(size_t)iy * nx + ix] = 1.0 * mx[ix] + mx2[ix] * my[iy];
+    }
+    if (!p.periodic && p.nbz2 > 1) {
+        // getZ2Mask with its Lf = (nx - 1*dx) expression (masks.f90:389-422)
+        const double dx = p.dz2;
+        auto x = linspace(0.0, (double)(nzp - 1) * dx, nzp);
+        const double lb = (p.nbz2 - 1) * dx;
+        const double lf = (p.nz2 - 1 * dx);
+        const double x0 = lf - lb / 2.0;
+        const int fbn = p.nz2 - p.nbz2 + 1;
+        for (int ind = 1; ind <= nzp; ++ind)
+            if (ind >= fbn) mz[ind - 1] = std::pow(std::cos(p.pi_ref * (x[ind - 1] - x0) / lb), 2.0);
+    }
+}
+
+// ---- per-stage scalars ------------------------------------------------------
+
+// adjUndPlace (puffin_equations.f90:218-254): 0 start, 1 main, 2 end, -1 error
+static int und_place(const puff_params& p, double z) {
+    if (!p.und_ends) return 1;
+    if (z < 0) return -1;
+    if (z <= p.szfs) return 0;
+    if (z >= p.szfe) return 2;
+    return 1;
+}
+
+// Coefficient map of the b-fields at zbar z (bfields.f90):
+//   elliptical / helical: bx = c0 + c3 y, by = c1 + c4 x, bz = c5 x + c6 y
+//   plane pole:           bx = c3 y,      by = c1 + c4 x, bz = c2 sinh(c7 y)
+//   curved pole:          bx = c0 sinh(c5 x) sinh(c6 y) + c3 y,
+//                         by = c1 cosh(c5 x) cosh(c6 y) + c4 x,
+//                         bz = c2 cosh(c5 x) sinh(c6 y)
+static int stage_scalars(const puff_params& p, double z, double n2col, StageScalars& s) {
+    for (double& v : s.c) v = 0.0;
+    s.n2col = n2col;
+    const int place = und_place(p, z);
+    if (place < 0) {
+        set_error("undulator section not recognised, zbar < 0");
+        return PUFF_OUT_Z2_MESH;
+    }
+    const double rho = p.rho, pi = p.pi_ref, se = std::sqrt(p.eta);
+    const double szt = z / 2.0 / rho;
+    int t = p.und_type;
+    double fx = p.fx, fy = p.fy;
+    if (t == PUFF_UND_HELICAL) {
+        fx = 1.0;
+        fy = 1.0;
+        t = PUFF_UND_ELLIPTICAL;
+    }
+    s.und_type = t;
+    if (t == PUFF_UND_ELLIPTICAL) {
+        if (place == 1) {                                   // bfields.f90:280-284, 496-500, 862-867
+            s.c[0] = fx * std::cos(szt);
+            s.c[1] = fy * std::sin(szt);
+            s.c[5] = -se / 2.0 / rho * fx * std::sin(szt);
+            s.c[6] = se / 2.0 / rho * fy * std::cos(szt);
+        } else if (place == 0) {                            // :239-257, 481-485, 763-787
+            double vx = fx * (z - pi * rho) / (6.0 * pi * rho) * std::cos(szt);
+            double czx = -se / 2.0 / rho * (z - pi * rho) / (6.0 * pi * rho) * fx * std::sin(szt);
+            if (z < pi * rho) {
+                vx = 0.0;
+                czx = 0.0;
+            } else if (z > 7.0 * pi * rho) {
+                vx = fx * std::cos(szt);
+                czx = -se / 2.0 / rho * fx * std::sin(szt);
+            }
+            s.c[0] = vx;
+            s.c[5] = czx;
+            s.c[1] = fy * szt / 4.0 / pi * std::sin(szt);
+            s.c[6] = szt / 4.0 / pi * se / 2.0 / rho * fy * std::cos(szt);
+        } else {                                            // :259-278, 487-494, 803-838
+            const double sl = z - p.szfe;
+            double vx = -fx * (sl - 7.0 * pi * rho) / (6.0 * pi * rho) * std::cos(sl / 2.0 / rho);
+            double czx = -se / 2.0 / rho * (sl - 7.0 * pi * rho) / (6.0 * pi * rho) * fx * std::sin(sl / 2.0 / rho);
+            if (sl < pi * rho) {
+                vx = fx * std::cos(sl / 2.0 / rho);
+                czx = -se / 2.0 / rho * fx * std::sin(sl / 2.0 / rho);
+            } else if (sl > 7.0 * pi * rho) {
+                vx = 0.0;
+                czx = 0.0;
+            }
+            const double s2 = sl / 2.0 / rho;
+            s.c[0] = vx;
+            s.c[5] = czx;
+            s.c[1] = fy * (-s2 / 4.0 / pi + 1.0) * std::sin(s2);
+            s.c[6] = (-sl / 8.0 / pi / rho + 1.0) * se / 2.0 / rho * fy * std::cos(sl / 2.0 / rho);
+        }
+    } else if (t == PUFF_UND_PLANEPOLE) {
+        s.c[7] = se / 2.0 / rho;
+        if (place == 1) {                                   // :409-415, 630-635
+            s.c[1] = std::sin(szt);
+            s.c[2] = std::cos(szt);
+        } else if (place == 0) {                            // :380-391, 613-618
+            s.c[1] = szt / 4.0 / pi * std::sin(szt);
+            s.c[2] = (szt / 4.0 / pi) * std::cos(szt);
+        } else {                                            // :395-407, 620-628
+            const double sl = (z - p.szfe) / 2.0 / rho;
+            s.c[1] = (-sl / 4.0 / pi + 1.0) * std::sin(sl);
+            s.c[2] = (-sl / 4.0 / pi + 1.0) * std::cos(sl);
+        }
+    } else {  // curved pole
+        const double kx = p.kx_und, ky = p.ky_und;
+        s.c[5] = kx;
+        s.c[6] = ky;
+        if (place == 1) {                                   // :103-109, 353-359, 588-595
+            s.c[0] = kx / ky * std::sin(szt);
+            s.c[1] = std::sin(szt);
+            s.c[2] = se / 2.0 / rho / kx * std::cos(szt);
+        } else if (place == 0) {                            // :84-90, 334-340, 568-575
+            const double f = szt / 4.0 / pi;
+            s.c[0] = kx / ky * f * std::sin(szt);
+            s.c[1] = f * std::sin(szt);
+            s.c[2] = se / 2.0 / rho / ky * f * std::cos(szt);
+        } else {                                            // :92-101, 342-351, 577-586
+            const double sl = (z - p.szfe) / 2.0 / rho;
+            const double f = (-sl / 4.0 / pi + 1.0);
+            s.c[0] = kx / ky * f * std::sin(sl);
+            s.c[1] = f * std::sin(sl);
+            s.c[2] = se / 2.0 / rho / kx * f * std::cos(sl);
+        }
+    }
+    if (p.focusing) {                                       // :299-306, 514-521
+        s.c[3] = se * p.kbeta_y * p.kbeta_y / p.kappa;
+        s.c[4] = -se * p.kbeta_x * p.kbeta_x / p.kappa;
+    }
+    if (p.one_d) {                                          // bz = 0 in 1-D (:544-551)
+        if (t == PUFF_UND_ELLIPTICAL) {
+            s.c[5] = 0.0;
+            s.c[6] = 0.0;
+        } else {
+            s.c[2] = 0.0;
+        }
+    }
+    return PUFF_OK;
+}
+
+// getAlpha with its side effect (wiggler_taper.f90:39-45), or the per-period
+// table when one is set.
+static double advance_taper(Ctx* c, double z) {
+    if (!c->taper_table.empty()) {
+        long long k = (long long)std::floor((z - c->taper_z0) / c->taper_period);
+        if (k < 0) k = 0;
+        if (k >= (long long)c->taper_table.size()) k = (long long)c->taper_table.size() - 1;
+        c->n2col = c->taper_table[(size_t)k];
+        return c->n2col;
+    }
+    if (z >= c->p.szfs && z <= c->p.szfe) {
+        c->n2col = c->n2col0 + c->undgrad * (z - c->p.szfs);
+    } else if (z > c->p.szfe) {
+        c->n2col0 = c->n2col;
+    }
+    return c->n2col;
+}
+
+// ---- context creation -------------------------------------------------------
+
+template <typename R>
+static int alloc_particles(Ctx* c) {
+    Particles<R>& P = parts<R>(c);
+    auto setp = [&](PartPtrs<R>& q) -> int {
+        PUFF_TRY(dalloc(c, (void**)&q.x, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.y, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.z2, sizeof(double) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.pr, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.pi, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.g, sizeof(double) * c->n));
+        return PUFF_OK;
+    };
+    PUFF_TRY(setp(P.y));
+    for (int k = 0; k < 3; ++k) PUFF_TRY(setp(P.k[k]));
+    PUFF_TRY(dalloc(c, (void**)&P.chi, sizeof(R) * c->n));
+    const int64_t nm = c->nmesh();
+    PUFF_TRY(dalloc(c, &c->A, sizeof(cplx<R>) * nm));
+    for (int k = 0; k < 4; ++k) PUFF_TRY(dalloc(c, &c->dA[k], sizeof(cplx<R>) * nm));
+    if (!c->p.one_d) {
+        PUFF_TRY(dalloc(c, &c->anb, sizeof(cplx<R>) * nm));
+        PUFF_TRY(make_axis<R>(c, c->p.nx, c->ax));
+        PUFF_TRY(make_axis<R>(c, c->p.ny, c->ay));
+        PUFF_TRY(make_axis<R>(c, c->nzp(), c->az));
+    }
+    return PUFF_OK;
+}
+
+static int validate(const puff_params* p, int64_t n) {
+    if (!p) return PUFF_ERR_ARG;
+    if (n < 0 || p->nx < 1 || p->ny < 1 || p->nz2 < 3) {
+        set_error("invalid sizes");
+        return PUFF_ERR_ARG;
+    }
+    if (p->one_d && (p->nx != 1 || p->ny != 1)) {
+        set_error("1-D mesh must have nx = ny = 1");
+        return PUFF_ERR_ARG;
+    }
+    if (!p->one_d && (p->nx < 2 || p->ny < 2)) {
+        set_error("3-D mesh needs nx, ny >= 2");
+        return PUFF_ERR_ARG;
+    }
+    if (!(p->dx > 0 && p->dy > 0 && p->dz2 > 0 && p->eta > 0 && p->rho > 0 && p->kappa > 0 &&
+          p->gamma_r > 0)) {
+        set_error("invalid physical constants");
+        return PUFF_ERR_ARG;
+    }
+    if ((int64_t)p->nx * p->ny * p->nz2 >= (int64_t)1 << 31) {
+        set_error("mesh too large for 32-bit node indices");
+        return PUFF_ERR_ARG;
+    }
+    return PUFF_OK;
+}
+
+static int create(Ctx* c) {
+    const puff_params& p = c->p;
+    MeshGeom& m = c->mesh;
+    m.nx = p.nx;
+    m.ny = p.ny;
+    m.nz2 = p.nz2;
+    m.nzp = c->nzp();
+    m.periodic = p.periodic;
+    m.one_d = p.one_d;
+    m.dx = p.dx;
+    m.dy = p.dy;
+    m.dz2 = p.dz2;
+    m.inv_dx = 1.0 / p.dx;
+    m.inv_dy = 1.0 / p.dy;
+    m.inv_dz2 = 1.0 / p.dz2;
+    m.halfx = ((p.nx - 1) / 2.0) * p.dx;   // rhs.f90:395-396
+    m.halfy = ((p.ny - 1) / 2.0) * p.dy;
+    Physics& ph = c->phys;
+    ph.eta = p.eta;
+    ph.rho = p.rho;
+    ph.kappa = p.kappa;
+    ph.gamma_r = p.gamma_r;
+    ph.aw = p.aw;
+    ph.inv2rho = 1.0 / (2.0 * p.rho);
+    ph.inv_dV3 = 1.0 / (p.dx * p.dy * p.dz2);
+    ph.c_dx = 2 * p.rho * p.kappa / std::sqrt(p.eta);
+    ph.coupling = p.coupling;
+    ph.electrons_evolve = p.electrons_evolve;
+    ph.field_evolve = p.field_evolve;
+
+    if (c->precision == PUFF_FP64) PUFF_TRY(alloc_particles<double>(c));
+    else PUFF_TRY(alloc_particles<float>(c));
+
+    PUFF_TRY(dalloc(c, (void**)&c->flags, sizeof(int)));
+    PUFF_TRY(dalloc(c, (void**)&c->box, sizeof(Box)));
+    PUFF_HIP(hipHostMalloc((void**)&c->h_flags, sizeof(int), hipHostMallocDefault));
+    c->stage_len = 2 * std::max<int64_t>(c->n, c->nmesh());
+    PUFF_TRY(dalloc(c, (void**)&c->stage_buf, sizeof(double) * c->stage_len));
+    c->diag_len = 3 * 4096 + c->p.nz2;
+    PUFF_TRY(dalloc(c, (void**)&c->diag, sizeof(double) * c->diag_len));
+    if (!p.one_d) {
+        std::vector<double> kx, ky, kz, mxy, mz;
+        kvalues(p, kx, ky, kz);
+        masks(p, mxy, mz);
+        PUFF_TRY(dalloc(c, (void**)&c->kx, sizeof(double) * kx.size()));
+        PUFF_TRY(dalloc(c, (void**)&c->ky, sizeof(double) * ky.size()));
+        PUFF_TRY(dalloc(c, (void**)&c->kz, sizeof(double) * kz.size()));
+        PUFF_TRY(dalloc(c, (void**)&c->mxy, sizeof(double) * mxy.size()));
+        PUFF_TRY(dalloc(c, (void**)&c->mz, sizeof(double) * mz.size()));
+        PUFF_HIP(hipMemcpy(c->kx, kx.data(), sizeof(double) * kx.size(), hipMemcpyHostToDevice));
+        PUFF_HIP(hipMemcpy(c->ky, ky.data(), sizeof(double) * ky.size(), hipMemcpyHostToDevice));
+        PUFF_HIP(hipMemcpy(c->kz, kz.data(), sizeof(double) * kz.size(), hipMemcpyHostToDevice));
+        PUFF_HIP(hipMemcpy(c->mxy, mxy.data(), sizeof(double) * mxy.size(), hipMemcpyHostToDevice));
+        PUFF_HIP(hipMemcpy(c->mz, mz.data(), sizeof(double) * mz.size(), hipMemcpyHostToDevice));
+    }
+    return PUFF_OK;
+}
+
+// ---- kernel timing ------------------------------------------------------------
+
+static hipEvent_t take_event(Ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+// Records the start event of a timed launch; returns its slot or -1.
+static int prof_start(Ctx* c, int kind) {
+    if (!c->prof) return -1;
+    hipEvent_t a = take_event(c), b = take_event(c);
+    if (!a || !b) return -1;
+    (void)hipEventRecord(a, c->stream);
+    c->pending.push_back({kind, a, b});
+    return (int)c->pending.size() - 1;
+}
+
+static void prof_stop(Ctx* c, int slot) {
+    if (slot < 0) return;
+    (void)hipEventRecord(c->pending[(size_t)slot].b, c->stream);
+}
+
+// After a stream synchronisation: accumulate the recorded durations.
+static void prof_collect(Ctx* c) {
+    for (const auto& p : c->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            c->prof_ms[p.kind] += ms;
+            c->prof_n[p.kind] += 1;
+        }
+    }
+    c->pending.clear();
+    c->ev_used = 0;
+}
+
+static void release(Ctx* c) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    c->allocs.clear();
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
+    if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// ---- the integrator step ----------------------------------------------------
+
+template <typename R>
+static int rk4par(Ctx* c, double sz, double h) {
+    Particles<R>& P = parts<R>(c);
+    hipStream_t s = c->stream;
+    const double hh = h * 0.5, h6 = h / 6.0;   // puffin_mpi_RK4.f90:89-91
+    const double zs[4] = {sz, sz + hh, sz + hh, sz + h};
+    const double saved[3] = {c->n2col, c->n2col0, c->undgrad};
+    StageScalars sc[4];
+    for (int k = 0; k < 4; ++k) {
+        const double n2 = advance_taper(c, zs[k]);
+        const int st = stage_scalars(c->p, zs[k], n2, sc[k]);
+        if (st != PUFF_OK) {
+            c->n2col = saved[0];
+            c->n2col0 = saved[1];
+            return st;
+        }
+    }
+    int slot = prof_start(c, 6);
+    launch_step_begin(c->flags, c->box, s);
+    prof_stop(c, slot);
+    StageArgs<R> a{};
+    a.n = c->n;
+    a.y0 = P.y;
+    a.chi = P.chi;
+    a.h6 = h6;
+    a.A0 = reinterpret_cast<const cplx<R>*>(c->A);
+    a.mesh = c->mesh;
+    a.phys = c->phys;
+    a.flags = c->flags;
+    a.box = c->box;
+    const double cin[4] = {0.0, hh, hh, h};
+    const double cnext[4] = {hh, hh, h, 0.0};
+    for (int st = 1; st <= 4; ++st) {
+        a.s = sc[st - 1];
+        a.cin = cin[st - 1];
+        a.cnext = cnext[st - 1];
+        a.kin = (st > 1) ? P.k[st - 2] : PartPtrs<R>{};
+        a.kout = (st < 4) ? P.k[st - 1] : PartPtrs<R>{};
+        if (st == 4) {
+            a.k1 = P.k[0];
+            a.k2 = P.k[1];
+        }
+        a.dAin = (st > 1) ? reinterpret_cast<const cplx<R>*>(c->dA[st - 2]) : nullptr;
+        a.dAout = reinterpret_cast<cplx<R>*>(c->dA[st - 1]);
+        slot = prof_start(c, st - 1);
+        launch_stage<R>(st, a, s);
+        prof_stop(c, slot);
+    }
+    CombineArgs<R> ca{};
+    ca.A = reinterpret_cast<cplx<R>*>(c->A);
+    for (int k = 0; k < 4; ++k) ca.d[k] = reinterpret_cast<cplx<R>*>(c->dA[k]);
+    ca.h6 = h6;
+    ca.box = c->box;
+    ca.flags = c->flags;
+    ca.nx = c->p.nx;
+    ca.ny = c->p.ny;
+    ca.nz2 = c->p.nz2;
+    ca.nzp = c->nzp();
+    ca.periodic = c->p.periodic;
+    ca.one_d = c->p.one_d;
+    slot = prof_start(c, 4);
+    launch_combine<R>(ca, s);
+    prof_stop(c, slot);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    PUFF_HIP(hipStreamSynchronize(s));
+    prof_collect(c);
+    const int f = *c->h_flags;
+    if (f) {
+        c->n2col = saved[0];
+        c->n2col0 = saved[1];
+        if (f & FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
+        if (f & FLAG_TRANSVERSE) return PUFF_OUT_TRANSVERSE;
+        return PUFF_OUT_Z2_BUFFER;
+    }
+    return PUFF_OK;
+}
+
+template <typename R>
+static int diffraction(Ctx* c, double h) {
+    if (c->p.one_d) {
+        set_error("no diffraction on a 1-D mesh (read_base_input.f90:389)");
+        return PUFF_ERR_STATE;
+    }
+    DiffArgs<R> a{};
+    a.a = reinterpret_cast<cplx<R>*>(c->A);
+    a.anb = reinterpret_cast<cplx<R>*>(c->anb);
+    a.kx = c->kx;
+    a.ky = c->ky;
+    a.kz = c->kz;
+    a.mxy = c->mxy;
+    a.mz = c->mz;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    a.nz2 = c->p.nz2;
+    a.nzp = c->nzp();
+    a.periodic = c->p.periodic;
+    a.filter = c->p.filter;
+    a.h = h;
+    a.beta = c->p.beta;
+    a.cutoff = c->p.filt_cutoff;
+    a.ffact = (double)c->p.nx * (double)c->p.ny * (double)a.nzp;   // setup.f90:334-346
+    a.ax = c->ax;
+    a.ay = c->ay;
+    a.az = c->az;
+    const int slot = prof_start(c, 5);
+    launch_diffraction<R>(a, c->stream);
+    prof_stop(c, slot);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    return PUFF_OK;
+}
+
+// ---- data exchange ------------------------------------------------------------
+
+template <typename R>
+static int upload_coord(Ctx* c, int kind, const double* src, R* dst) {
+    if (!src) return PUFF_OK;
+    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (sizeof(R) == sizeof(double)) {
+        PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
+    } else {
+        PUFF_HIP(hipMemcpyAsync(c->stage_buf, src, sizeof(double) * c->n, k, c->stream));
+        launch_convert<R>(c->stage_buf, dst, c->n, c->stream);
+        PUFF_HIP(hipGetLastError());
+    }
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int download_coord(Ctx* c, int kind, const R* src, double* dst) {
+    if (!dst) return PUFF_OK;
+    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (sizeof(R) == sizeof(double)) {
+        PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
+    } else {
+        launch_convert_back<R>(src, c->stage_buf, c->n, c->stream);
+        PUFF_HIP(hipGetLastError());
+        PUFF_HIP(hipMemcpyAsync(dst, c->stage_buf, sizeof(double) * c->n, k, c->stream));
+    }
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+static int upload_f64(Ctx* c, int kind, const double* src, double* dst) {
+    if (!src) return PUFF_OK;
+    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+static int download_f64(Ctx* c, int kind, const double* src, double* dst) {
+    if (!dst) return PUFF_OK;
+    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int upload_particles(Ctx* c, int kind, const double* x, const double* y, const double* z2,
+                            const double* px, const double* py, const double* g, const double* chi) {
+    Particles<R>& P = parts<R>(c);
+    PUFF_TRY(upload_coord<R>(c, kind, x, P.y.x));
+    PUFF_TRY(upload_coord<R>(c, kind, y, P.y.y));
+    PUFF_TRY(upload_f64(c, kind, z2, P.y.z2));
+    PUFF_TRY(upload_coord<R>(c, kind, px, P.y.pr));
+    PUFF_TRY(upload_coord<R>(c, kind, py, P.y.pi));
+    PUFF_TRY(upload_f64(c, kind, g, P.y.g));
+    PUFF_TRY(upload_coord<R>(c, kind, chi, P.chi));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int download_particles(Ctx* c, int kind, double* x, double* y, double* z2, double* px,
+                              double* py, double* g) {
+    Particles<R>& P = parts<R>(c);
+    PUFF_TRY(download_coord<R>(c, kind, P.y.x, x));
+    PUFF_TRY(download_coord<R>(c, kind, P.y.y, y));
+    PUFF_TRY(download_f64(c, kind, P.y.z2, z2));
+    PUFF_TRY(download_coord<R>(c, kind, P.y.pr, px));
+    PUFF_TRY(download_coord<R>(c, kind, P.y.pi, py));
+    PUFF_TRY(download_f64(c, kind, P.y.g, g));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int upload_field(Ctx* c, int kind, const double* ar, const double* ai) {
+    const int64_t nm = c->nmesh();
+    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    PUFF_HIP(hipMemcpyAsync(c->stage_buf, ar, sizeof(double) * nm, k, c->stream));
+    PUFF_HIP(hipMemcpyAsync(c->stage_buf + nm, ai, sizeof(double) * nm, k, c->stream));
+    launch_pack_field<R>(c->stage_buf, c->stage_buf + nm, reinterpret_cast<cplx<R>*>(c->A), nm, c->stream);
+    PUFF_HIP(hipGetLastError());
+    if (c->p.periodic) {
+        // the last plane of a periodic mesh aliases plane 1 (para_field.f90:1075-1138)
+        const int64_t ntr = (int64_t)c->p.nx * c->p.ny;
+        cplx<R>* A = reinterpret_cast<cplx<R>*>(c->A);
+        PUFF_HIP(hipMemcpyAsync(A + (int64_t)(c->p.nz2 - 1) * ntr, A, sizeof(cplx<R>) * ntr,
+                                hipMemcpyDeviceToDevice, c->stream));
+    }
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int download_field(Ctx* c, int kind, double* ar, double* ai) {
+    const int64_t nm = c->nmesh();
+    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    launch_unpack_field<R>(reinterpret_cast<const cplx<R>*>(c->A), c->stage_buf, c->stage_buf + nm, nm, c->stream);
+    PUFF_HIP(hipGetLastError());
+    if (ar) PUFF_HIP(hipMemcpyAsync(ar, c->stage_buf, sizeof(double) * nm, k, c->stream));
+    if (ai) PUFF_HIP(hipMemcpyAsync(ai, c->stage_buf + nm, sizeof(double) * nm, k, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int transform(Ctx* c, int sign) {
+    if (c->p.one_d) return PUFF_ERR_STATE;
+    launch_fft3d<R>(reinterpret_cast<cplx<R>*>(c->A), c->p.nx, c->p.ny, c->nzp(), c->ax, c->ay, c->az, sign,
+                    c->stream);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int power(Ctx* c, double* out) {
+    launch_power<R>(reinterpret_cast<const cplx<R>*>(c->A), c->p.nx, c->p.ny, c->p.nz2, c->p.dx, c->p.dy,
+                    c->diag, c->stream);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipMemcpyAsync(out, c->diag, sizeof(double) * c->p.nz2, hipMemcpyDeviceToHost, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int bunching(Ctx* c, int harmonic, int nslices, double slice, double* out) {
+    if (nslices < 1 || nslices > 2048) {
+        set_error("nslices must be in [1, 2048]");
+        return PUFF_ERR_ARG;
+    }
+    Particles<R>& P = parts<R>(c);
+    launch_fill_zero(c->diag, 3 * (int64_t)nslices, c->stream);
+    launch_bunching<R>(P.y.z2, P.chi, c->n, c->p.rho, harmonic, nslices, slice, c->p.periodic, c->diag,
+                       c->stream);
+    PUFF_HIP(hipGetLastError());
+    std::vector<double> acc(3 * (size_t)nslices);
+    PUFF_HIP(hipMemcpyAsync(acc.data(), c->diag, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    for (int is = 0; is < nslices; ++is) {
+        const double br = acc[is], bi = acc[nslices + is], sd = acc[2 * nslices + is];
+        out[is] = (sd > 0.0) ? std::sqrt(br * br + bi * bi) / slice : 0.0;   // av_write.f90:725-749
+    }
+    return PUFF_OK;
+}
+
+}  // namespace puff
+
+// ============================================================================
+// C ABI
+// ============================================================================
+using puff::Ctx;
+
+struct puff_ctx {
+    Ctx c;
+};
+
+#define CTX_CHECK(ctx)                                   \
+    do {                                                 \
+        if (!(ctx)) {                                    \
+            puff::set_error("null context");             \
+            return PUFF_ERR_ARG;                         \
+        }                                                \
+        hipError_t e_ = hipSetDevice((ctx)->c.device);   \
+        if (e_ != hipSuccess) {                          \
+            puff::set_error(hipGetErrorString(e_));      \
+            return PUFF_ERR_HIP;                         \
+        }                                                \
+    } while (0)
+
+#define DISPATCH(ctx, fn, ...)                                                      \
+    ((ctx)->c.precision == PUFF_FP64 ? puff::fn<double>(&(ctx)->c, __VA_ARGS__)     \
+                                     : puff::fn<float>(&(ctx)->c, __VA_ARGS__))
+
+extern "C" {
+
+const char* puff_last_error(void) { return puff::g_err.c_str(); }
+
+int puff_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int puff_create(puff_ctx** out, const puff_params* params, int64_t n, int32_t precision) {
+    if (!out) return PUFF_ERR_ARG;
+    *out = nullptr;
+    int st = puff::validate(params, n);
+    if (st != PUFF_OK) return st;
+    if (precision != PUFF_FP64 && precision != PUFF_MIXED) {
+        puff::set_error("unknown precision");
+        return PUFF_ERR_ARG;
+    }
+    puff_ctx* ctx = new (std::nothrow) puff_ctx();
+    if (!ctx) return PUFF_ERR_ALLOC;
+    Ctx& c = ctx->c;
+    c.p = *params;
+    c.n = n;
+    c.precision = precision;
+    if (hipGetDevice(&c.device) != hipSuccess) {
+        puff::set_error("no HIP device");
+        delete ctx;
+        return PUFF_ERR_HIP;
+    }
+    if (hipStreamCreate(&c.stream) != hipSuccess) {
+        puff::set_error("hipStreamCreate failed");
+        delete ctx;
+        return PUFF_ERR_HIP;
+    }
+    c.own_stream = true;
+    st = puff::create(&c);
+    if (st != PUFF_OK) {
+        puff::release(&c);
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return PUFF_OK;
+}
+
+int puff_destroy(puff_ctx* ctx) {
+    if (!ctx) return PUFF_OK;
+    (void)hipSetDevice(ctx->c.device);
+    puff::release(&ctx->c);
+    delete ctx;
+    return PUFF_OK;
+}
+
+int puff_set_stream(puff_ctx* ctx, void* stream) {
+    CTX_CHECK(ctx);
+    Ctx& c = ctx->c;
+    (void)hipStreamSynchronize(c.stream);
+    if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
+    if (stream) {
+        c.stream = (hipStream_t)stream;
+        c.own_stream = false;
+    } else {
+        if (hipStreamCreate(&c.stream) != hipSuccess) return PUFF_ERR_HIP;
+        c.own_stream = true;
+    }
+    return PUFF_OK;
+}
+
+int puff_synchronize(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    return hipStreamSynchronize(ctx->c.stream) == hipSuccess ? PUFF_OK : PUFF_ERR_HIP;
+}
+
+int puff_upload_particles(puff_ctx* ctx, int32_t kind, const double* x, const double* y,
+                          const double* z2, const double* px, const double* py, const double* gam,
+                          const double* chi) {
+    CTX_CHECK(ctx);
+    return DISPATCH(ctx, upload_particles, kind, x, y, z2, px, py, gam, chi);
+}
+
+int puff_download_particles(puff_ctx* ctx, int32_t kind, double* x, double* y, double* z2,
+                            double* px, double* py, double* gam) {
+    CTX_CHECK(ctx);
+    return DISPATCH(ctx, download_particles, kind, x, y, z2, px, py, gam);
+}
+
+int puff_upload_field(puff_ctx* ctx, int32_t kind, const double* ar, const double* ai) {
+    CTX_CHECK(ctx);
+    if (!ar || !ai) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, upload_field, kind, ar, ai);
+}
+
+int puff_download_field(puff_ctx* ctx, int32_t kind, double* ar, double* ai) {
+    CTX_CHECK(ctx);
+    return DISPATCH(ctx, download_field, kind, ar, ai);
+}
+
+int puff_set_taper(puff_ctx* ctx, double n2col, double n2col0, double undgrad) {
+    if (!ctx) return PUFF_ERR_ARG;
+    ctx->c.n2col = n2col;
+    ctx->c.n2col0 = n2col0;
+    ctx->c.undgrad = undgrad;
+    return PUFF_OK;
+}
+
+int puff_get_taper(puff_ctx* ctx, double* n2col, double* n2col0) {
+    if (!ctx) return PUFF_ERR_ARG;
+    if (n2col) *n2col = ctx->c.n2col;
+    if (n2col0) *n2col0 = ctx->c.n2col0;
+    return PUFF_OK;
+}
+
+int puff_set_taper_table(puff_ctx* ctx, const double* values, int32_t nvals, double z0, double period) {
+    if (!ctx || nvals < 0 || (nvals > 0 && (!values || !(period > 0)))) return PUFF_ERR_ARG;
+    ctx->c.taper_table.assign(values, values + nvals);
+    ctx->c.taper_z0 = z0;
+    ctx->c.taper_period = period;
+    return PUFF_OK;
+}
+
+int puff_rk4par(puff_ctx* ctx, double sz, double h) {
+    CTX_CHECK(ctx);
+    return ctx->c.precision == PUFF_FP64 ? puff::rk4par<double>(&ctx->c, sz, h)
+                                         : puff::rk4par<float>(&ctx->c, sz, h);
+}
+
+int puff_diffraction_step(puff_ctx* ctx, double h) {
+    CTX_CHECK(ctx);
+    return ctx->c.precision == PUFF_FP64 ? puff::diffraction<double>(&ctx->c, h)
+                                         : puff::diffraction<float>(&ctx->c, h);
+}
+
+int puff_transform(puff_ctx* ctx, int32_t sign) {
+    CTX_CHECK(ctx);
+    if (sign != -1 && sign != 1) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, transform, sign);
+}
+
+int puff_wrap_z2(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    Ctx& c = ctx->c;
+    if (!c.p.periodic) return PUFF_OK;
+    double* z2 = c.precision == PUFF_FP64 ? c.pd.y.z2 : c.pf.y.z2;
+    puff::launch_wrap(z2, c.n, c.p.dz2 * (double)(c.p.nz2 - 1), c.stream);
+    if (hipGetLastError() != hipSuccess) return PUFF_ERR_HIP;
+    return hipStreamSynchronize(c.stream) == hipSuccess ? PUFF_OK : PUFF_ERR_HIP;
+}
+
+int puff_power(puff_ctx* ctx, double* out) {
+    CTX_CHECK(ctx);
+    if (!out) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, power, out);
+}
+
+int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices, double slice_len, double* out) {
+    CTX_CHECK(ctx);
+    if (!out || !(slice_len > 0)) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, bunching, harmonic, nslices, slice_len, out);
+}
+
+int puff_profile_enable(puff_ctx* ctx, int32_t on) {
+    CTX_CHECK(ctx);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    puff::prof_collect(&ctx->c);
+    ctx->c.prof = on != 0;
+    return PUFF_OK;
+}
+
+int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds, int32_t reset) {
+    CTX_CHECK(ctx);
+    if (nkinds < 0 || nkinds > PUFF_PROF_KINDS) return PUFF_ERR_ARG;
+    for (int k = 0; k < nkinds; ++k) {
+        if (ms) ms[k] = ctx->c.prof_ms[k];
+        if (counts) counts[k] = ctx->c.prof_n[k];
+    }
+    if (reset) {
+        for (int k = 0; k < PUFF_PROF_KINDS; ++k) {
+            ctx->c.prof_ms[k] = 0.0;
+            ctx->c.prof_n[k] = 0;
+        }
+    }
+    return PUFF_OK;
+}
+
+int puff_active_box(puff_ctx* ctx, int32_t* box6) {
+    CTX_CHECK(ctx);
+    if (!box6) return PUFF_ERR_ARG;
+    puff::Box b;
+    if (hipMemcpy(&b, ctx->c.box, sizeof(b), hipMemcpyDeviceToHost) != hipSuccess) return PUFF_ERR_HIP;
+    for (int d = 0; d < 3; ++d) {
+        box6[2 * d] = b.lo[d];
+        box6[2 * d + 1] = b.hi[d];
+    }
+    return PUFF_OK;
+}
+
+int puff_particle_count(puff_ctx* ctx, int64_t* n) {
+    if (!ctx || !n) return PUFF_ERR_ARG;
+    *n = ctx->c.n;
+    return PUFF_OK;
+}
+
+}  // extern "C"

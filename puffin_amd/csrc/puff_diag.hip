@@ -1,0 +1,167 @@
+// Diagnostics (power, slice bunching) and layout conversion kernels.
+//
+//   power    gPowerP / fPower_3D / fPower_1D, io/av_write.f90:39-202
+//   bunching getSliceTwiss, io/av_write.f90:405-773 (harmonic sums 562-650,
+//            magnitudes 725-729)
+#include "puff_common.hpp"
+#include "puff_kernels.hpp"
+
+namespace puff {
+
+// Power of one z2 plane: trapezoidal integral of |A|^2 over x then y
+// (m_trapz2D, av_write.f90:217-238) on a uniform axis; |A|^2 in 1-D.
+// Accumulated in FP64 (the reference's m_trapz is a default-real function).
+template <typename R>
+__global__ __launch_bounds__(256) void k_power(const cplx<R>* a, int nx, int ny, double dx,
+                                               double dy, double* out) {
+    const int iz = blockIdx.x;
+    const int64_t ntr = (int64_t)nx * ny;
+    const cplx<R>* p = a + (int64_t)iz * ntr;
+    __shared__ double red[4];
+    double s = 0.0;
+    if (nx == 1 && ny == 1) {
+        if (threadIdx.x == 0) {
+            const double r = (double)p[0].x, i = (double)p[0].y;
+            out[iz] = r * r + i * i;
+        }
+        return;
+    }
+    for (int64_t t = threadIdx.x; t < ntr; t += blockDim.x) {
+        const int ix = (int)(t % nx), iy = (int)(t / nx);
+        const double r = (double)p[t].x, i = (double)p[t].y;
+        double w = 1.0;
+        if (ix == 0 || ix == nx - 1) w *= 0.5;
+        if (iy == 0 || iy == ny - 1) w *= 0.5;
+        s += w * (r * r + i * i);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[iz] = (red[0] + red[1] + red[2] + red[3]) * dx * dy;
+}
+
+// Per-slice sums sum chi cos(h z2 / 2 rho) w, sum chi sin(...) w and the
+// slice charge, with the linear split between slices is and is+1 on a
+// temporal mesh (av_write.f90:505-530, 562-572).  acc = [br | bi | sdata].
+template <typename R>
+__global__ __launch_bounds__(256) void k_bunching(const double* z2, const R* chi, int64_t n,
+                                                  double rho, int harmonic, int nslices,
+                                                  double slice, int periodic, double* acc) {
+    extern __shared__ double sh[];
+    for (int t = threadIdx.x; t < 3 * nslices; t += blockDim.x) sh[t] = 0.0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double z = z2[i];
+        int is = (int)ceil(z / slice);
+        if (periodic) {
+            if (is >= nslices) is = is - ((int)floor((double)is / (double)nslices) * nslices) + 1;
+            if (is < 1) continue;
+        } else {
+            if (is > nslices || is < 1) continue;
+        }
+        const double c = (double)chi[i];
+        double sn, cs;
+        sincos((double)harmonic * z / (2.0 * rho), &sn, &cs);
+        const double w1 = ((is * slice - z) / slice);
+        atomicAdd(&sh[is - 1], c * cs * w1);
+        atomicAdd(&sh[nslices + is - 1], c * sn * w1);
+        atomicAdd(&sh[2 * nslices + is - 1], c);
+        if (!periodic && is < nslices) {
+            const double w2 = (-((is - 1) * slice - z) / slice);
+            atomicAdd(&sh[is], c * cs * w2);
+            atomicAdd(&sh[nslices + is], c * sn * w2);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * nslices; t += blockDim.x)
+        if (sh[t] != 0.0) atomicAdd(&acc[t], sh[t]);
+}
+
+template <typename R>
+__global__ void k_pack(const double* ar, const double* ai, cplx<R>* a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = {(R)ar[i], (R)ai[i]};
+}
+
+template <typename R>
+__global__ void k_unpack(const cplx<R>* a, double* ar, double* ai, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ar[i] = (double)a[i].x;
+        ai[i] = (double)a[i].y;
+    }
+}
+
+template <typename R>
+__global__ void k_convert(const double* s, R* d, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (R)s[i];
+}
+
+template <typename R>
+__global__ void k_convert_back(const R* s, double* d, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (double)s[i];
+}
+
+__global__ void k_fill_zero(double* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+static dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); }
+
+void launch_fill_zero(double* p, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_zero, grid_for(n), dim3(256), 0, s, p, n);
+}
+
+template <typename R>
+void launch_power(const cplx<R>* a, int nx, int ny, int nz2, double dx, double dy, double* out,
+                  hipStream_t s) {
+    hipLaunchKernelGGL((k_power<R>), dim3(nz2), dim3(256), 0, s, a, nx, ny, dx, dy, out);
+}
+
+template <typename R>
+void launch_bunching(const double* z2, const R* chi, int64_t n, double rho, int harmonic,
+                     int nslices, double slice, int periodic, double* acc, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL((k_bunching<R>), dim3((unsigned)blocks), dim3(256),
+                       3 * (size_t)nslices * sizeof(double), s, z2, chi, n, rho, harmonic, nslices,
+                       slice, periodic, acc);
+}
+
+template <typename R>
+void launch_pack_field(const double* ar, const double* ai, cplx<R>* a, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_pack<R>), grid_for(n), dim3(256), 0, s, ar, ai, a, n);
+}
+template <typename R>
+void launch_unpack_field(const cplx<R>* a, double* ar, double* ai, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_unpack<R>), grid_for(n), dim3(256), 0, s, a, ar, ai, n);
+}
+template <typename R>
+void launch_convert(const double* src, R* dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert<R>), grid_for(n), dim3(256), 0, s, src, dst, n);
+}
+template <typename R>
+void launch_convert_back(const R* src, double* dst, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL((k_convert_back<R>), grid_for(n), dim3(256), 0, s, src, dst, n);
+}
+
+#define PUFF_INST(R)                                                                              \
+    template void launch_power<R>(const cplx<R>*, int, int, int, double, double, double*,          \
+                                  hipStream_t);                                                   \
+    template void launch_bunching<R>(const double*, const R*, int64_t, double, int, int, double,  \
+                                     int, double*, hipStream_t);                                  \
+    template void launch_pack_field<R>(const double*, const double*, cplx<R>*, int64_t,           \
+                                       hipStream_t);                                              \
+    template void launch_unpack_field<R>(const cplx<R>*, double*, double*, int64_t, hipStream_t); \
+    template void launch_convert<R>(const double*, R*, int64_t, hipStream_t);                     \
+    template void launch_convert_back<R>(const R*, double*, int64_t, hipStream_t);
+PUFF_INST(double)
+PUFF_INST(float)
+#undef PUFF_INST
+
+}  // namespace puff

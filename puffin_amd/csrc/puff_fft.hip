@@ -1,0 +1,332 @@
+// Batched mixed-radix complex FFT along one axis of the field mesh, and the
+// diffraction step built from it.
+//
+// Convention of the reference's FFTW plans (transforms.f90:186-211, 423):
+// sign -1 forward, +1 backward, unnormalised; the backward result is divided
+// by ffact = NX NY NZ2' afterwards (diffraction.f90:310, 513).
+//
+// Each workgroup holds B whole lines in LDS and runs a Stockham autosort
+// transform (natural order in and out), one radix pass per LDS round trip.
+// Radices 2, 3 and 4 have explicit butterflies; any other factor (11, 43,
+// 127, 257, ...) is a direct DFT pass computing one output per work item.
+//
+// DiffractionStep (diffraction.f90:209-348) becomes nine line passes:
+//   x fwd | y fwd | z fwd . multiplyexp . z inv | y inv |
+//   x inv . /ffact . mask split . x fwd | y fwd | z fwd . damping . z inv |
+//   y inv | x inv . /ffact . + interior
+#include "puff_common.hpp"
+#include "puff_kernels.hpp"
+
+namespace puff {
+
+// LDS index of element j of line b.  Interleaved layout (b fastest) keeps
+// the strided y / z2 loads conflict free; line-major suits contiguous x lines.
+template <bool IL>
+__device__ __forceinline__ int lidx(int b, int j, int B, int N) {
+    return IL ? j * B + b : b * N + j;
+}
+
+template <typename R>
+__device__ __forceinline__ cplx<R> tw_get(const cplx<R>* tw, int e, int sign) {
+    cplx<R> w = tw[e];
+    if (sign > 0) w.y = -w.y;
+    return w;
+}
+
+// In-LDS transform of B lines of length N.  src holds the input; returns the
+// buffer holding the output (src or dst).
+template <typename R, bool IL>
+__device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
+                            const cplx<R>* tw, int sign) {
+    const int N = ax.n;
+    int p = 1;
+    for (int ps = 0; ps < ax.npass; ++ps) {
+        const int r = ax.radix[ps];
+        const int T = N / r;
+        const int twst = N / (p * r);
+        if (r == 2 || r == 3 || r == 4) {
+            for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
+                int b, i;
+                if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
+                const int k = i % p;
+                const int ob = (i / p) * p * r + k;
+                cplx<R> v[4];
+                for (int j = 0; j < r; ++j) {
+                    v[j] = src[lidx<IL>(b, i + j * T, B, N)];
+                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, (j * k * twst) % N, sign));
+                }
+                if (r == 2) {
+                    dst[lidx<IL>(b, ob, B, N)] = cadd(v[0], v[1]);
+                    dst[lidx<IL>(b, ob + p, B, N)] = csub(v[0], v[1]);
+                } else if (r == 3) {
+                    const R s3 = (R)0.86602540378443864676 * (R)(sign < 0 ? -1 : 1);
+                    const cplx<R> t1 = cadd(v[1], v[2]);
+                    const cplx<R> t2 = {v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y};
+                    const cplx<R> d = csub(v[1], v[2]);
+                    const cplx<R> t3 = {-s3 * d.y, s3 * d.x};   // i s s3 (x1 - x2)
+                    dst[lidx<IL>(b, ob, B, N)] = cadd(v[0], t1);
+                    dst[lidx<IL>(b, ob + p, B, N)] = cadd(t2, t3);
+                    dst[lidx<IL>(b, ob + 2 * p, B, N)] = csub(t2, t3);
+                } else {
+                    const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+                    const cplx<R> b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+                    // W4 b1: sign -1 -> -i b1, sign +1 -> +i b1
+                    const cplx<R> wb = (sign < 0) ? cplx<R>{b1.y, -b1.x} : cplx<R>{-b1.y, b1.x};
+                    dst[lidx<IL>(b, ob, B, N)] = cadd(a0, b0);
+                    dst[lidx<IL>(b, ob + p, B, N)] = cadd(a1, wb);
+                    dst[lidx<IL>(b, ob + 2 * p, B, N)] = csub(a0, b0);
+                    dst[lidx<IL>(b, ob + 3 * p, B, N)] = csub(a1, wb);
+                }
+            }
+        } else {
+            // direct DFT pass: work item (b, i, q) -> one output
+            const int nr = N / r;
+            for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+                int b, t;
+                if (IL) { b = w % B; t = w / B; } else { t = w % N; b = w / N; }
+                const int q = t / T;
+                const int i = t % T;
+                const int k = i % p;
+                const int de = (k * twst + q * nr) % N;   // exponent step per input
+                int e = 0;
+                R sr = 0, si = 0;
+                for (int j = 0; j < r; ++j) {
+                    const cplx<R> x = src[lidx<IL>(b, i + j * T, B, N)];
+                    const cplx<R> wv = tw_get(tw, e, sign);
+                    sr += x.x * wv.x - x.y * wv.y;
+                    si += x.x * wv.y + x.y * wv.x;
+                    e += de;
+                    if (e >= N) e -= N;
+                }
+                dst[lidx<IL>(b, (i / p) * p * r + k + q * p, B, N)] = {sr, si};
+            }
+        }
+        __syncthreads();
+        cplx<R>* t = src;
+        src = dst;
+        dst = t;
+        p *= r;
+    }
+    return src;
+}
+
+// ---- x lines (contiguous) --------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int sign) {
+    extern __shared__ unsigned char smem_raw[];
+    const int N = a.nx;
+    cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* s1 = s0 + B * N;
+    const int64_t nlines = (int64_t)a.ny * a.nzp;
+    const int64_t L0 = (int64_t)blockIdx.x * B;
+    const int nb = (int)min<int64_t>(B, nlines - L0);
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.ax.tw);
+    cplx<R>* base = a.a + L0 * N;
+    for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+        const int b = w / N;
+        s0[w] = (b < nb) ? base[w] : cplx<R>{0, 0};
+    }
+    __syncthreads();
+    const int first_sign = (op == XOP_FORWARD) ? sign : +1;
+    cplx<R>* res = lds_fft<R, false>(s0, s1, B, a.ax, tw, first_sign);
+    if (op == XOP_INV_SPLIT_FWD) {
+        // /ffact then AbsorptionStep's split (diffraction.f90:310, 454-470)
+        cplx<R>* anb = a.anb + L0 * N;
+        for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
+            const int b = w / N, j = w % N;
+            const int64_t L = L0 + b;
+            const int iy = (int)(L % a.ny), iz = (int)(L / a.ny);
+            const double mxy = a.mxy[(int64_t)iy * a.nx + j];
+            const double m = mxy + (a.mz[iz] * (1.0 - mxy));
+            cplx<R> v = res[w];
+            const double vr = (double)v.x / a.ffact, vi = (double)v.y / a.ffact;
+            anb[w] = {(R)((1.0 - m) * vr), (R)((1.0 - m) * vi)};
+            res[w] = {(R)(m * vr), (R)(m * vi)};
+        }
+        __syncthreads();
+        cplx<R>* other = (res == s0) ? s1 : s0;
+        res = lds_fft<R, false>(res, other, B, a.ax, tw, -1);
+    } else if (op == XOP_INV_RECOMBINE) {
+        // /ffact + interior (diffraction.f90:513-517)
+        const cplx<R>* anb = a.anb + L0 * N;
+        for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
+            const cplx<R> v = res[w];
+            const cplx<R> q = anb[w];
+            if (sizeof(R) == 8) {
+                res[w] = {(R)((double)v.x / a.ffact + (double)q.x), (R)((double)v.y / a.ffact + (double)q.y)};
+            } else {
+                res[w] = {(R)((double)v.x / a.ffact) + q.x, (R)((double)v.y / a.ffact) + q.y};
+            }
+        }
+        __syncthreads();
+    }
+    for (int w = threadIdx.x; w < nb * N; w += blockDim.x) base[w] = res[w];
+    if (op == XOP_INV_RECOMBINE && a.periodic) {
+        // keep the aliased plane NZ2 equal to plane 1 (para_field.f90:3197-3244)
+        const int64_t ntr = (int64_t)a.nx * a.ny;
+        for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
+            const int64_t L = L0 + w / N;
+            if (L < a.ny) a.a[(int64_t)(a.nz2 - 1) * ntr + L * N + (w % N)] = res[w];
+        }
+    }
+}
+
+// ---- y lines (stride nx), B consecutive x per block -------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int B, int sign) {
+    extern __shared__ unsigned char smem_raw[];
+    const int N = a.ny;
+    cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* s1 = s0 + B * N;
+    const int x0 = blockIdx.x * B;
+    const int iz = blockIdx.y;
+    const int nb = min(B, a.nx - x0);
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.ay.tw);
+    cplx<R>* base = a.a + (int64_t)iz * a.nx * a.ny + x0;
+    for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+        const int b = w % B, j = w / B;
+        s0[w] = (b < nb) ? base[(int64_t)j * a.nx + b] : cplx<R>{0, 0};
+    }
+    __syncthreads();
+    cplx<R>* res = lds_fft<R, true>(s0, s1, B, a.ay, tw, sign);
+    for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+        const int b = w % B, j = w / B;
+        if (b < nb) base[(int64_t)j * a.nx + b] = res[w];
+    }
+}
+
+// ---- z2 lines (stride nx ny), B consecutive x per block ----------------------
+// op < 0: plain transform with `sign`; else forward, spectral op, inverse.
+template <typename R>
+__global__ __launch_bounds__(256) void k_fft_z(DiffArgs<R> a, int B, int op, int sign) {
+    extern __shared__ unsigned char smem_raw[];
+    const int N = a.nzp;
+    cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* s1 = s0 + B * N;
+    const int x0 = blockIdx.x * B;
+    const int iy = blockIdx.y;
+    const int nb = min(B, a.nx - x0);
+    const int64_t ntr = (int64_t)a.nx * a.ny;
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.az.tw);
+    cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
+    for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+        const int b = w % B, j = w / B;
+        s0[w] = (b < nb) ? base[(int64_t)j * ntr + b] : cplx<R>{0, 0};
+    }
+    __syncthreads();
+    cplx<R>* res;
+    if (op < 0) {
+        res = lds_fft<R, true>(s0, s1, B, a.az, tw, sign);
+    } else {
+        res = lds_fft<R, true>(s0, s1, B, a.az, tw, -1);
+        const double ky = a.ky[iy];
+        for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+            const int b = w % B, q = w / B;
+            if (b >= nb) continue;
+            const double kx = a.kx[x0 + b];
+            const double kz = a.kz[q];
+            cplx<R> v = res[w];
+            if (op == SPEC_PROPAGATE) {
+                // multiplyexp (diffraction.f90:150-168)
+                if ((kz > a.cutoff) || (kz < -a.cutoff)) {
+                    if (kz != 0.0) {
+                        const double th = a.h * (kx * kx + ky * ky) / (2.0 * kz);
+                        double sn, cs;
+                        sincos(th, &sn, &cs);
+                        const double vr = (double)v.x, vi = (double)v.y;
+                        v = {(R)(cs * vr - sn * vi), (R)(sn * vr + cs * vi)};
+                    }
+                } else if (a.filter) {
+                    v = {(R)0, (R)0};
+                }
+            } else {
+                // AbsorptionStep damping (diffraction.f90:484-497)
+                if (kz != 0.0) {
+                    const double f = exp(-a.h * a.beta * (fabs(kx) + fabs(ky)) / (sqrt(fabs(2.0 * kz))));
+                    v = {(R)(f * (double)v.x), (R)(f * (double)v.y)};
+                }
+            }
+            res[w] = v;
+        }
+        __syncthreads();
+        cplx<R>* other = (res == s0) ? s1 : s0;
+        res = lds_fft<R, true>(res, other, B, a.az, tw, +1);
+    }
+    for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+        const int b = w % B, j = w / B;
+        if (b < nb) base[(int64_t)j * ntr + b] = res[w];
+    }
+}
+
+// ---- host side -----------------------------------------------------------------
+
+static int lines_per_block(int n, size_t elem, int cap) {
+    const size_t budget = 64 * 1024;   // two LDS buffers per block
+    int b = (int)(budget / (2 * (size_t)n * elem));
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return b;
+}
+
+template <typename R>
+static void run_x(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
+    const int B = lines_per_block(a.nx, sizeof(cplx<R>), 64);
+    const int64_t nlines = (int64_t)a.ny * a.nzp;
+    const size_t sh = 2 * (size_t)B * a.nx * sizeof(cplx<R>);
+    hipLaunchKernelGGL((k_fft_x<R>), dim3((unsigned)((nlines + B - 1) / B)), dim3(256), sh, s, a, B, op, sign);
+}
+
+template <typename R>
+static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
+    const int B = lines_per_block(a.ny, sizeof(cplx<R>), 16);
+    const size_t sh = 2 * (size_t)B * a.ny * sizeof(cplx<R>);
+    hipLaunchKernelGGL((k_fft_y<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(256), sh, s,
+                       a, B, sign);
+}
+
+template <typename R>
+static void run_z(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
+    const int B = lines_per_block(a.nzp, sizeof(cplx<R>), 16);
+    const size_t sh = 2 * (size_t)B * a.nzp * sizeof(cplx<R>);
+    hipLaunchKernelGGL((k_fft_z<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.ny), dim3(256), sh, s,
+                       a, B, op, sign);
+}
+
+template <typename R>
+void launch_diffraction(const DiffArgs<R>& a, hipStream_t s) {
+    run_x(a, XOP_FORWARD, -1, s);
+    run_y(a, -1, s);
+    run_z(a, SPEC_PROPAGATE, 0, s);
+    run_y(a, +1, s);
+    run_x(a, XOP_INV_SPLIT_FWD, 0, s);
+    run_y(a, -1, s);
+    run_z(a, SPEC_ABSORB, 0, s);
+    run_y(a, +1, s);
+    run_x(a, XOP_INV_RECOMBINE, 0, s);
+}
+
+template <typename R>
+void launch_fft3d(cplx<R>* data, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
+                  const FftAxis& az, int sign, hipStream_t s) {
+    DiffArgs<R> a{};
+    a.a = data;
+    a.nx = nx;
+    a.ny = ny;
+    a.nz2 = nz;
+    a.nzp = nz;
+    a.ax = ax;
+    a.ay = ay;
+    a.az = az;
+    run_x(a, XOP_FORWARD, sign, s);
+    run_y(a, sign, s);
+    run_z(a, -1, sign, s);
+}
+
+template void launch_diffraction<double>(const DiffArgs<double>&, hipStream_t);
+template void launch_diffraction<float>(const DiffArgs<float>&, hipStream_t);
+template void launch_fft3d<double>(cplx<double>*, int, int, int, const FftAxis&, const FftAxis&,
+                                   const FftAxis&, int, hipStream_t);
+template void launch_fft3d<float>(cplx<float>*, int, int, int, const FftAxis&, const FftAxis&,
+                                  const FftAxis&, int, hipStream_t);
+
+}  // namespace puff

@@ -1,0 +1,129 @@
+// Kernel argument blocks and host launchers shared by the .hip files.
+#pragma once
+
+#include "puff_common.hpp"
+
+namespace puff {
+
+// Mesh geometry as seen by the push kernels.  The whole mesh is the
+// interpolation window: halfx = ((NX-1)/2) dx (rhs.f90:395-396 with
+// nspinDX = NX).  nzp is the z2 period (NZ2-1) on a periodic mesh, else NZ2.
+struct MeshGeom {
+    int nx, ny, nz2, nzp;
+    int periodic, one_d;
+    double dx, dy, dz2;
+    double inv_dx, inv_dy, inv_dz2;
+    double halfx, halfy;
+};
+
+struct Physics {
+    double eta, rho, kappa, gamma_r, aw;
+    double inv2rho;     // 1 / (2 rho)                    (rhs.f90:366)
+    double inv_dV3;     // 1 / (dx dy dz2)                (rhs.f90:361)
+    double c_dx;        // 2 rho kappa / sqrt(eta)        (puffin_equations.f90:115)
+    int coupling, electrons_evolve, field_evolve;
+};
+
+// Six particle coordinates: x, y, p_r, p_i in R; z2 and Gamma in FP64.
+template <typename R>
+struct PartPtrs {
+    R* x;
+    R* y;
+    double* z2;
+    R* pr;
+    R* pi;
+    double* g;
+};
+
+template <typename R>
+struct StageArgs {
+    int64_t n;
+    PartPtrs<R> y0;     // state at the start of the step (written by stage 4)
+    PartPtrs<R> kin;    // derivative of the previous stage
+    PartPtrs<R> kout;   // derivative of this stage (stages 1-3)
+    PartPtrs<R> k1, k2; // stage 4: first two derivatives (k3 = kin)
+    const R* chi;
+    double cin;         // trial coefficient of this stage (0, h/2, h/2, h)
+    double cnext;       // trial coefficient of the next stage
+    double h6;
+    const cplx<R>* A0;
+    const cplx<R>* dAin;
+    cplx<R>* dAout;
+    MeshGeom mesh;
+    Physics phys;
+    StageScalars s;
+    int* flags;
+    Box* box;
+};
+
+template <typename R>
+struct CombineArgs {
+    cplx<R>* A;
+    cplx<R>* d[4];
+    double h6;
+    const Box* box;
+    const int* flags;
+    int nx, ny, nz2, nzp, periodic, one_d;
+};
+
+template <typename R>
+void launch_stage(int stage, const StageArgs<R>& a, hipStream_t s);
+void launch_step_begin(int* flags, Box* box, hipStream_t s);
+template <typename R>
+void launch_combine(const CombineArgs<R>& a, hipStream_t s);
+void launch_wrap(double* z2, int64_t n, double lenz2, hipStream_t s);
+
+// ---- diffraction -----------------------------------------------------------
+
+// Spectral operation applied between the forward and inverse z2 transforms.
+enum SpecOp { SPEC_PROPAGATE = 0, SPEC_ABSORB = 1 };
+
+// x-line epilogues of the diffraction pipeline.
+enum XOp {
+    XOP_FORWARD = 0,        // forward transform only
+    XOP_INV_SPLIT_FWD = 1,  // inverse, /ffact, split into boundary and interior, forward
+    XOP_INV_RECOMBINE = 2   // inverse, /ffact, + interior
+};
+
+template <typename R>
+struct DiffArgs {
+    cplx<R>* a;          // field, nz2 planes (only the first nzp take part)
+    cplx<R>* anb;        // interior part (AbsorptionStep's sAnb)
+    const double* kx;    // GetKValues
+    const double* ky;
+    const double* kz;
+    const double* mxy;   // transverse mask
+    const double* mz;    // z2 mask
+    int nx, ny, nz2, nzp, periodic, filter;
+    double h, beta, cutoff, ffact;
+    FftAxis ax, ay, az;
+};
+
+template <typename R>
+void launch_diffraction(const DiffArgs<R>& a, hipStream_t s);
+
+// Plain 3-D transform (tests): sign -1 forward, +1 backward, unnormalised.
+template <typename R>
+void launch_fft3d(cplx<R>* a, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
+                  const FftAxis& az, int sign, hipStream_t s);
+
+// ---- diagnostics -----------------------------------------------------------
+template <typename R>
+void launch_power(const cplx<R>* a, int nx, int ny, int nz2, double dx, double dy,
+                  double* out, hipStream_t s);
+template <typename R>
+void launch_bunching(const double* z2, const R* chi, int64_t n, double rho, int harmonic,
+                     int nslices, double slice, int periodic, double* acc, hipStream_t s);
+
+// ---- data movement ---------------------------------------------------------
+void launch_fill_zero(double* p, int64_t n, hipStream_t s);
+template <typename R>
+void launch_pack_field(const double* ar, const double* ai, cplx<R>* a, int64_t n, hipStream_t s);
+template <typename R>
+void launch_unpack_field(const cplx<R>* a, double* ar, double* ai, int64_t n, hipStream_t s);
+template <typename R>
+void launch_convert(const double* src, R* dst, int64_t n, hipStream_t s);
+template <typename R>
+void launch_convert_back(const R* src, double* dst, int64_t n, hipStream_t s);
+
+}  // namespace puff

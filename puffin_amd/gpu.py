@@ -1,0 +1,241 @@
+"""Python front end of the C ABI (include/puffin_gpu.h).
+
+``Integrator`` holds one device context: the macroparticles and the A_perp
+mesh of one undulator module, exactly the module globals the reference's
+rk4par / DiffractionStep read (deriv_globals.f90:94,128-133;
+puffin_mpi_RK4.f90:24).  Every call goes to libpuffin_gpu.so; there is no CPU
+fallback: a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .params import FP64, MEM_DEVICE, MEM_HOST, MIXED, STATUS_NAMES, ModuleParams, PuffParams
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpuffin_gpu.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# Exported symbols and their signatures (include/puffin_gpu.h).
+SIGNATURES = {
+    "puff_last_error": (ctypes.c_char_p, []),
+    "puff_device_count": (ctypes.c_int, []),
+    "puff_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.POINTER(PuffParams), ctypes.c_int64, ctypes.c_int32]),
+    "puff_destroy": (ctypes.c_int, [_vp]),
+    "puff_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "puff_synchronize": (ctypes.c_int, [_vp]),
+    "puff_upload_particles": (ctypes.c_int, [_vp, ctypes.c_int32] + [_vp] * 7),
+    "puff_download_particles": (ctypes.c_int, [_vp, ctypes.c_int32] + [_vp] * 6),
+    "puff_upload_field": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp]),
+    "puff_download_field": (ctypes.c_int, [_vp, ctypes.c_int32, _vp, _vp]),
+    "puff_set_taper": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "puff_get_taper": (ctypes.c_int, [_vp, _dp, _dp]),
+    "puff_set_taper_table": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double]),
+    "puff_rk4par": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "puff_diffraction_step": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "puff_transform": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "puff_wrap_z2": (ctypes.c_int, [_vp]),
+    "puff_power": (ctypes.c_int, [_vp, _dp]),
+    "puff_bunching": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp]),
+    "puff_particle_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "puff_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "puff_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
+                                         ctypes.c_int32]),
+    "puff_active_box": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
+}
+
+PROFILE_KINDS = ("stage1", "stage2", "stage3", "stage4", "combine", "diffraction", "step_begin")
+
+
+class PuffinError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        msg = f"{where}: {STATUS_NAMES.get(status, 'error')} (status {status})"
+        if detail:
+            msg += f": {detail}"
+        super().__init__(msg)
+
+
+def lib() -> ctypes.CDLL:
+    """Load libpuffin_gpu.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C puffin_amd` "
+                               "or __graft_entry__.build()")
+        l = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(l, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = l
+    return _lib
+
+
+def last_error() -> str:
+    return lib().puff_last_error().decode(errors="replace")
+
+
+def device_count() -> int:
+    return int(lib().puff_device_count())
+
+
+def _check(status: int, where: str) -> int:
+    if status < 0:
+        raise PuffinError(status, where, last_error())
+    return status
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data
+
+
+PARTICLE_KEYS = ("x", "y", "z2", "px", "py", "gam")
+
+
+class Integrator:
+    """One module's particles and field on the current HIP device."""
+
+    def __init__(self, mp: ModuleParams, n: int, precision: int = FP64):
+        self.mp = mp
+        self.n = int(n)
+        self.precision = precision
+        self._params = mp.to_ctypes()
+        h = _vp()
+        _check(lib().puff_create(ctypes.byref(h), ctypes.byref(self._params), self.n, precision),
+               "puff_create")
+        self._h = h
+
+    # ---- lifetime
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().puff_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- data exchange (host numpy arrays, reference layout)
+    def upload_particles(self, p: dict):
+        arrs = [np.ascontiguousarray(p[k], dtype=np.float64) for k in PARTICLE_KEYS + ("chi",)]
+        for a in arrs:
+            assert a.size == self.n
+        _check(lib().puff_upload_particles(self._h, MEM_HOST, *[_ptr(a) for a in arrs]),
+               "puff_upload_particles")
+
+    def download_particles(self) -> dict:
+        out = {k: np.empty(self.n) for k in PARTICLE_KEYS}
+        _check(lib().puff_download_particles(self._h, MEM_HOST, *[_ptr(out[k]) for k in PARTICLE_KEYS]),
+               "puff_download_particles")
+        return out
+
+    def upload_field(self, ar: np.ndarray, ai: np.ndarray):
+        ar = np.ascontiguousarray(ar, dtype=np.float64).reshape(-1)
+        ai = np.ascontiguousarray(ai, dtype=np.float64).reshape(-1)
+        assert ar.size == self.mp.nmesh and ai.size == self.mp.nmesh
+        _check(lib().puff_upload_field(self._h, MEM_HOST, _ptr(ar), _ptr(ai)), "puff_upload_field")
+
+    def download_field(self):
+        ar = np.empty(self.mp.nmesh)
+        ai = np.empty(self.mp.nmesh)
+        _check(lib().puff_download_field(self._h, MEM_HOST, _ptr(ar), _ptr(ai)), "puff_download_field")
+        return ar, ai
+
+    # device-pointer variants (torch tensors, FP64, reference layout)
+    def upload_particles_device(self, ptrs: list[int]):
+        _check(lib().puff_upload_particles(self._h, MEM_DEVICE, *ptrs), "puff_upload_particles")
+
+    def download_particles_device(self, ptrs: list[int]):
+        _check(lib().puff_download_particles(self._h, MEM_DEVICE, *ptrs), "puff_download_particles")
+
+    def upload_field_device(self, ar_ptr: int, ai_ptr: int):
+        _check(lib().puff_upload_field(self._h, MEM_DEVICE, ar_ptr, ai_ptr), "puff_upload_field")
+
+    def download_field_device(self, ar_ptr: int, ai_ptr: int):
+        _check(lib().puff_download_field(self._h, MEM_DEVICE, ar_ptr, ai_ptr), "puff_download_field")
+
+    def set_stream(self, stream_ptr: int | None):
+        _check(lib().puff_set_stream(self._h, stream_ptr), "puff_set_stream")
+
+    def synchronize(self):
+        _check(lib().puff_synchronize(self._h), "puff_synchronize")
+
+    # ---- taper
+    def set_taper(self, n2col: float, n2col0: float, undgrad: float):
+        _check(lib().puff_set_taper(self._h, n2col, n2col0, undgrad), "puff_set_taper")
+
+    def get_taper(self):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().puff_get_taper(self._h, ctypes.byref(a), ctypes.byref(b)), "puff_get_taper")
+        return a.value, b.value
+
+    def set_taper_table(self, values, z0: float, period: float):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(lib().puff_set_taper_table(self._h, v.ctypes.data_as(_dp), v.size, z0, period),
+               "puff_set_taper_table")
+
+    # ---- hot path
+    def rk4par(self, sz: float, h: float) -> int:
+        """One RK4 step; returns the status (0 ok, 1..3 flag raised, state kept)."""
+        return _check(lib().puff_rk4par(self._h, sz, h), "puff_rk4par")
+
+    def diffraction_step(self, h: float):
+        _check(lib().puff_diffraction_step(self._h, h), "puff_diffraction_step")
+
+    def transform(self, sign: int):
+        _check(lib().puff_transform(self._h, sign), "puff_transform")
+
+    def wrap_z2(self):
+        _check(lib().puff_wrap_z2(self._h), "puff_wrap_z2")
+
+    # ---- instrumentation
+    def profile(self, on: bool = True):
+        _check(lib().puff_profile_enable(self._h, int(on)), "puff_profile_enable")
+
+    def profile_read(self, reset: bool = False) -> dict:
+        k = len(PROFILE_KINDS)
+        ms = np.zeros(k)
+        cnt = np.zeros(k, dtype=np.int64)
+        _check(lib().puff_profile_read(self._h, ms.ctypes.data_as(_dp),
+                                       cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), k, int(reset)),
+               "puff_profile_read")
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(PROFILE_KINDS)}
+
+    def active_box(self):
+        b = (ctypes.c_int32 * 6)()
+        _check(lib().puff_active_box(self._h, b), "puff_active_box")
+        return tuple(b)
+
+    # ---- diagnostics
+    def power(self) -> np.ndarray:
+        out = np.empty(self.mp.nz2)
+        _check(lib().puff_power(self._h, out.ctypes.data_as(_dp)), "puff_power")
+        return out
+
+    def bunching(self, harmonic: int, nslices: int, slice_len: float) -> np.ndarray:
+        out = np.empty(nslices)
+        _check(lib().puff_bunching(self._h, harmonic, nslices, slice_len, out.ctypes.data_as(_dp)),
+               "puff_bunching")
+        return out

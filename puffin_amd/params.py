@@ -198,4 +198,5 @@ def step_size(rho: float, steps_per_period: int) -> float:
 def steps_for_diffraction(rho: float, h: float, diff_frac: float = 1.0) -> int:
     """iSteps4Diff = nint(diffStep / sStepSize), diffStep = sDiffFrac 4 pi rho
     (setup_calcs.f90:240, undulator.f90:106)."""
-    return int(round(diff_frac * 4.0 * PI_REF * rho / h))
+    v = diff_frac * 4.0 * PI_REF * rho / h
+    return int(math.floor(v + 0.5)) if v >= 0 else -int(math.floor(-v + 0.5))
