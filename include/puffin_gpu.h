@@ -149,6 +149,13 @@ int puff_set_taper_table(puff_ctx* ctx, const double* values, int32_t nvals,
                          double z0, double period);
 
 /*
+ * Macroparticles are kept sorted by mesh cell on the device (re-sorted every
+ * `steps` RK4 steps, default 30; 0 disables sorting).  The order is internal:
+ * downloads always return the reference order of the last upload.
+ */
+int puff_set_sort_interval(puff_ctx* ctx, int32_t steps);
+
+/*
  * One RK4 step of particles and field together.
  * Replaces rk4par(sZ, h, qD) (puffin_mpi_RK4.f90:38-366) with its four
  * derivs / getrhs evaluations (derivative.f90:36-155, rhs.f90:60-330).

@@ -46,11 +46,18 @@ constexpr int FLAG_TRANSVERSE = 1;
 constexpr int FLAG_BUFFER = 2;
 constexpr int FLAG_Z2MESH = 4;
 
-// Active box of one RK4 step: min/max cell index along x, y, z2.
+// Active box of one RK4 step: min/max cell index along x, y, z2.  The stage
+// kernels update BOX_COPIES copies (by workgroup) to spread the atomics.
 struct Box {
     int lo[3];
     int hi[3];
 };
+constexpr int BOX_COPIES = 64;
+
+// Workgroup of the RK4 stage kernels (one macroparticle per thread) and the
+// node capacity of its LDS deposit tile (2 FP64 per node: 32 KiB).
+constexpr int STAGE_THREADS = 1024;
+constexpr int TILE_NODES = 2048;
 
 // Wavefront (64-lane) reductions.
 __device__ __forceinline__ int wave_min(int v) {

@@ -7,6 +7,7 @@
 //   k-vectors                    diffraction/transforms.f90:472-590
 //   masks                        diffraction/masks.f90:50-430
 //   ffact                        setup/setup.f90:334-346
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -68,6 +69,19 @@ struct Ctx {
     double taper_z0 = 0.0, taper_period = 1.0;
     MeshGeom mesh{};
     Physics phys{};
+    // particle order: orig[j] = reference index of slot j
+    int* orig = nullptr;
+    int* orig_spare = nullptr;
+    void* chi_spare = nullptr;
+    int* sort_key = nullptr;
+    int* sort_counts = nullptr;
+    int* sort_perm = nullptr;
+    int* sort_tiles = nullptr;
+    int ncell = 0;
+    bool identity_order = true;
+    bool sorted = false;
+    int sort_every = 30;
+    int64_t steps_since_sort = 0;
     // kernel timing (puff_profile_*)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -367,6 +381,16 @@ static int alloc_particles(Ctx* c) {
     PUFF_TRY(setp(P.y));
     for (int k = 0; k < 3; ++k) PUFF_TRY(setp(P.k[k]));
     PUFF_TRY(dalloc(c, (void**)&P.chi, sizeof(R) * c->n));
+    PUFF_TRY(dalloc(c, &c->chi_spare, sizeof(R) * c->n));
+    PUFF_TRY(dalloc(c, (void**)&c->orig, sizeof(int) * c->n));
+    PUFF_TRY(dalloc(c, (void**)&c->orig_spare, sizeof(int) * c->n));
+    PUFF_TRY(dalloc(c, (void**)&c->sort_key, sizeof(int) * c->n));
+    PUFF_TRY(dalloc(c, (void**)&c->sort_perm, sizeof(int) * c->n));
+    c->ncell = c->p.nx * c->p.ny * c->nzp();
+    PUFF_TRY(dalloc(c, (void**)&c->sort_counts, sizeof(int) * ((size_t)c->ncell + 1)));
+    PUFF_TRY(dalloc(c, (void**)&c->sort_tiles, sizeof(int) * ((size_t)c->ncell / 4096 + 2)));
+    launch_iota(c->orig, c->n, nullptr);
+    PUFF_HIP(hipDeviceSynchronize());
     const int64_t nm = c->nmesh();
     PUFF_TRY(dalloc(c, &c->A, sizeof(cplx<R>) * nm));
     for (int k = 0; k < 4; ++k) PUFF_TRY(dalloc(c, &c->dA[k], sizeof(cplx<R>) * nm));
@@ -439,7 +463,7 @@ static int create(Ctx* c) {
     else PUFF_TRY(alloc_particles<float>(c));
 
     PUFF_TRY(dalloc(c, (void**)&c->flags, sizeof(int)));
-    PUFF_TRY(dalloc(c, (void**)&c->box, sizeof(Box)));
+    PUFF_TRY(dalloc(c, (void**)&c->box, sizeof(Box) * BOX_COPIES));
     PUFF_HIP(hipHostMalloc((void**)&c->h_flags, sizeof(int), hipHostMallocDefault));
     c->stage_len = 2 * std::max<int64_t>(c->n, c->nmesh());
     PUFF_TRY(dalloc(c, (void**)&c->stage_buf, sizeof(double) * c->stage_len));
@@ -515,6 +539,32 @@ static void release(Ctx* c) {
 // ---- the integrator step ----------------------------------------------------
 
 template <typename R>
+static void sort_particles(Ctx* c) {
+    Particles<R>& P = parts<R>(c);
+    SortArgs<R> a{};
+    a.n = c->n;
+    a.ncell = c->ncell;
+    a.mesh = c->mesh;
+    a.y = &P.y;
+    a.spare = &P.k[0];
+    a.chi = &P.chi;
+    R* spare_chi = reinterpret_cast<R*>(c->chi_spare);
+    a.chi_spare = &spare_chi;
+    a.orig = &c->orig;
+    a.orig_spare = &c->orig_spare;
+    a.key = c->sort_key;
+    a.counts = c->sort_counts;
+    a.cursor = c->sort_counts;
+    a.perm = c->sort_perm;
+    a.tile_sums = c->sort_tiles;
+    launch_sort<R>(a, c->stream);
+    c->chi_spare = spare_chi;
+    c->sorted = true;
+    c->identity_order = false;
+    c->steps_since_sort = 0;
+}
+
+template <typename R>
 static int rk4par(Ctx* c, double sz, double h) {
     Particles<R>& P = parts<R>(c);
     hipStream_t s = c->stream;
@@ -531,6 +581,10 @@ static int rk4par(Ctx* c, double sz, double h) {
             return st;
         }
     }
+    if (c->sort_every > 0 && (!c->sorted || c->steps_since_sort >= c->sort_every)) {
+        sort_particles<R>(c);
+    }
+    c->steps_since_sort++;
     int slot = prof_start(c, 6);
     launch_step_begin(c->flags, c->box, s);
     prof_stop(c, slot);
@@ -630,48 +684,37 @@ static int diffraction(Ctx* c, double h) {
 
 // ---- data exchange ------------------------------------------------------------
 
-template <typename R>
-static int upload_coord(Ctx* c, int kind, const double* src, R* dst) {
+// One coordinate array from the reference order (FP64) into the slot order
+// and storage type T of the device.
+template <typename T>
+static int upload_array(Ctx* c, int kind, const double* src, T* dst) {
     if (!src) return PUFF_OK;
     const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    if (sizeof(R) == sizeof(double)) {
+    if (c->identity_order && sizeof(T) == sizeof(double)) {
         PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
     } else {
         PUFF_HIP(hipMemcpyAsync(c->stage_buf, src, sizeof(double) * c->n, k, c->stream));
-        launch_convert<R>(c->stage_buf, dst, c->n, c->stream);
+        if (c->identity_order) launch_convert<T>(c->stage_buf, dst, c->n, c->stream);
+        else launch_permute_in<T>(c->stage_buf, c->orig, c->n, dst, c->stream);
         PUFF_HIP(hipGetLastError());
     }
     PUFF_HIP(hipStreamSynchronize(c->stream));
     return PUFF_OK;
 }
 
-template <typename R>
-static int download_coord(Ctx* c, int kind, const R* src, double* dst) {
+// One coordinate array from the slot order back to the reference order (FP64).
+template <typename T>
+static int download_array(Ctx* c, int kind, const T* src, double* dst) {
     if (!dst) return PUFF_OK;
     const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    if (sizeof(R) == sizeof(double)) {
+    if (c->identity_order && sizeof(T) == sizeof(double)) {
         PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
     } else {
-        launch_convert_back<R>(src, c->stage_buf, c->n, c->stream);
+        if (c->identity_order) launch_convert_back<T>(src, c->stage_buf, c->n, c->stream);
+        else launch_unpermute<T>(src, c->orig, c->n, c->stage_buf, c->stream);
         PUFF_HIP(hipGetLastError());
         PUFF_HIP(hipMemcpyAsync(dst, c->stage_buf, sizeof(double) * c->n, k, c->stream));
     }
-    PUFF_HIP(hipStreamSynchronize(c->stream));
-    return PUFF_OK;
-}
-
-static int upload_f64(Ctx* c, int kind, const double* src, double* dst) {
-    if (!src) return PUFF_OK;
-    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
-    PUFF_HIP(hipStreamSynchronize(c->stream));
-    return PUFF_OK;
-}
-
-static int download_f64(Ctx* c, int kind, const double* src, double* dst) {
-    if (!dst) return PUFF_OK;
-    const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    PUFF_HIP(hipMemcpyAsync(dst, src, sizeof(double) * c->n, k, c->stream));
     PUFF_HIP(hipStreamSynchronize(c->stream));
     return PUFF_OK;
 }
@@ -680,13 +723,20 @@ template <typename R>
 static int upload_particles(Ctx* c, int kind, const double* x, const double* y, const double* z2,
                             const double* px, const double* py, const double* g, const double* chi) {
     Particles<R>& P = parts<R>(c);
-    PUFF_TRY(upload_coord<R>(c, kind, x, P.y.x));
-    PUFF_TRY(upload_coord<R>(c, kind, y, P.y.y));
-    PUFF_TRY(upload_f64(c, kind, z2, P.y.z2));
-    PUFF_TRY(upload_coord<R>(c, kind, px, P.y.pr));
-    PUFF_TRY(upload_coord<R>(c, kind, py, P.y.pi));
-    PUFF_TRY(upload_f64(c, kind, g, P.y.g));
-    PUFF_TRY(upload_coord<R>(c, kind, chi, P.chi));
+    if (x && y && z2 && px && py && g && chi) {
+        // a complete state: back to the reference order, sorted on the next step
+        launch_iota(c->orig, c->n, c->stream);
+        PUFF_HIP(hipGetLastError());
+        c->identity_order = true;
+        c->sorted = false;
+    }
+    PUFF_TRY(upload_array<R>(c, kind, x, P.y.x));
+    PUFF_TRY(upload_array<R>(c, kind, y, P.y.y));
+    PUFF_TRY(upload_array<double>(c, kind, z2, P.y.z2));
+    PUFF_TRY(upload_array<R>(c, kind, px, P.y.pr));
+    PUFF_TRY(upload_array<R>(c, kind, py, P.y.pi));
+    PUFF_TRY(upload_array<double>(c, kind, g, P.y.g));
+    PUFF_TRY(upload_array<R>(c, kind, chi, P.chi));
     return PUFF_OK;
 }
 
@@ -694,12 +744,12 @@ template <typename R>
 static int download_particles(Ctx* c, int kind, double* x, double* y, double* z2, double* px,
                               double* py, double* g) {
     Particles<R>& P = parts<R>(c);
-    PUFF_TRY(download_coord<R>(c, kind, P.y.x, x));
-    PUFF_TRY(download_coord<R>(c, kind, P.y.y, y));
-    PUFF_TRY(download_f64(c, kind, P.y.z2, z2));
-    PUFF_TRY(download_coord<R>(c, kind, P.y.pr, px));
-    PUFF_TRY(download_coord<R>(c, kind, P.y.pi, py));
-    PUFF_TRY(download_f64(c, kind, P.y.g, g));
+    PUFF_TRY(download_array<R>(c, kind, P.y.x, x));
+    PUFF_TRY(download_array<R>(c, kind, P.y.y, y));
+    PUFF_TRY(download_array<double>(c, kind, P.y.z2, z2));
+    PUFF_TRY(download_array<R>(c, kind, P.y.pr, px));
+    PUFF_TRY(download_array<R>(c, kind, P.y.pi, py));
+    PUFF_TRY(download_array<double>(c, kind, P.y.g, g));
     return PUFF_OK;
 }
 
@@ -924,6 +974,12 @@ int puff_set_taper_table(puff_ctx* ctx, const double* values, int32_t nvals, dou
     return PUFF_OK;
 }
 
+int puff_set_sort_interval(puff_ctx* ctx, int32_t steps) {
+    if (!ctx || steps < 0) return PUFF_ERR_ARG;
+    ctx->c.sort_every = steps;
+    return PUFF_OK;
+}
+
 int puff_rk4par(puff_ctx* ctx, double sz, double h) {
     CTX_CHECK(ctx);
     return ctx->c.precision == PUFF_FP64 ? puff::rk4par<double>(&ctx->c, sz, h)
@@ -991,11 +1047,16 @@ int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds
 int puff_active_box(puff_ctx* ctx, int32_t* box6) {
     CTX_CHECK(ctx);
     if (!box6) return PUFF_ERR_ARG;
-    puff::Box b;
-    if (hipMemcpy(&b, ctx->c.box, sizeof(b), hipMemcpyDeviceToHost) != hipSuccess) return PUFF_ERR_HIP;
+    puff::Box b[puff::BOX_COPIES];
+    if (hipMemcpy(b, ctx->c.box, sizeof(b), hipMemcpyDeviceToHost) != hipSuccess) return PUFF_ERR_HIP;
     for (int d = 0; d < 3; ++d) {
-        box6[2 * d] = b.lo[d];
-        box6[2 * d + 1] = b.hi[d];
+        int lo = 0x7fffffff, hi = -1;
+        for (int k = 0; k < puff::BOX_COPIES; ++k) {
+            lo = std::min(lo, b[k].lo[d]);
+            hi = std::max(hi, b[k].hi[d]);
+        }
+        box6[2 * d] = lo;
+        box6[2 * d + 1] = hi;
     }
     return PUFF_OK;
 }

@@ -73,6 +73,33 @@ template <typename R>
 void launch_combine(const CombineArgs<R>& a, hipStream_t s);
 void launch_wrap(double* z2, int64_t n, double lenz2, hipStream_t s);
 
+// ---- particle sort by cell --------------------------------------------------
+template <typename R>
+struct SortArgs {
+    int64_t n;
+    int ncell;             // valid cells; bin ncell collects particles off the mesh
+    MeshGeom mesh;
+    PartPtrs<R>* y;        // sorted in place (pointers swapped with spare)
+    PartPtrs<R>* spare;
+    R** chi;
+    R** chi_spare;
+    int** orig;            // reference index of each slot
+    int** orig_spare;
+    int* key;
+    int* counts;           // ncell + 1 bins, scanned in place
+    int* cursor;
+    int* perm;
+    int* tile_sums;
+};
+
+template <typename R>
+void launch_sort(SortArgs<R>& a, hipStream_t s);
+void launch_iota(int* p, int64_t n, hipStream_t s);
+template <typename T>
+void launch_unpermute(const T* src, const int* orig, int64_t n, double* dst, hipStream_t s);
+template <typename T>
+void launch_permute_in(const double* src, const int* orig, int64_t n, T* dst, hipStream_t s);
+
 // ---- diffraction -----------------------------------------------------------
 
 // Spectral operation applied between the forward and inverse z2 transforms.

@@ -98,8 +98,13 @@ __device__ __forceinline__ void atomic_add(R* p, R v) {
 
 // One RK4 stage.  STAGE = 1..4 (rk4par's four derivs calls).
 template <typename R, int STAGE>
-__global__ __launch_bounds__(256) void k_stage(StageArgs<R> a) {
-    if (*a.flags) return;   // an earlier kernel of this step raised a flag
+__global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
+    // an earlier kernel of this step raised a flag: skip (block-uniform, the
+    // box reduction below uses a barrier)
+    __shared__ int s_skip;
+    if (threadIdx.x == 0) s_skip = *a.flags;
+    __syncthreads();
+    if (s_skip) return;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < a.n;
     const MeshGeom& m = a.mesh;
@@ -141,84 +146,169 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<R> a) {
     const double fac = (1.0 + ph.eta * p2) * invg;   // (1 + eta p2) / Gamma
 
     // ---- node and weights
-    Cell<R> c;
+    Cell<R> c{};
     int flag = in ? locate<R>(m, x, y, z2, c) : 0;
     const bool ok = in && flag == 0;
     if (flag) atomicOr(a.flags, flag);
 
+    // ---- bounding box of this workgroup's cells (wavefront, then workgroup
+    // reduction); max values are reduced as min of their negatives
+    __shared__ int s_red[6][STAGE_THREADS / 64];
+    __shared__ int s_bb[6];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    {
+        int v[6] = {ok ? c.ix : 0x7fffffff, ok ? -c.ix : 1,
+                    ok ? c.iy : 0x7fffffff, ok ? -c.iy : 1,
+                    ok ? c.iz0 : 0x7fffffff, ok ? -c.iz0 : 1};
+#pragma unroll
+        for (int d = 0; d < 6; ++d) {
+            v[d] = wave_min(v[d]);
+            if (lane == 0) s_red[d][wv] = v[d];
+        }
+        __syncthreads();
+        if (threadIdx.x < 6) {
+            const int d = threadIdx.x;
+            int r = s_red[d][0];
+            for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = min(r, s_red[d][k]);
+            s_bb[d] = (d & 1) ? -r : r;
+            // active box of the step: one atomic per workgroup into one of
+            // BOX_COPIES spread copies (same-address atomics would serialise)
+            const bool empty = (d & 1) ? (r == 1) : (r == 0x7fffffff);
+            if (!empty) {
+                Box* b = a.box + (blockIdx.x % BOX_COPIES);
+                if (d & 1) atomicMax(&b->hi[d >> 1], -r);
+                else atomicMin(&b->lo[d >> 1], r);
+            }
+        }
+        __syncthreads();
+    }
+    const bool any_ok = s_bb[1] >= 0;
+    const int bx0 = s_bb[0], by0 = s_bb[2], bz0 = s_bb[4];
+    const int tx = any_ok ? (m.one_d ? 1 : s_bb[1] - bx0 + 2) : 0;
+    const int ty = any_ok ? (m.one_d ? 1 : s_bb[3] - by0 + 2) : 0;
+    const int tz = any_ok ? s_bb[5] - bz0 + 2 : 0;
+    // tile-private deposit when the workgroup's node box fits in LDS and does
+    // not wrap around a periodic mesh
+    const bool use_tile = ph.field_evolve && any_ok && (int64_t)tx * ty * tz <= TILE_NODES &&
+                          !(m.periodic && s_bb[5] + 1 >= m.nzp);
+
+    // ---- nodes and tri-linear weights (zero weights on lanes without a cell)
+    R w[8];
+    int nd[8];
+    int lt[8];    // tile index of each corner
+    int nnb, key;
+    if (m.one_d) {
+        // lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1)
+        nnb = 2;
+        w[0] = ok ? (R)1 - c.fz : (R)0;
+        w[1] = ok ? (R)1 - ((R)1 - c.fz) : (R)0;
+        nd[0] = ok ? c.iz0 : 0;
+        nd[1] = ok ? c.iz1 : 0;
+        lt[0] = ok ? c.iz0 - bz0 : 0;
+        lt[1] = lt[0] + 1;
+        key = ok ? c.iz0 : -1 - lane;
+    } else {
+        nnb = 8;
+        const R x1 = (R)1 - c.fx, x2 = c.fx;
+        const R y1 = (R)1 - c.fy, y2 = c.fy;
+        const R z1 = (R)1 - c.fz, z2w = c.fz;
+        w[0] = x1 * y1 * z1;  w[1] = x2 * y1 * z1;  w[2] = x1 * y2 * z1;  w[3] = x2 * y2 * z1;
+        w[4] = x1 * y1 * z2w; w[5] = x2 * y1 * z2w; w[6] = x1 * y2 * z2w; w[7] = x2 * y2 * z2w;
+        const int ntr = m.nx * m.ny;
+        const int b0 = ok ? c.iz0 * ntr + c.iy * m.nx + c.ix : 0;
+        const int b1 = ok ? c.iz1 * ntr + c.iy * m.nx + c.ix : 0;
+        nd[0] = b0; nd[1] = b0 + 1; nd[2] = b0 + m.nx; nd[3] = b0 + m.nx + 1;
+        nd[4] = b1; nd[5] = b1 + 1; nd[6] = b1 + m.nx; nd[7] = b1 + m.nx + 1;
+        const int t0 = ok ? ((c.iz0 - bz0) * ty + (c.iy - by0)) * tx + (c.ix - bx0) : 0;
+        const int txy = tx * ty;
+        lt[0] = t0; lt[1] = t0 + 1; lt[2] = t0 + tx; lt[3] = t0 + tx + 1;
+        lt[4] = t0 + txy; lt[5] = t0 + txy + 1; lt[6] = t0 + txy + tx; lt[7] = t0 + txy + tx + 1;
+        key = ok ? b0 : -1 - lane;
+        if (!ok) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) w[k] = (R)0;
+        }
+    }
+
+    // ---- gather F = sum_k w_k (A0 + c dA_{s-1})[node_k]  (getFFelecs)
     R fr = 0, fi = 0;
     if (ok) {
-        // active box (nodes touched are cell .. cell+1)
-        const cplx<R>* A0 = a.A0;
-        const cplx<R>* dA = a.dAin;
         const R cA = (R)a.cin;
-        const R sr = (R)((double)chi * ph.inv_dV3 * fac * prd);
-        const R si = (R)((double)chi * ph.inv_dV3 * fac * pid);
-        if (m.one_d) {
-            // lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1)
-            const R w[2] = {(R)1 - c.fz, (R)1 - ((R)1 - c.fz)};
-            const int nd[2] = {c.iz0, c.iz1};
 #pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                cplx<R> v = A0[nd[k]];
+        for (int k = 0; k < 8; ++k) {
+            if (k < nnb) {
+                cplx<R> v = a.A0[nd[k]];
                 if (STAGE > 1) {
-                    const cplx<R> d = dA[nd[k]];
+                    const cplx<R> d = a.dAin[nd[k]];
                     v.x = fma(cA, d.x, v.x);
                     v.y = fma(cA, d.y, v.y);
                 }
                 fr = fma(w[k], v.x, fr);
                 fi = fma(w[k], v.y, fi);
-            }
-            if (ph.field_evolve) {
-#pragma unroll
-                for (int k = 0; k < 2; ++k) {
-                    atomic_add(&a.dAout[nd[k]].x, w[k] * sr);
-                    atomic_add(&a.dAout[nd[k]].y, w[k] * si);
-                }
-            }
-        } else {
-            const R x1 = (R)1 - c.fx, x2 = c.fx;
-            const R y1 = (R)1 - c.fy, y2 = c.fy;
-            const R z1 = (R)1 - c.fz, z2w = c.fz;
-            const R w[8] = {x1 * y1 * z1, x2 * y1 * z1, x1 * y2 * z1, x2 * y2 * z1,
-                            x1 * y1 * z2w, x2 * y1 * z2w, x1 * y2 * z2w, x2 * y2 * z2w};
-            const int64_t ntr = (int64_t)m.nx * m.ny;
-            const int64_t b0 = (int64_t)c.iz0 * ntr + (int64_t)c.iy * m.nx + c.ix;
-            const int64_t b1 = (int64_t)c.iz1 * ntr + (int64_t)c.iy * m.nx + c.ix;
-            const int64_t nd[8] = {b0, b0 + 1, b0 + m.nx, b0 + m.nx + 1,
-                                   b1, b1 + 1, b1 + m.nx, b1 + m.nx + 1};
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                cplx<R> v = A0[nd[k]];
-                if (STAGE > 1) {
-                    const cplx<R> d = dA[nd[k]];
-                    v.x = fma(cA, d.x, v.x);
-                    v.y = fma(cA, d.y, v.y);
-                }
-                fr = fma(w[k], v.x, fr);
-                fi = fma(w[k], v.y, fi);
-            }
-            if (ph.field_evolve) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    atomic_add(&a.dAout[nd[k]].x, w[k] * sr);
-                    atomic_add(&a.dAout[nd[k]].y, w[k] * si);
-                }
             }
         }
     }
-    // active box of this step (for the field combination)
-    {
-        int lx = ok ? c.ix : 0x7fffffff, hx = ok ? c.ix : -1;
-        int ly = ok ? c.iy : 0x7fffffff, hy = ok ? c.iy : -1;
-        int lz = ok ? c.iz0 : 0x7fffffff, hz = ok ? c.iz0 : -1;
-        lx = wave_min(lx); hx = wave_max(hx);
-        ly = wave_min(ly); hy = wave_max(hy);
-        lz = wave_min(lz); hz = wave_max(hz);
-        if ((threadIdx.x & 63) == 0 && hx >= 0) {
-            atomicMin(&a.box->lo[0], lx); atomicMax(&a.box->hi[0], hx);
-            atomicMin(&a.box->lo[1], ly); atomicMax(&a.box->hi[1], hy);
-            atomicMin(&a.box->lo[2], lz); atomicMax(&a.box->hi[2], hz);
+
+    // ---- deposit w_k s into dA_s (getSource)
+    if (ph.field_evolve) {
+        const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
+        const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
+        if (use_tile) {
+            // FP64 accumulation in an LDS tile, then one global atomic per
+            // touched node and component
+            __shared__ double s_tile[2 * TILE_NODES];
+            const int nt = tx * ty * tz;
+            for (int t = threadIdx.x; t < 2 * nt; t += blockDim.x) s_tile[t] = 0.0;
+            __syncthreads();
+            if (ok) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (k < nnb) {
+                        atomicAdd(&s_tile[2 * lt[k]], (double)(w[k] * sr));
+                        atomicAdd(&s_tile[2 * lt[k] + 1], (double)(w[k] * si));
+                    }
+                }
+            }
+            __syncthreads();
+            const int ntr = m.nx * m.ny;
+            for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+                const double vr = s_tile[2 * t], vi = s_tile[2 * t + 1];
+                if (vr != 0.0 || vi != 0.0) {
+                    const int lx = t % tx, r = t / tx;
+                    const int ly = r % ty, lz = r / ty;
+                    const int node = (bz0 + lz) * ntr + (by0 + ly) * m.nx + (bx0 + lx);
+                    atomic_add(&a.dAout[node].x, (R)vr);
+                    atomic_add(&a.dAout[node].y, (R)vi);
+                }
+            }
+        } else {
+            // wavefront reduction over runs of lanes sharing a cell, then one
+            // global atomic per run and corner
+            const int prev = __shfl_up(key, 1, 64);
+            const bool head = (lane == 0) || (prev != key);
+            const unsigned long long heads = __ballot(head);
+            const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+            const int seg_end = after ? lane + __builtin_ctzll(after) : 63;
+            const int maxlen = wave_max(head ? seg_end - lane + 1 : 1);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                if (k < nnb) {
+                    R vr = w[k] * sr, vi = w[k] * si;
+                    for (int o = 1; o < maxlen; o <<= 1) {
+                        const R tr = __shfl_down(vr, o, 64);
+                        const R ti = __shfl_down(vi, o, 64);
+                        if (lane + o <= seg_end) {
+                            vr += tr;
+                            vi += ti;
+                        }
+                    }
+                    if (ok && head) {
+                        atomic_add(&a.dAout[nd[k]].x, vr);
+                        atomic_add(&a.dAout[nd[k]].y, vi);
+                    }
+                }
+            }
         }
     }
     if (!in) return;
@@ -269,13 +359,28 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs<R> a) {
 
 // Start of a step: clear flags and the active box.
 __global__ void k_step_begin(int* flags, Box* box) {
-    if (threadIdx.x == 0) {
-        *flags = 0;
+    if (threadIdx.x == 0) *flags = 0;
+    for (int k = threadIdx.x; k < BOX_COPIES; k += blockDim.x) {
         for (int d = 0; d < 3; ++d) {
-            box->lo[d] = 0x7fffffff;
-            box->hi[d] = -1;
+            box[k].lo[d] = 0x7fffffff;
+            box[k].hi[d] = -1;
         }
     }
+}
+
+// Union of the spread box copies.
+__device__ __forceinline__ Box merged_box(const Box* box) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = 0x7fffffff;
+        b.hi[d] = -1;
+    }
+    for (int k = 0; k < BOX_COPIES; ++k)
+        for (int d = 0; d < 3; ++d) {
+            b.lo[d] = min(b.lo[d], box[k].lo[d]);
+            b.hi[d] = max(b.hi[d], box[k].hi[d]);
+        }
+    return b;
 }
 
 // A += h/6 (dA1 + dA4 + 2 (dA2 + dA3)) over the active box
@@ -283,7 +388,7 @@ __global__ void k_step_begin(int* flags, Box* box) {
 // The periodic mesh keeps its aliased last plane equal to plane 0.
 template <typename R>
 __global__ __launch_bounds__(256) void k_field_combine(CombineArgs<R> a) {
-    const Box b = *a.box;
+    const Box b = merged_box(a.box);
     if (b.hi[0] < 0) return;   // no particle deposited
     const int x0 = b.lo[0], x1 = min(b.hi[0] + 1, a.nx - 1);
     const int y0 = a.one_d ? 0 : b.lo[1], y1 = a.one_d ? 0 : min(b.hi[1] + 1, a.ny - 1);
@@ -334,7 +439,7 @@ __global__ void k_wrap_z2(double* z2, int64_t n, double lenz2) {
 
 template <typename R>
 void launch_stage(int stage, const StageArgs<R>& a, hipStream_t s) {
-    const int threads = 256;
+    const int threads = STAGE_THREADS;
     const int64_t blocks = (a.n + threads - 1) / threads;
     dim3 grid((unsigned)(blocks > 0 ? blocks : 1));
     switch (stage) {

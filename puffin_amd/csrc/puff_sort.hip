@@ -1,0 +1,223 @@
+// Counting sort of the macroparticles by mesh cell, so that the lanes of a
+// wavefront gather from and deposit to the same few cache lines.  The
+// reference keeps its particles in generation order; the permutation is kept
+// (orig[j] = reference index of slot j) and undone on download.
+#include "puff_common.hpp"
+#include "puff_kernels.hpp"
+
+namespace puff {
+
+// Linear cell index (node index of the cell's first corner) of each particle;
+// particles outside the mesh go to the last bin.
+template <typename R>
+__global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const double* z2,
+                                                   int64_t n, MeshGeom m, int ncell, int* key) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k = ncell;
+    const double zq = floor(z2[i] * m.inv_dz2);
+    int iz = -1;
+    if (m.periodic) {
+        if (fabs(zq) < 1.0e9) {
+            long long r = (long long)zq % m.nzp;
+            if (r < 0) r += m.nzp;
+            iz = (int)r;
+        }
+    } else if (zq >= 0.0 && zq <= (double)(m.nzp - 2)) {
+        iz = (int)zq;
+    }
+    if (iz >= 0) {
+        if (m.one_d) {
+            k = iz;
+        } else {
+            const R xq = floor((x[i] + (R)m.halfx) * (R)m.inv_dx);
+            const R yq = floor((y[i] + (R)m.halfy) * (R)m.inv_dy);
+            if (xq >= (R)0 && xq <= (R)(m.nx - 2) && yq >= (R)0 && yq <= (R)(m.ny - 2))
+                k = iz * m.nx * m.ny + (int)yq * m.nx + (int)xq;
+        }
+    }
+    key[i] = k;
+}
+
+__global__ void k_fill_int(int* p, int64_t n, int v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_iota(int* p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int)i;
+}
+
+__global__ __launch_bounds__(256) void k_histogram(const int* key, int64_t n, int* counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&counts[key[i]], 1);
+}
+
+// Exclusive scan in three passes over tiles of SCAN_TILE elements.
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* sh, int* total) {
+    // inclusive scan within the wavefront
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int s = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(s, o, 64);
+        if (lane >= o) s += t;
+    }
+    if (lane == 63) sh[wv] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < SCAN_THREADS / 64; ++k) {
+            const int t = sh[k];
+            sh[k] = acc;
+            acc += t;
+        }
+        sh[SCAN_THREADS / 64] = acc;
+    }
+    __syncthreads();
+    const int excl = s - v + sh[wv];
+    *total = sh[SCAN_THREADS / 64];
+    __syncthreads();
+    return excl;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tiles(const int* in, int64_t n, int* tile_sums) {
+    __shared__ int sh[SCAN_THREADS / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k)
+        if (base + k < n) s += in[base + k];
+    int total;
+    block_exclusive_scan(s, sh, &total);
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the tile sums in place.
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(int* sums, int64_t n) {
+    __shared__ int sh[SCAN_THREADS / 64 + 1];
+    int carry = 0;
+    for (int64_t off = 0; off < n; off += SCAN_TILE) {
+        const int64_t base = off + (int64_t)threadIdx.x * SCAN_ITEMS;
+        int v[SCAN_ITEMS];
+        int s = 0;
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            v[k] = (base + k < n) ? sums[base + k] : 0;
+            s += v[k];
+        }
+        int total;
+        int e = block_exclusive_scan(s, sh, &total) + carry;
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            if (base + k < n) sums[base + k] = e;
+            e += v[k];
+        }
+        carry += total;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* in, int64_t n, const int* tile_sums,
+                                                             int* out) {
+    __shared__ int sh[SCAN_THREADS / 64 + 1];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int total;
+    int e = block_exclusive_scan(s, sh, &total) + tile_sums[blockIdx.x];
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        if (base + k < n) out[base + k] = e;
+        e += v[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter_perm(const int* key, int64_t n, int* cursor, int* perm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) perm[atomicAdd(&cursor[key[i]], 1)] = (int)i;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather(const T* src, const int* perm, int64_t n, T* dst) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dst[j] = src[perm[j]];
+}
+
+// dst[orig[j]] = (double) src[j]: slot order back to the reference order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_unpermute(const T* src, const int* orig, int64_t n, double* dst) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dst[orig[j]] = (double)src[j];
+}
+
+// dst[j] = (T) src[orig[j]]: reference order into slot order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute_in(const double* src, const int* orig, int64_t n, T* dst) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) dst[j] = (T)src[orig[j]];
+}
+
+static dim3 g1(int64_t n) { return dim3((unsigned)(n > 0 ? (n + 255) / 256 : 1)); }
+
+template <typename R>
+void launch_sort(SortArgs<R>& a, hipStream_t s) {
+    const int64_t n = a.n;
+    const int64_t nb = (int64_t)a.ncell + 1;
+    hipLaunchKernelGGL((k_cell_keys<R>), g1(n), dim3(256), 0, s, a.y->x, a.y->y, a.y->z2, n, a.mesh, a.ncell,
+                       a.key);
+    hipLaunchKernelGGL(k_fill_int, g1(nb), dim3(256), 0, s, a.counts, nb, 0);
+    hipLaunchKernelGGL(k_histogram, g1(n), dim3(256), 0, s, a.key, n, a.counts);
+    const int64_t ntiles = (nb + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_THREADS), 0, s, a.counts, nb, a.tile_sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, a.tile_sums, ntiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)ntiles), dim3(SCAN_THREADS), 0, s, a.counts, nb,
+                       a.tile_sums, a.cursor);
+    hipLaunchKernelGGL(k_scatter_perm, g1(n), dim3(256), 0, s, a.key, n, a.cursor, a.perm);
+    // permute every coordinate into the spare arrays, then swap the pointers
+    PartPtrs<R>& y = *a.y;
+    PartPtrs<R>& t = *a.spare;
+    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.x, a.perm, n, t.x);
+    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.y, a.perm, n, t.y);
+    hipLaunchKernelGGL((k_gather<double>), g1(n), dim3(256), 0, s, y.z2, a.perm, n, t.z2);
+    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.pr, a.perm, n, t.pr);
+    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.pi, a.perm, n, t.pi);
+    hipLaunchKernelGGL((k_gather<double>), g1(n), dim3(256), 0, s, y.g, a.perm, n, t.g);
+    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, *a.chi, a.perm, n, *a.chi_spare);
+    hipLaunchKernelGGL((k_gather<int>), g1(n), dim3(256), 0, s, *a.orig, a.perm, n, *a.orig_spare);
+    std::swap(y.x, t.x);
+    std::swap(y.y, t.y);
+    std::swap(y.z2, t.z2);
+    std::swap(y.pr, t.pr);
+    std::swap(y.pi, t.pi);
+    std::swap(y.g, t.g);
+    std::swap(*a.chi, *a.chi_spare);
+    std::swap(*a.orig, *a.orig_spare);
+}
+
+void launch_iota(int* p, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_iota, g1(n), dim3(256), 0, s, p, n);
+}
+
+template <typename T>
+void launch_unpermute(const T* src, const int* orig, int64_t n, double* dst, hipStream_t s) {
+    hipLaunchKernelGGL((k_unpermute<T>), g1(n), dim3(256), 0, s, src, orig, n, dst);
+}
+
+template <typename T>
+void launch_permute_in(const double* src, const int* orig, int64_t n, T* dst, hipStream_t s) {
+    hipLaunchKernelGGL((k_permute_in<T>), g1(n), dim3(256), 0, s, src, orig, n, dst);
+}
+
+template void launch_sort<double>(SortArgs<double>&, hipStream_t);
+template void launch_sort<float>(SortArgs<float>&, hipStream_t);
+template void launch_unpermute<double>(const double*, const int*, int64_t, double*, hipStream_t);
+template void launch_unpermute<float>(const float*, const int*, int64_t, double*, hipStream_t);
+template void launch_permute_in<double>(const double*, const int*, int64_t, double*, hipStream_t);
+template void launch_permute_in<float>(const double*, const int*, int64_t, float*, hipStream_t);
+
+}  // namespace puff

@@ -37,6 +37,7 @@ SIGNATURES = {
     "puff_set_taper": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "puff_get_taper": (ctypes.c_int, [_vp, _dp, _dp]),
     "puff_set_taper_table": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double]),
+    "puff_set_sort_interval": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "puff_rk4par": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
     "puff_diffraction_step": (ctypes.c_int, [_vp, ctypes.c_double]),
     "puff_transform": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -195,6 +196,9 @@ class Integrator:
         v = np.ascontiguousarray(values, dtype=np.float64)
         _check(lib().puff_set_taper_table(self._h, v.ctypes.data_as(_dp), v.size, z0, period),
                "puff_set_taper_table")
+
+    def set_sort_interval(self, steps: int):
+        _check(lib().puff_set_sort_interval(self._h, steps), "puff_set_sort_interval")
 
     # ---- hot path
     def rk4par(self, sz: float, h: float) -> int:
