@@ -256,35 +256,51 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
         const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
         if (use_tile) {
             // FP64 accumulation in an LDS tile, then one global atomic per
-            // touched node and component
+            // touched node and component.  Each lane visits the corners in a
+            // rotated order so that lanes of one cell do not hit the same LDS
+            // address in the same instruction.
             __shared__ double s_tile[2 * TILE_NODES];
             const int nt = tx * ty * tz;
             for (int t = threadIdx.x; t < 2 * nt; t += blockDim.x) s_tile[t] = 0.0;
             __syncthreads();
             if (ok) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    if (k < nnb) {
-                        atomicAdd(&s_tile[2 * lt[k]], (double)(w[k] * sr));
-                        atomicAdd(&s_tile[2 * lt[k] + 1], (double)(w[k] * si));
+                const int txy = tx * ty;
+                const int t0 = lt[0];
+                const double dsr = (double)sr, dsi = (double)si;
+                const double wx[2] = {(double)((R)1 - c.fx), (double)c.fx};
+                const double wy[2] = {(double)((R)1 - c.fy), (double)c.fy};
+                const double wz[2] = {(double)((R)1 - c.fz), (double)c.fz};
+                for (int q = 0; q < nnb; ++q) {
+                    const int k = (q + lane) & (nnb - 1);
+                    double wk;
+                    int tk;
+                    if (m.one_d) {
+                        wk = (double)w[0];
+                        if (k) wk = (double)w[1];
+                        tk = t0 + k;
+                    } else {
+                        wk = (double)((R)wx[k & 1] * (R)wy[(k >> 1) & 1] * (R)wz[k >> 2]);
+                        tk = t0 + (k & 1) + ((k & 2) ? tx : 0) + ((k & 4) ? txy : 0);
                     }
+                    atomicAdd(&s_tile[2 * tk], wk * dsr);
+                    atomicAdd(&s_tile[2 * tk + 1], wk * dsi);
                 }
             }
             __syncthreads();
             const int ntr = m.nx * m.ny;
             for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-                const double vr = s_tile[2 * t], vi = s_tile[2 * t + 1];
-                if (vr != 0.0 || vi != 0.0) {
+                const double tr = s_tile[2 * t], ti = s_tile[2 * t + 1];
+                if (tr != 0.0 || ti != 0.0) {
                     const int lx = t % tx, r = t / tx;
                     const int ly = r % ty, lz = r / ty;
                     const int node = (bz0 + lz) * ntr + (by0 + ly) * m.nx + (bx0 + lx);
-                    atomic_add(&a.dAout[node].x, (R)vr);
-                    atomic_add(&a.dAout[node].y, (R)vi);
+                    atomic_add(&a.dAout[node].x, (R)tr);
+                    atomic_add(&a.dAout[node].y, (R)ti);
                 }
             }
         } else {
-            // wavefront reduction over runs of lanes sharing a cell, then one
-            // global atomic per run and corner
+            // Lanes of a wavefront that share a cell are summed first (runs of
+            // equal keys), then one global atomic per run and corner.
             const int prev = __shfl_up(key, 1, 64);
             const bool head = (lane == 0) || (prev != key);
             const unsigned long long heads = __ballot(head);

@@ -49,9 +49,37 @@ __global__ void k_iota(int* p, int64_t n) {
     if (i < n) p[i] = (int)i;
 }
 
+// Runs of equal keys within a wavefront (the input is in the previous sorted
+// order, so equal cells are mostly adjacent): the run's first lane does one
+// atomic for the whole run.
+struct Run {
+    int head_lane;
+    int len;
+    bool head;
+};
+
+__device__ __forceinline__ Run run_of(int k, bool valid) {
+    const int lane = threadIdx.x & 63;
+    const int kk = valid ? k : -1 - lane;
+    const int prev = __shfl_up(kk, 1, 64);
+    const bool head = (lane == 0) || (prev != kk);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long le = (lane == 63) ? heads : (heads & ((2ull << lane) - 1ull));
+    const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+    Run r;
+    r.head = head;
+    r.head_lane = 63 - __builtin_clzll(le);
+    const int seg_end = after ? lane + __builtin_ctzll(after) : 63;
+    r.len = seg_end - lane + 1;   // meaningful on the head lane
+    return r;
+}
+
 __global__ __launch_bounds__(256) void k_histogram(const int* key, int64_t n, int* counts) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&counts[key[i]], 1);
+    const bool valid = i < n;
+    const int k = valid ? key[i] : 0;
+    const Run r = run_of(k, valid);
+    if (valid && r.head) atomicAdd(&counts[k], r.len);
 }
 
 // Exclusive scan in three passes over tiles of SCAN_TILE elements.
@@ -139,7 +167,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* in, int6
 
 __global__ __launch_bounds__(256) void k_scatter_perm(const int* key, int64_t n, int* cursor, int* perm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) perm[atomicAdd(&cursor[key[i]], 1)] = (int)i;
+    const bool valid = i < n;
+    const int k = valid ? key[i] : 0;
+    const Run r = run_of(k, valid);
+    int base = 0;
+    if (valid && r.head) base = atomicAdd(&cursor[k], r.len);
+    base = __shfl(base, r.head_lane, 64);
+    const int lane = threadIdx.x & 63;
+    if (valid) perm[base + (lane - r.head_lane)] = (int)i;
 }
 
 template <typename T>
