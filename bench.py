@@ -22,6 +22,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 
+WORKLOADS = {
+    "c1": "planar tapered FEL, temporal mesh 129x129x1024, 2^24 macroparticles",
+    "c0": "elliptical (helical-like) FEL, periodic mesh 33x33x128, 2^18 macroparticles",
+    "c3": "elliptical FEL with per-period a_w, periodic mesh 33x33x256, 2^20 macroparticles",
+}
+
 
 def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
@@ -113,6 +119,30 @@ def roofline(case, precision: int, res: dict, steps: int):
     return kernels, dom, n_act, step_bytes
 
 
+def measured_traffic(precision: int, kernel: str):
+    """HBM bytes per dispatch of a stage kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/hbm_traffic_c1_*.md), scaled by
+    the copy-kernel calibration recorded there; None when not measured."""
+    name = "fp64" if precision == 0 else "mixed"
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_c1_{name}.md")
+    if not os.path.exists(path) or not kernel.startswith("stage"):
+        return None
+    typ = "double" if precision == 0 else "float"
+    want = f"`k_stage<{typ}, {kernel[-1]}>`"
+    sf = sw = None
+    row = None
+    for line in open(path):
+        if line.startswith("SCALE"):
+            parts = dict(p.split("=") for p in line.split()[1:])
+            sf, sw = float(parts["fetch"]), float(parts["write"])
+        if line.startswith("| " + want):
+            cols = [c.strip() for c in line.strip().strip("|").split("|")]
+            row = (float(cols[2]), float(cols[3]))
+    if row is None or sf is None:
+        return None
+    return (row[0] * sf + row[1] * sw) * 1024.0
+
+
 def integrated_power(power: np.ndarray, dz2: float) -> float:
     return float(np.trapezoid(power, dx=dz2)) if hasattr(np, "trapezoid") else float(np.trapz(power, dx=dz2))
 
@@ -150,6 +180,19 @@ def cpu_baseline(case, nsample: int, steps: int):
                         f"(no diffraction), FP64 C oracle with OpenMP atomics"))
 
 
+def calibrate_copy(mib: int):
+    """A plain device-to-device copy of known size (reads and writes mib MiB
+    each, 5 times), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB 5 times")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,15 +203,22 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--case", choices=("c1", "c0", "c3"), default="c1",
+                    help="workload (the JSON line is quoted on c1)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(z2 slab decomposition is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
 
-    from puffin_amd.configs import case_c1
+    from puffin_amd.configs import CASES
     t = time.perf_counter()
-    case = case_c1(n=args.particles)
+    case = CASES[args.case](n=args.particles) if args.case == "c1" else CASES[args.case]()
     log(f"workload generated in {time.perf_counter() - t:.1f} s")
 
     results = {}
@@ -185,7 +235,8 @@ def main():
             value=case.n / (r["ms_per_step"] * 1e-3),
             ms_per_step=r["ms_per_step"],
             roofline=dict(bound="hbm", kernel=dom, achieved=k["GBps"], peak=HBM_PEAK_GBS, unit="GB/s",
-                          frac=k["frac"], traffic=None),
+                          frac=k["frac"], traffic=measured_traffic(prec, dom) if case.n == 1 << 24 and
+                          args.case == "c1" else None),
             step_roofline=dict(bytes_per_step=sbytes,
                                achieved=sbytes / (r["ms_per_step"] * 1e-3) / 1e9,
                                frac=sbytes / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS),
@@ -210,7 +261,8 @@ def main():
         vs_baseline=None,
         dtype="f64" if prim_name == "fp64" else "f32-mixed",
         data="synthetic (seeded Philox beam and Gaussian seed field)",
-        config=dict(workload="planar tapered FEL, temporal mesh 129x129x1024, 2^24 macroparticles",
+        config=dict(workload=WORKLOADS.get(args.case, case.name) if args.case != "c1" or case.n == 1 << 24
+                    else f"{WORKLOADS['c1']} (reduced to {case.n} macroparticles)",
                     mesh=[case.mp.nx, case.mp.ny, case.mp.nz2], macroparticles=case.n,
                     steps_per_diffraction=case.steps4diff, rho=case.mp.rho,
                     precision=prim_name),

@@ -1,0 +1,278 @@
+! Fortran binding of the gfx950 integrator's C ABI (include/puffin_gpu.h)
+! through iso_c_binding, and drop-in replacements of the reference's
+! rk4par / diffractIM / DiffractionStep that keep its call signatures and its
+! module-global, scaled-variable arrays:
+!
+!   rk4par(sZ, h, qD)                  interaction/puffin_mpi_RK4.f90:38-59
+!   diffractIM(sStep, qDiffrctd, qOK)  diffraction/diffraction.f90:46-52
+!   DiffractionStep(h, sAr, sAi, qOK)  diffraction/diffraction.f90:209-220
+!   allact_rk4_arrs / deallact_rk4_arrs  interaction/puffin_mpi_RK4.f90:374-460
+!
+! The particle arrays sElX_G .. sElGam_G, s_chi_bar_G (deriv_globals.f90:94,
+! 128-133) and the field ac_rfield_in / ac_ifield_in (puffin_mpi_RK4.f90:24,
+! whole mesh here, x fastest) stay resident on the device between
+! allact_rk4_arrs and deallact_rk4_arrs; the flags qPArrOK_G / qInnerXYOK_G
+! (deriv_globals.f90:407-409) are set from the status code of each step.
+
+module puffin_gpu_capi
+
+  use iso_c_binding
+  implicit none
+
+  integer(c_int32_t), parameter :: PUFF_FP64 = 0, PUFF_MIXED = 1
+  integer(c_int32_t), parameter :: PUFF_MEM_HOST = 0, PUFF_MEM_DEVICE = 1
+  integer(c_int), parameter :: PUFF_OK = 0, PUFF_OUT_TRANSVERSE = 1, &
+                               PUFF_OUT_Z2_BUFFER = 2, PUFF_OUT_Z2_MESH = 3
+
+  ! puff_params (include/puffin_gpu.h)
+  type, bind(C) :: puff_params_t
+    integer(c_int32_t) :: nx, ny, nz2, periodic, one_d, und_type, und_ends, &
+                          focusing, electrons_evolve, field_evolve, coupling, &
+                          filter, nbx, nby, nbz2, reserved
+    real(c_double) :: dx, dy, dz2, rho, eta, kappa, gamma_r, aw, fx, fy, &
+                      szfs, szfe, kbeta_x, kbeta_y, kx_und, ky_und, beta, &
+                      filt_cutoff, pi_ref
+  end type puff_params_t
+
+  interface
+
+    function puff_device_count() bind(C, name="puff_device_count") result(r)
+      import :: c_int
+      integer(c_int) :: r
+    end function
+
+    function puff_last_error() bind(C, name="puff_last_error") result(r)
+      import :: c_ptr
+      type(c_ptr) :: r
+    end function
+
+    function puff_create(ctx, params, n, precision) bind(C, name="puff_create") result(r)
+      import :: c_ptr, c_int, c_int64_t, c_int32_t, puff_params_t
+      type(c_ptr), intent(out) :: ctx
+      type(puff_params_t), intent(in) :: params
+      integer(c_int64_t), value :: n
+      integer(c_int32_t), value :: precision
+      integer(c_int) :: r
+    end function
+
+    function puff_destroy(ctx) bind(C, name="puff_destroy") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: r
+    end function
+
+    function puff_upload_particles(ctx, kind, x, y, z2, px, py, gam, chi) &
+        bind(C, name="puff_upload_particles") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kind
+      real(c_double), intent(in) :: x(*), y(*), z2(*), px(*), py(*), gam(*), chi(*)
+      integer(c_int) :: r
+    end function
+
+    function puff_download_particles(ctx, kind, x, y, z2, px, py, gam) &
+        bind(C, name="puff_download_particles") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kind
+      real(c_double), intent(out) :: x(*), y(*), z2(*), px(*), py(*), gam(*)
+      integer(c_int) :: r
+    end function
+
+    function puff_upload_field(ctx, kind, ar, ai) bind(C, name="puff_upload_field") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kind
+      real(c_double), intent(in) :: ar(*), ai(*)
+      integer(c_int) :: r
+    end function
+
+    function puff_download_field(ctx, kind, ar, ai) bind(C, name="puff_download_field") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: kind
+      real(c_double), intent(out) :: ar(*), ai(*)
+      integer(c_int) :: r
+    end function
+
+    function puff_set_taper(ctx, n2col, n2col0, undgrad) bind(C, name="puff_set_taper") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: n2col, n2col0, undgrad
+      integer(c_int) :: r
+    end function
+
+    function puff_get_taper(ctx, n2col, n2col0) bind(C, name="puff_get_taper") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: n2col, n2col0
+      integer(c_int) :: r
+    end function
+
+    function puff_rk4par(ctx, sz, h) bind(C, name="puff_rk4par") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: sz, h
+      integer(c_int) :: r
+    end function
+
+    function puff_diffraction_step(ctx, h) bind(C, name="puff_diffraction_step") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: h
+      integer(c_int) :: r
+    end function
+
+    function puff_wrap_z2(ctx) bind(C, name="puff_wrap_z2") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: r
+    end function
+
+    function puff_power(ctx, power) bind(C, name="puff_power") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: power(*)
+      integer(c_int) :: r
+    end function
+
+  end interface
+
+end module puffin_gpu_capi
+
+
+module puffin_gpu
+
+  use iso_c_binding
+  use puffin_gpu_capi
+  implicit none
+
+  integer, parameter :: wp = c_double
+
+  ! module constants of the current undulator module (filled by the caller as
+  ! setup / initUndulator leave them)
+  type(puff_params_t), save :: tParams_G
+  integer(c_int32_t), save :: iPrecision_G = PUFF_FP64
+
+  ! macroparticles (deriv_globals.f90:94,128-133)
+  real(wp), allocatable, save :: sElX_G(:), sElY_G(:), sElZ2_G(:), &
+                                 sElPX_G(:), sElPY_G(:), sElGam_G(:), s_chi_bar_G(:)
+  ! field on the whole mesh (puffin_mpi_RK4.f90:24)
+  real(wp), allocatable, save :: ac_rfield_in(:), ac_ifield_in(:)
+  ! taper state (wiggler_taper.f90)
+  real(wp), save :: n2col = 1.0_wp, n2col0 = 1.0_wp, undgrad = 0.0_wp
+  ! parallel-layout flags (deriv_globals.f90:407-409)
+  logical, save :: qPArrOK_G = .true., qInnerXYOK_G = .true.
+  ! fatal temporal-mesh overflow (the reference STOPs)
+  logical, save :: qZ2MeshOK_G = .true.
+
+  type(c_ptr), save :: ctx_G = c_null_ptr
+
+contains
+
+  subroutine gpu_check(r, where)
+    integer(c_int), intent(in) :: r
+    character(*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: i
+    if (r < 0) then
+      call c_f_pointer(puff_last_error(), msg, [512])
+      write(*, '(a)', advance='no') 'puffin_gpu: '//where//' failed: '
+      do i = 1, 512
+        if (msg(i) == c_null_char) exit
+        write(*, '(a)', advance='no') msg(i)
+      end do
+      write(*, *)
+      error stop 2
+    end if
+  end subroutine gpu_check
+
+  ! Upload the module globals to the device (replaces allact_rk4_arrs,
+  ! puffin_mpi_RK4.f90:374-418, and its outer2Inner copy).
+  subroutine allact_rk4_arrs()
+    integer(c_int64_t) :: n
+    if (c_associated(ctx_G)) call deallact_rk4_arrs()
+    n = int(size(sElX_G), c_int64_t)
+    call gpu_check(puff_create(ctx_G, tParams_G, n, iPrecision_G), 'puff_create')
+    call gpu_check(puff_upload_particles(ctx_G, PUFF_MEM_HOST, sElX_G, sElY_G, sElZ2_G, &
+                   sElPX_G, sElPY_G, sElGam_G, s_chi_bar_G), 'puff_upload_particles')
+    call gpu_check(puff_upload_field(ctx_G, PUFF_MEM_HOST, ac_rfield_in, ac_ifield_in), &
+                   'puff_upload_field')
+    call gpu_check(puff_set_taper(ctx_G, n2col, n2col0, undgrad), 'puff_set_taper')
+    qInnerXYOK_G = .true.
+    qPArrOK_G = .true.
+  end subroutine allact_rk4_arrs
+
+  ! Copy the device state back into the module globals.
+  subroutine gpu_sync_to_host()
+    call gpu_check(puff_download_particles(ctx_G, PUFF_MEM_HOST, sElX_G, sElY_G, sElZ2_G, &
+                   sElPX_G, sElPY_G, sElGam_G), 'puff_download_particles')
+    call gpu_check(puff_download_field(ctx_G, PUFF_MEM_HOST, ac_rfield_in, ac_ifield_in), &
+                   'puff_download_field')
+    call gpu_check(puff_get_taper(ctx_G, n2col, n2col0), 'puff_get_taper')
+  end subroutine gpu_sync_to_host
+
+  ! Download and release (replaces deallact_rk4_arrs, puffin_mpi_RK4.f90:423-460).
+  subroutine deallact_rk4_arrs()
+    integer(c_int) :: r
+    if (.not. c_associated(ctx_G)) return
+    call gpu_sync_to_host()
+    r = puff_destroy(ctx_G)
+    ctx_G = c_null_ptr
+  end subroutine deallact_rk4_arrs
+
+  ! One RK4 step of particles and field (rk4par, puffin_mpi_RK4.f90:38-366).
+  ! qD is vestigial in the reference (only reset, :154-166).
+  subroutine rk4par(sZ, h, qD)
+    real(wp), intent(in) :: sZ
+    real(wp), intent(in) :: h
+    logical, intent(inout) :: qD
+    integer(c_int) :: r
+    qD = .false.
+    r = puff_rk4par(ctx_G, sZ, h)
+    call gpu_check(r, 'puff_rk4par')
+    select case (r)
+    case (PUFF_OUT_TRANSVERSE)
+      qInnerXYOK_G = .false.
+      qPArrOK_G = .false.
+    case (PUFF_OUT_Z2_BUFFER)
+      qPArrOK_G = .false.
+    case (PUFF_OUT_Z2_MESH)
+      qZ2MeshOK_G = .false.
+      qPArrOK_G = .false.
+    end select
+  end subroutine rk4par
+
+  ! Diffract the device field over sStep (diffractIM, diffraction.f90:46-91).
+  subroutine diffractIM(sStep, qDiffrctd, qOK)
+    real(wp), intent(in) :: sStep
+    logical, intent(out) :: qDiffrctd, qOK
+    call gpu_check(puff_diffraction_step(ctx_G, sStep), 'puff_diffraction_step')
+    qDiffrctd = .true.
+    qOK = .true.
+  end subroutine diffractIM
+
+  ! Diffract a host field held in split real / imaginary arrays
+  ! (DiffractionStep, diffraction.f90:209-348); uses the module's context.
+  subroutine DiffractionStep(h, sAr, sAi, qOK)
+    real(wp), intent(in) :: h
+    real(wp), intent(inout) :: sAr(:), sAi(:)
+    logical, intent(out) :: qOK
+    call gpu_check(puff_upload_field(ctx_G, PUFF_MEM_HOST, sAr, sAi), 'puff_upload_field')
+    call gpu_check(puff_diffraction_step(ctx_G, h), 'puff_diffraction_step')
+    call gpu_check(puff_download_field(ctx_G, PUFF_MEM_HOST, sAr, sAi), 'puff_download_field')
+    qOK = .true.
+  end subroutine DiffractionStep
+
+  ! Periodic z2 wrap done by getLocalFieldIndices (para_field.f90:212-215).
+  subroutine wrap_z2()
+    call gpu_check(puff_wrap_z2(ctx_G), 'puff_wrap_z2')
+  end subroutine wrap_z2
+
+  ! Power per z2 node of the device field (gPowerP, av_write.f90:39-64).
+  subroutine gPowerP(power)
+    real(wp), intent(out) :: power(:)
+    call gpu_check(puff_power(ctx_G, power), 'puff_power')
+  end subroutine gPowerP
+
+end module puffin_gpu

@@ -1,0 +1,120 @@
+! One undulator module driven from Fortran through the drop-in routines of
+! module puffin_gpu: the split-step schedule of UndSection
+! (undulator.f90:103-333) with rk4par, diffractIM, the periodic re-layout and
+! the power writes.
+!
+! usage: und_section <input file> <output file>
+!
+! Input (stream, native byte order): the puff_params record, then
+!   int64  n, nsteps, steps4diff, redist_every, diffraction, periodic, write_every
+!   real64 h, n2col, n2col0, undgrad
+!   real64 x(n), y(n), z2(n), px(n), py(n), gam(n), chi(n), ar(nmesh), ai(nmesh)
+! Output: int64 nwrites, then real64 x, y, z2, px, py, gam (n each), ar, ai
+! (nmesh each), power(nz2, nwrites), and the int64 final status.
+
+program und_section
+
+  use iso_c_binding
+  use puffin_gpu_capi
+  use puffin_gpu
+  implicit none
+
+  character(len=4096) :: fin, fout
+  integer(c_int64_t) :: n, nsteps, steps4diff, redist_every, idiff, iper, write_every
+  integer(c_int64_t) :: nmesh, istep, iCsteps, nw, status
+  real(wp) :: h, szl, dzdS, dzdF, dzd
+  real(wp), allocatable :: powers(:,:), pw(:)
+  logical :: qD, qDiffrctd, qOK, qWrite, qDone
+  integer :: u
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  if (puff_device_count() < 1) then
+    write(*, *) 'und_section: no HIP device'
+    error stop 1
+  end if
+
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) tParams_G
+  read(u) n, nsteps, steps4diff, redist_every, idiff, iper, write_every
+  read(u) h, n2col, n2col0, undgrad
+  nmesh = int(tParams_G%nx, c_int64_t) * tParams_G%ny * tParams_G%nz2
+  allocate(sElX_G(n), sElY_G(n), sElZ2_G(n), sElPX_G(n), sElPY_G(n), sElGam_G(n), s_chi_bar_G(n))
+  allocate(ac_rfield_in(nmesh), ac_ifield_in(nmesh))
+  read(u) sElX_G, sElY_G, sElZ2_G, sElPX_G, sElPY_G, sElGam_G, s_chi_bar_G
+  read(u) ac_rfield_in, ac_ifield_in
+  close(u)
+
+  allocate(powers(tParams_G%nz2, max(1_c_int64_t, nsteps)), pw(tParams_G%nz2))
+  nw = 0
+  status = 0
+  iCsteps = 0
+  szl = 0.0_wp
+  qD = .true.
+
+  call allact_rk4_arrs()
+  if (iper /= 0) call wrap_z2()                       ! undulator.f90:103
+  dzdS = 0.0_wp
+  if (idiff /= 0) then                                 ! undulator.f90:156-164
+    dzdS = real(steps4diff, wp) * h / 2.0_wp
+    call diffractIM(dzdS, qDiffrctd, qOK)
+  end if
+
+  istep = 0
+  do
+    istep = istep + 1
+    if (istep > nsteps) exit
+    iCsteps = iCsteps + 1
+    call rk4par(szl, h, qD)                           ! undulator.f90:199
+    if (.not. qPArrOK_G) then
+      status = 2
+      if (.not. qInnerXYOK_G) status = 1
+      if (.not. qZ2MeshOK_G) status = 3
+      exit
+    end if
+    szl = szl + h                                      ! undulator.f90:222
+    qWrite = (write_every > 0) .and. ((mod(iCsteps, write_every) == 0) .or. (iCsteps == nsteps))
+    qDone = .false.
+    if (idiff /= 0) then
+      if ((mod(istep, steps4diff) == 0) .or. (istep == nsteps)) then   ! :229-289
+        dzdF = dzdS
+        if (istep == nsteps) then
+          dzdS = 0.0_wp
+        else if ((istep + steps4diff) <= nsteps) then
+          dzdS = real(steps4diff, wp) * h / 2
+        else
+          dzdS = real(nsteps - istep, wp) * h / 2
+        end if
+        if (.not. qWrite) then
+          dzd = dzdF + dzdS
+          call diffractIM(dzd, qDiffrctd, qOK)
+        else
+          call diffractIM(dzdF, qDiffrctd, qOK)
+          nw = nw + 1
+          call gPowerP(pw)
+          powers(:, nw) = pw
+          qDone = .true.
+          if (dzdS > 0.0_wp) call diffractIM(dzdS, qDiffrctd, qOK)
+        end if
+      end if
+    end if
+    if (qWrite .and. .not. qDone) then
+      nw = nw + 1
+      call gPowerP(pw)
+      powers(:, nw) = pw
+    end if
+    if (iper /= 0 .and. redist_every > 0) then        ! :325-331
+      if (mod(iCsteps, redist_every) == 0) call wrap_z2()
+    end if
+  end do
+  call deallact_rk4_arrs()
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) nw
+  write(u) sElX_G, sElY_G, sElZ2_G, sElPX_G, sElPY_G, sElGam_G
+  write(u) ac_rfield_in, ac_ifield_in
+  if (nw > 0) write(u) powers(:, 1:nw)
+  write(u) status
+  close(u)
+
+end program und_section
