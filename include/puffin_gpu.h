@@ -196,10 +196,12 @@ int puff_particle_count(puff_ctx* ctx, int64_t* n);
  * reference's wall-clock prints per step and per diffraction phase
  * (Get_time, undulator.f90:316-321; diffraction.f90:234-320).
  * Kinds: 0..3 RK4 stage kernels 1..4, 4 field combination, 5 diffraction
- * (all passes), 6 step start.  ms[k] is the summed duration, counts[k] the
- * number of launches since the last reset; nkinds <= PUFF_PROF_KINDS.
+ * (all passes), 6 step start, 7 adjoint stage kernels (all four), 8 adjoint
+ * field combination, 9 adjoint diffraction, 10 forward stages replayed by
+ * the adjoint step.  ms[k] is the summed duration, counts[k] the number of
+ * launches since the last reset; nkinds <= PUFF_PROF_KINDS.
  */
-#define PUFF_PROF_KINDS 7
+#define PUFF_PROF_KINDS 11
 int puff_profile_enable(puff_ctx* ctx, int32_t on);
 int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds,
                       int32_t reset);
@@ -207,6 +209,57 @@ int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds
 /* Active box of the last step in cells (x0, x1, y0, y1, z0, z1), the region
    whose nodes the deposit touched. */
 int puff_active_box(puff_ctx* ctx, int32_t* box6);
+
+/*
+ * State snapshots.  No reference counterpart: the reference resumes from
+ * HDF5 dumps (io/hdf5_puff.f90), which is out of this library's scope.  A
+ * blob holds the particles in the device's slot order, the field and the
+ * taper and sort state; it lives in caller-owned device memory of
+ * puff_state_bytes bytes (e.g. a torch uint8 CUDA tensor).  Used to take
+ * checkpoints for the adjoint's recomputation.
+ */
+int puff_state_bytes(puff_ctx* ctx, int64_t* bytes);
+int puff_state_save(puff_ctx* ctx, void* device_dst);
+int puff_state_load(puff_ctx* ctx, const void* device_src);
+/* Sort the macroparticles by mesh cell now (otherwise done every
+   puff_set_sort_interval steps inside puff_rk4par).  The order is internal:
+   downloads always return the caller's order. */
+int puff_sort_now(puff_ctx* ctx);
+
+/*
+ * Discrete adjoint of the step.  No reference counterpart (the reference
+ * computes no derivatives): the exact transpose of puff_rk4par and
+ * puff_diffraction_step as implemented here, for reverse-mode gradients of
+ * a scalar output (e.g. the integrated power) through a sequence of steps.
+ *
+ * The context holds a cotangent of the state (particles and field) beside
+ * the state.  puff_adjoint_enable allocates it (zeroed); puff_adjoint_zero
+ * clears it; puff_adjoint_seed_power adds scale * d/dA of
+ * sum_iz wz[iz] power(iz) at the current field (power as puff_power,
+ * wz host array of NZ2 weights).
+ *
+ * puff_rk4par_adjoint: the current state must be the state at the start of
+ * the step (as puff_state_load restores it) and the cotangent that of the
+ * state at the end of the step.  It replaces the cotangent by that of the
+ * start state, leaves the state unchanged, and returns dn2col[k] =
+ * d(output)/d(n2col of RK4 stage k) and slot[k] = the per-period taper
+ * table slot of stage k (-1 without a table).  Status as puff_rk4par.
+ * puff_diffraction_adjoint applies the transpose of one diffraction step of
+ * length h to the field cotangent.
+ *
+ * puff_adjoint_upload / _download exchange the cotangent in the caller's
+ * particle order and split field layout (either field pointer may be null).
+ */
+int puff_adjoint_enable(puff_ctx* ctx);
+int puff_adjoint_zero(puff_ctx* ctx);
+int puff_adjoint_seed_power(puff_ctx* ctx, const double* wz_host, double scale);
+int puff_rk4par_adjoint(puff_ctx* ctx, double sz, double h, double* dn2col, int32_t* slot);
+int puff_diffraction_adjoint(puff_ctx* ctx, double h);
+int puff_adjoint_upload(puff_ctx* ctx, int32_t mem_kind, const double* x, const double* y,
+                        const double* z2, const double* px, const double* py,
+                        const double* gam, const double* ar, const double* ai);
+int puff_adjoint_download(puff_ctx* ctx, int32_t mem_kind, double* x, double* y, double* z2,
+                          double* px, double* py, double* gam, double* ar, double* ai);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,16 @@ def lib():
         l.orc_match.argtypes = [P, ctypes.c_double, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp,
                                 ctypes.c_double, ctypes.c_int]
         l.orc_set_deposit_atomic.argtypes = [ctypes.c_int]
+        l.orc_set_taper_table.argtypes = [_dp, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        l.orc_taper_slot.restype = ctypes.c_int
+        l.orc_taper_slot.argtypes = [ctypes.c_double]
+        l.orc_bfields_grad.argtypes = [P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, _dp]
+        l.orc_rk4par_adjoint.restype = ctypes.c_int
+        l.orc_rk4par_adjoint.argtypes = ([P, _dp, ctypes.c_int64] + [_dp] * 9 + [_dp] * 8 +
+                                         [ctypes.c_double, ctypes.c_double, _dp, ctypes.c_int])
+        l.orc_diffraction_adjoint.restype = ctypes.c_int
+        l.orc_diffraction_adjoint.argtypes = [P, _dp, _dp, ctypes.c_double, ctypes.c_int]
         l.orc_correct_trans.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
         l.orc_und_section.restype = ctypes.c_int
         l.orc_und_section.argtypes = ([P, _dp, ctypes.c_int64] + [_dp] * 9 +
@@ -232,3 +242,56 @@ def correct_trans(parts: dict):
     n = parts["x"].size
     lib().orc_correct_trans(n, _ptr(parts["x"]), _ptr(parts["y"]), _ptr(parts["px"]),
                             _ptr(parts["py"]), _ptr(parts["chi"]))
+
+
+# ---- taper table and the discrete adjoint ------------------------------------
+
+def set_taper_table(values, z0: float = 0.0, period: float = 1.0):
+    """Per-period n2col table used in place of the linear taper (None clears)."""
+    if values is None:
+        lib().orc_set_taper_table(None, 0, 0.0, 1.0)
+        return
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    lib().orc_set_taper_table(_ptr(v), v.size, z0, period)
+
+
+def taper_slot(sz: float) -> int:
+    return lib().orc_taper_slot(sz)
+
+
+def bfields_grad(mp: ModuleParams, place: int, sz: float, x: float, y: float):
+    p = mp.to_ctypes()
+    g = np.zeros(6)
+    lib().orc_bfields_grad(ctypes.byref(p), place, sz, x, y, _ptr(g))
+    return g
+
+
+ADJ_KEYS = ("x", "y", "z2", "px", "py", "gam")
+
+
+def rk4par_adjoint(mp: ModuleParams, taper, parts: dict, ar, ai, lam: dict, lar, lai,
+                   sz: float, h: float, nthreads: int = 1):
+    """Cotangent of one rk4par step pulled back to the start of the step.
+
+    parts / ar / ai: the state at the start of the step (unchanged).
+    lam / lar / lai: cotangent of the end state, replaced in place by the
+    cotangent of the start state.  Returns (status, dJ/dn2col per stage)."""
+    p = mp.to_ctypes()
+    t = np.ascontiguousarray(taper, dtype=np.float64)
+    n = parts["x"].size
+    P = [np.ascontiguousarray(parts[k], dtype=np.float64) for k in ADJ_KEYS + ("chi",)]
+    A = [np.ascontiguousarray(ar, dtype=np.float64), np.ascontiguousarray(ai, dtype=np.float64)]
+    for k in ADJ_KEYS:
+        assert lam[k].dtype == np.float64 and lam[k].flags["C_CONTIGUOUS"]
+    dn = np.zeros(4)
+    st = lib().orc_rk4par_adjoint(ctypes.byref(p), _ptr(t), n, *[_ptr(a) for a in P],
+                                  *[_ptr(a) for a in A], *[_ptr(lam[k]) for k in ADJ_KEYS],
+                                  _ptr(lar), _ptr(lai), sz, h, _ptr(dn), nthreads)
+    return st, dn
+
+
+def diffraction_adjoint(mp: ModuleParams, lr: np.ndarray, li: np.ndarray, h: float,
+                        nthreads: int = 1) -> int:
+    """Apply the adjoint of one diffraction step to the cotangent, in place."""
+    p = mp.to_ctypes()
+    return lib().orc_diffraction_adjoint(ctypes.byref(p), _ptr(lr), _ptr(li), h, nthreads)

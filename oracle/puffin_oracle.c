@@ -55,8 +55,41 @@ double orc_get_gamma(double p2, double pr, double pi, double eta,
 /* ------------------------------------------------------------------------ */
 /* Taper: getAlpha, wiggler_taper.f90:39-45 (with its n2col0 side effect)    */
 /* ------------------------------------------------------------------------ */
+/* Per-period taper table (no reference counterpart: the reference has only the
+   linear taper).  When set, n2col at zbar is values[floor((zbar - z0) /
+   period)], clamped to the table, in place of getAlpha. */
+static double* g_tt = NULL;
+static int g_tt_n = 0;
+static double g_tt_z0 = 0.0, g_tt_period = 1.0;
+
+void orc_set_taper_table(const double* values, int nvals, double z0, double period) {
+    free(g_tt);
+    g_tt = NULL;
+    g_tt_n = 0;
+    if (nvals > 0 && values) {
+        g_tt = (double*)malloc(sizeof(double) * nvals);
+        memcpy(g_tt, values, sizeof(double) * nvals);
+        g_tt_n = nvals;
+        g_tt_z0 = z0;
+        g_tt_period = period;
+    }
+}
+
+/* Table slot of zbar (clamped), or -1 without a table. */
+int orc_taper_slot(double sz) {
+    if (g_tt_n <= 0) return -1;
+    long long k = (long long)floor((sz - g_tt_z0) / g_tt_period);
+    if (k < 0) k = 0;
+    if (k >= g_tt_n) k = g_tt_n - 1;
+    return (int)k;
+}
+
 static double orc_alpha(const puff_params* p, double* taper /* n2col, n2col0, undgrad */,
                         double sz) {
+    if (g_tt_n > 0) {
+        taper[0] = g_tt[orc_taper_slot(sz)];
+        return taper[0];
+    }
     if (sz >= p->szfs && sz <= p->szfe) {
         taper[0] = taper[1] + taper[2] * (sz - p->szfs);
     } else if (sz > p->szfe) {
@@ -751,6 +784,12 @@ static void orc_maskf4(const double* x, double dx, int nx, int nb, double pi,
    Maskf3_y(iy) Mask1D2_x(ix) + Maskf4_x(ix) Mask1D2_y(iy), Maskf3 = 1. */
 void orc_mask_xy(const puff_params* p, double* mask) {
     int nx = p->nx, ny = p->ny;
+    /* a boundary band wider than the mesh would index outside the
+       reference's arrays: no mask then (as the HIP side) */
+    if (!(nx > 1 && ny > 1 && p->nbx > 1 && p->nby > 1 && nx >= p->nbx && ny >= p->nby)) {
+        for (int64_t j = 0; j < (int64_t)nx * ny; j++) mask[j] = 0.0;
+        return;
+    }
     double* x = (double*)malloc(sizeof(double) * nx);
     double* y = (double*)malloc(sizeof(double) * ny);
     double* mx = (double*)malloc(sizeof(double) * nx);
@@ -775,7 +814,7 @@ void orc_mask_xy(const puff_params* p, double* mask) {
    (evaluates to nx - dx); zero on a periodic mesh (diffraction.f90:447). */
 void orc_mask_z2(const puff_params* p, double* mz) {
     int nzp = p->periodic ? p->nz2 - 1 : p->nz2;
-    if (p->periodic) {
+    if (p->periodic || p->nbz2 <= 1) {   /* nbz2 = 1 would divide by lb = 0 */
         for (int i = 0; i < nzp; i++) mz[i] = 0.0;
         return;
     }
@@ -1103,4 +1142,424 @@ int orc_und_section(const puff_params* p, double* taper, int64_t n, double* x,
     }
     *sz_local = szl;
     return status;
+}
+
+/* ======================================================================== */
+/* Discrete adjoint of one rk4par step and of DiffractionStep.               */
+/*                                                                          */
+/* No reference counterpart: the reference computes no derivatives.  These  */
+/* are the exact transposes of the oracle's own FP64 steps above (the        */
+/* tri-linear weights are differentiated inside each cell), derived by hand  */
+/* from the stage equations.  They are pinned by central finite differences  */
+/* of orc_rk4par / orc_diffraction_step and by dot-product identities.       */
+/* ======================================================================== */
+
+/* Weights of orc_interp and their derivatives along x, y and z2. */
+static int orc_interp_d(const puff_params* p, double x, double y, double z2,
+                        int64_t* node, double* w, double* wx, double* wy, double* wz) {
+    int f = orc_interp(p, x, y, z2, node, w);
+    if (f) return f;
+    const double zq = floor(z2 / p->dz2);
+    const double fz = (z2 - zq * p->dz2) / p->dz2;
+    if (p->one_d) {
+        wx[0] = wx[1] = wy[0] = wy[1] = 0.0;
+        wz[0] = -1.0 / p->dz2;
+        wz[1] = 1.0 / p->dz2;
+        return 0;
+    }
+    const double halfx = ((p->nx - 1) / 2.0) * p->dx, halfy = ((p->ny - 1) / 2.0) * p->dy;
+    const double xq = floor((x + halfx) / p->dx), yq = floor((y + halfy) / p->dy);
+    const double fx = (x + halfx - xq * p->dx) / p->dx, fy = (y + halfy - yq * p->dy) / p->dy;
+    const double X[2] = {1.0 - fx, fx}, Y[2] = {1.0 - fy, fy}, Z[2] = {1.0 - fz, fz};
+    const double dX[2] = {-1.0 / p->dx, 1.0 / p->dx};
+    const double dY[2] = {-1.0 / p->dy, 1.0 / p->dy};
+    const double dZ[2] = {-1.0 / p->dz2, 1.0 / p->dz2};
+    for (int k = 0; k < 8; k++) {
+        const int a = k & 1, b = (k >> 1) & 1, c = k >> 2;
+        wx[k] = dX[a] * Y[b] * Z[c];
+        wy[k] = X[a] * dY[b] * Z[c];
+        wz[k] = X[a] * Y[b] * dZ[c];
+    }
+    return 0;
+}
+
+/* Transverse derivatives of orc_bfields: g = {dbx/dx, dbx/dy, dby/dx, dby/dy,
+   dbz/dx, dbz/dy}. */
+void orc_bfields_grad(const puff_params* p, int place, double sz, double x, double y,
+                      double* g) {
+    const double rho = p->rho, pi = p->pi_ref, seta = sqrt(p->eta);
+    const double szt = sz / 2.0 / rho;
+    int t = p->und_type;
+    for (int k = 0; k < 6; k++) g[k] = 0.0;
+    if (t == PUFF_UND_HELICAL) t = PUFF_UND_ELLIPTICAL;
+    if (t == PUFF_UND_CURVED) {
+        const double kx = p->kx_und, ky = p->ky_und;
+        double fs, fc, cz;
+        if (place == 0) {
+            fs = szt / 4.0 / pi * sin(szt);
+            fc = (szt / 4.0 / pi) * cos(szt);
+            cz = seta / 2.0 / rho / ky;
+        } else if (place == 2) {
+            const double s = (sz - p->szfe) / 2.0 / rho;
+            fs = (-s / 4.0 / pi + 1.0) * sin(s);
+            fc = (-s / 4.0 / pi + 1.0) * cos(s);
+            cz = seta / 2.0 / rho / kx;
+        } else {
+            fs = sin(szt);
+            fc = cos(szt);
+            cz = seta / 2.0 / rho / kx;
+        }
+        const double shx = sinh(kx * x), chx = cosh(kx * x), shy = sinh(ky * y), chy = cosh(ky * y);
+        g[0] = kx / ky * kx * chx * shy * fs;
+        g[1] = kx / ky * shx * ky * chy * fs;
+        g[2] = kx * shx * chy * fs;
+        g[3] = ky * chx * shy * fs;
+        g[4] = cz * kx * shx * shy * fc;
+        g[5] = cz * ky * chx * chy * fc;
+    } else if (t == PUFF_UND_PLANEPOLE) {
+        double fc;
+        if (place == 0) {
+            fc = (szt / 4.0 / pi) * cos(szt);
+        } else if (place == 2) {
+            const double s = (sz - p->szfe) / 2.0 / rho;
+            fc = (-s / 4.0 / pi + 1.0) * cos(s);
+        } else {
+            fc = cos(szt);
+        }
+        g[5] = seta / 2.0 / rho * cosh(seta / 2.0 / rho * y) * fc;
+    } else {
+        /* elliptical: bz is linear in x and y with no constant term, bx and by
+           depend on x, y only through the focusing terms */
+        puff_params q = *p;
+        double bx, by, bz;
+        q.focusing = 0;
+        orc_bfields(&q, place, sz, 1.0, 0.0, &bx, &by, &bz);
+        g[4] = bz;
+        orc_bfields(&q, place, sz, 0.0, 1.0, &bx, &by, &bz);
+        g[5] = bz;
+    }
+    if (p->focusing) {
+        g[1] += seta * p->kbeta_y * p->kbeta_y / p->kappa;
+        g[2] += -seta * p->kbeta_x * p->kbeta_x / p->kappa;
+    }
+    if (p->one_d) {
+        g[4] = 0.0;
+        g[5] = 0.0;
+    }
+}
+
+/* Adjoint of one orc_derivs evaluation at trial particles s* and trial field
+   (ar, ai), with stage taper n2col.  Cotangents: c* of the six particle
+   derivatives and (gr, gi) of the field source.  m* (overwritten) receive
+   d<c, f>/d(particle); (mr, mi) accumulate d<c, f>/d(field).  Returns
+   d<c, f>/d(n2col); *flags_out gets any bounds flag. */
+static double orc_derivs_adjoint(const puff_params* p, double n2col, int place, double sz,
+                                 int64_t n, const double* ar, const double* ai,
+                                 const double* gr, const double* gi, const double* sx,
+                                 const double* sy, const double* sz2, const double* spr,
+                                 const double* spi, const double* sgam, const double* chi,
+                                 const double* cx, const double* cy, const double* cz2,
+                                 const double* cpr, const double* cpi, const double* cg,
+                                 double* mx, double* my, double* mz2, double* mpr, double* mpi,
+                                 double* mg, double* mr, double* mi, int nthreads,
+                                 int* flags_out) {
+    const int nnb = p->one_d ? 2 : 8;
+    const double dV3 = p->dx * p->dy * p->dz2;
+    const double inv2rho = 1.0 / (2.0 * p->rho);
+    const double eta = p->eta, kap = p->kappa, rho = p->rho;
+    const double c_dx = 2 * rho * kap / sqrt(eta);
+    const double gr2 = p->gamma_r * p->gamma_r, aw2 = p->aw * p->aw;
+    double dn2 = 0.0;
+    int flags = 0;
+#pragma omp parallel for num_threads(nthreads) reduction(+ : dn2) reduction(| : flags) schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        int64_t nd[8];
+        double w[8], wxk[8], wyk[8], wzk[8];
+        const int f = orc_interp_d(p, sx[i], sy[i], sz2[i], nd, w, wxk, wyk, wzk);
+        if (f) {
+            flags |= f;
+            mx[i] = my[i] = mz2[i] = mpr[i] = mpi[i] = mg[i] = 0.0;
+            continue;
+        }
+        const double qr = spr[i], qi = spi[i], g = sgam[i];
+        const double s2 = qr * qr + qi * qi;
+        /* u = 1 + eta p2 = D^(-1/2), D = 1 - (1 + aw^2 |p|^2) / (gamma_r Gamma)^2 */
+        const double D = 1.0 - 1.0 / (gr2 * g * g) * (1.0 + aw2 * s2);
+        const double u = 1.0 / sqrt(D);
+        const double u3 = u * u * u;
+        const double p2 = orc_p2(g, qr, qi, eta, p->gamma_r, p->aw);
+        const double fac = (1 + eta * p2) / g;
+        const double du_dpr = u3 * aw2 * qr / (gr2 * g * g);
+        const double du_dpi = u3 * aw2 * qi / (gr2 * g * g);
+        const double du_dg = -u3 * (1.0 + aw2 * s2) / (gr2 * g * g * g);
+        double fr = 0.0, fi = 0.0, Gr = 0.0, Gi = 0.0;
+        for (int k = 0; k < nnb; k++) {
+            fr += w[k] * ar[nd[k]];
+            fi += w[k] * ai[nd[k]];
+            Gr += w[k] * gr[nd[k]];
+            Gi += w[k] * gi[nd[k]];
+        }
+        if (!p->coupling) { fr = 0.0; fi = 0.0; }
+        const double dep = p->field_evolve ? chi[i] / dV3 : 0.0;
+        double ax = 0, ay = 0, az = 0, apr = 0, api = 0, ag = 0;
+        if (p->electrons_evolve) {
+            ax = cx[i]; ay = cy[i]; az = cz2[i]; apr = cpr[i]; api = cpi[i]; ag = cg[i];
+        }
+        double bx, by, bz, bg[6];
+        orc_bfields(p, place, sz, sx[i], sy[i], &bx, &by, &bz);
+        orc_bfields_grad(p, place, sz, sx[i], sy[i], bg);
+        const double ep2k = eta * p2 / (kap * kap);
+        /* d/dF of the dp_r, dp_i and dGamma equations */
+        const double Lfr = p->coupling ? (-apr * inv2rho * ep2k - ag * 2.0 * rho * fac * qr) : 0.0;
+        const double Lfi = p->coupling ? (-api * inv2rho * ep2k - ag * 2.0 * rho * fac * qi) : 0.0;
+        /* partials through u (p2 = (u-1)/eta) and fac, and the explicit ones */
+        const double Lu = az / eta - (apr * fr + api * fi) * inv2rho / (kap * kap);
+        const double Lfac = ax * c_dx * qr - ay * c_dx * qi + kap * n2col * bz * (apr * qi - api * qr) -
+                            2.0 * rho * ag * (qr * fr + qi * fi) + dep * (qr * Gr + qi * Gi);
+        const double Epr = ax * c_dx * fac - api * kap * fac * n2col * bz - 2.0 * rho * ag * fac * fr +
+                           dep * fac * Gr;
+        const double Epi = -ay * c_dx * fac + apr * kap * fac * n2col * bz - 2.0 * rho * ag * fac * fi +
+                           dep * fac * Gi;
+        const double dfac_dpr = du_dpr / g, dfac_dpi = du_dpi / g, dfac_dg = du_dg / g - fac / g;
+        mpr[i] = Epr + Lu * du_dpr + Lfac * dfac_dpr;
+        mpi[i] = Epi + Lu * du_dpi + Lfac * dfac_dpi;
+        mg[i] = Lu * du_dg + Lfac * dfac_dg;
+        const double Lbx = api * inv2rho * n2col, Lby = apr * inv2rho * n2col;
+        const double Lbz = kap * fac * n2col * (apr * qi - api * qr);
+        double sxv = 0.0, syv = 0.0, szv = 0.0;
+        for (int k = 0; k < nnb; k++) {
+            const double V = Lfr * ar[nd[k]] + Lfi * ai[nd[k]] + dep * fac * (qr * gr[nd[k]] + qi * gi[nd[k]]);
+            sxv += wxk[k] * V;
+            syv += wyk[k] * V;
+            szv += wzk[k] * V;
+        }
+        mx[i] = sxv + Lbx * bg[0] + Lby * bg[2] + Lbz * bg[4];
+        my[i] = syv + Lbx * bg[1] + Lby * bg[3] + Lbz * bg[5];
+        mz2[i] = szv;
+        for (int k = 0; k < nnb; k++) {
+#pragma omp atomic
+            mr[nd[k]] += w[k] * Lfr;
+#pragma omp atomic
+            mi[nd[k]] += w[k] * Lfi;
+        }
+        dn2 += apr * (inv2rho * by + kap * qi * fac * bz) + api * (inv2rho * bx - kap * qr * fac * bz);
+    }
+    *flags_out |= flags;
+    return dn2;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of orc_rk4par.  Input: the state at the start of the step (x ..  */
+/* gam, ar, ai; not modified), taper = {n2col, n2col0, undgrad} at the start */
+/* of the step (not modified), and the cotangent (lx .. lg, lar, lai) of the */
+/* state at the end of the step.  Output: the cotangent of the state at the  */
+/* start, in place, and dn2col[s] = dJ/d(n2col of stage s), s = 0..3.        */
+/*                                                                          */
+/* RK4 transposed: with w = (1, 2, 2, 1) h/6 and the trial coefficients      */
+/* c = (h/2, h/2, h), the cotangent of k_s is a_s = w_s L + c_s mu_{s+1}     */
+/* (a_4 = w_4 L), mu_s = J_f(Y_s)^T a_s, and L_start = L + sum_s mu_s, for   */
+/* the particle and field parts alike.  The aliased last plane of a periodic */
+/* mesh is a copy of plane 1: its cotangent is folded into plane 1.          */
+/* ------------------------------------------------------------------------ */
+int orc_rk4par_adjoint(const puff_params* p, const double* taper_in, int64_t n,
+                       const double* x, const double* y, const double* z2,
+                       const double* pr, const double* pi, const double* gam,
+                       const double* chi, const double* ar, const double* ai,
+                       double* lx, double* ly, double* lz2, double* lpr, double* lpi,
+                       double* lg, double* lar, double* lai, double sz, double h,
+                       double* dn2col, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    const int64_t nmesh = (int64_t)p->nx * p->ny * p->nz2;
+    const int64_t ntr = (int64_t)p->nx * p->ny;
+    const int64_t nn = n > 0 ? n : 1;
+    const double hh = h * 0.5, h6 = h / 6.0;
+    const double zs[4] = {sz, sz + hh, sz + hh, sz + h};
+    const double cin[4] = {0.0, hh, hh, h};
+    const double wts[4] = {1.0, 2.0, 2.0, 1.0};
+    double taper[3] = {taper_in[0], taper_in[1], taper_in[2]};
+    double n2c[4];
+    int flags = 0, status = PUFF_OK;
+
+    orc_work wk;
+    wk.p = p;
+    wk.taper = taper;
+    wk.n = n;
+    wk.nthreads = 1;
+    wk.nmesh = nmesh;
+    wk.priv = NULL;
+    wk.nodes = (int64_t*)malloc(sizeof(int64_t) * 8 * nn);
+    wk.wts = (double*)malloc(sizeof(double) * 8 * nn);
+    wk.p2 = (double*)malloc(sizeof(double) * nn);
+
+    /* particle sets: Y[s] trial states (Y[0] = y0), K[s] derivatives,
+       M[s] stage adjoints, C cotangent of the current stage */
+    double* pbuf = (double*)calloc((size_t)(6 * 15) * nn, sizeof(double));
+    double *Y[4][6], *K[3][6], *M[4][6], *C[6];
+    for (int c = 0; c < 6; c++) {
+        for (int s = 1; s < 4; s++) Y[s][c] = pbuf + ((s - 1) * 6 + c) * nn;
+        for (int s = 0; s < 3; s++) K[s][c] = pbuf + ((18 + s * 6) + c) * nn;
+        for (int s = 0; s < 4; s++) M[s][c] = pbuf + ((36 + s * 6) + c) * nn;
+        C[c] = pbuf + (60 + c) * nn;
+    }
+    Y[0][0] = (double*)x; Y[0][1] = (double*)y; Y[0][2] = (double*)z2;
+    Y[0][3] = (double*)pr; Y[0][4] = (double*)pi; Y[0][5] = (double*)gam;
+    /* fields: At[s] trial fields (At[0] = A0), dA[s] sources, MA[s] stage
+       adjoints, CA cotangent of the current stage's source */
+    double* fbuf = (double*)calloc((size_t)(2 * 12) * nmesh, sizeof(double));
+    double *At[4][2], *dA[3][2], *MA[4][2], *CA[2];
+    for (int c = 0; c < 2; c++) {
+        for (int s = 1; s < 4; s++) At[s][c] = fbuf + ((s - 1) * 2 + c) * nmesh;
+        for (int s = 0; s < 3; s++) dA[s][c] = fbuf + ((6 + s * 2) + c) * nmesh;
+        for (int s = 0; s < 4; s++) MA[s][c] = fbuf + ((12 + s * 2) + c) * nmesh;
+        CA[c] = fbuf + (20 + c) * nmesh;
+    }
+    At[0][0] = (double*)ar;
+    At[0][1] = (double*)ai;
+
+    /* forward stages 1-3 again, keeping trial states and sources */
+    for (int s = 0; s < 3; s++) {
+        flags |= orc_derivs(&wk, zs[s], At[s][0], At[s][1], Y[s][0], Y[s][1], Y[s][2], Y[s][3],
+                            Y[s][4], Y[s][5], chi, K[s][0], K[s][1], K[s][2], K[s][3], K[s][4],
+                            K[s][5], dA[s][0], dA[s][1]);
+        n2c[s] = taper[0];
+        if (flags) goto done;
+        for (int c = 0; c < 6; c++)
+            for (int64_t i = 0; i < n; i++) Y[s + 1][c][i] = Y[0][c][i] + cin[s + 1] * K[s][c][i];
+        for (int c = 0; c < 2; c++)
+            for (int64_t j = 0; j < nmesh; j++) At[s + 1][c][j] = At[0][c][j] + cin[s + 1] * dA[s][c][j];
+    }
+    n2c[3] = orc_alpha(p, taper, zs[3]);
+
+    if (p->periodic) {
+        const int64_t al = (int64_t)(p->nz2 - 1) * ntr;
+        for (int64_t t = 0; t < ntr; t++) {
+            lar[t] += lar[al + t];
+            lai[t] += lai[al + t];
+            lar[al + t] = 0.0;
+            lai[al + t] = 0.0;
+        }
+    }
+    double* L[6] = {lx, ly, lz2, lpr, lpi, lg};
+    double* LA[2] = {lar, lai};
+    for (int s = 3; s >= 0; s--) {
+        const double ws = wts[s] * h6;
+        const double cn = (s < 3) ? cin[s + 1] : 0.0;
+        for (int c = 0; c < 6; c++)
+            for (int64_t i = 0; i < n; i++) C[c][i] = ws * L[c][i] + ((s < 3) ? cn * M[s + 1][c][i] : 0.0);
+        for (int c = 0; c < 2; c++)
+            for (int64_t j = 0; j < nmesh; j++) CA[c][j] = ws * LA[c][j] + ((s < 3) ? cn * MA[s + 1][c][j] : 0.0);
+        dn2col[s] = orc_derivs_adjoint(p, n2c[s], orc_und_place(p, zs[s]), zs[s], n, At[s][0], At[s][1],
+                                       CA[0], CA[1], Y[s][0], Y[s][1], Y[s][2], Y[s][3], Y[s][4],
+                                       Y[s][5], chi, C[0], C[1], C[2], C[3], C[4], C[5], M[s][0],
+                                       M[s][1], M[s][2], M[s][3], M[s][4], M[s][5], MA[s][0],
+                                       MA[s][1], nthreads, &flags);
+        if (flags) goto done;
+    }
+    for (int c = 0; c < 6; c++)
+        for (int64_t i = 0; i < n; i++) L[c][i] += M[0][c][i] + M[1][c][i] + M[2][c][i] + M[3][c][i];
+    for (int c = 0; c < 2; c++)
+        for (int64_t j = 0; j < nmesh; j++) LA[c][j] += MA[0][c][j] + MA[1][c][j] + MA[2][c][j] + MA[3][c][j];
+
+done:
+    status = orc_status(flags);
+    free(wk.nodes);
+    free(wk.wts);
+    free(wk.p2);
+    free(pbuf);
+    free(fbuf);
+    return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of orc_diffraction_step (in place on the cotangent lr, li).       */
+/* With W the unnormalised forward DFT, N = ffact, E the multiplyexp factor  */
+/* (with its filter), G the real damping and M the real mask, the forward   */
+/* step is D = Q P, P = W^H E W / N, Q = (1 - M) + W^H G W M / N, so        */
+/* D^H = P^H Q^H with P^H = W^H conj(E) W / N and Q^H = (1 - M) +            */
+/* M W^H G W / N.  The output's aliased periodic plane is folded into plane  */
+/* 1 first; the input's aliased plane is not read, so its cotangent is 0.   */
+/* ------------------------------------------------------------------------ */
+int orc_diffraction_adjoint(const puff_params* p, double* lr, double* li, double h,
+                            int nthreads) {
+    const int nx = p->nx, ny = p->ny;
+    const int nzp = p->periodic ? p->nz2 - 1 : p->nz2;
+    const int64_t ntr = (int64_t)nx * ny, nn = ntr * nzp;
+    const double ffact = (double)nx * (double)ny * (double)nzp;
+    double* a = (double*)malloc(sizeof(double) * 2 * nn);
+    double* lam = (double*)malloc(sizeof(double) * 2 * nn);
+    double* kx = (double*)malloc(sizeof(double) * nx);
+    double* ky = (double*)malloc(sizeof(double) * ny);
+    double* kz = (double*)malloc(sizeof(double) * nzp);
+    double* mxy = (double*)malloc(sizeof(double) * ntr);
+    double* mz = (double*)malloc(sizeof(double) * nzp);
+    orc_kvalues(p, kx, ky, kz);
+    orc_mask_xy(p, mxy);
+    orc_mask_z2(p, mz);
+    if (p->periodic) {
+        const int64_t al = (int64_t)(p->nz2 - 1) * ntr;
+        for (int64_t t = 0; t < ntr; t++) {
+            lr[t] += lr[al + t];
+            li[t] += li[al + t];
+            lr[al + t] = 0.0;
+            li[al + t] = 0.0;
+        }
+    }
+    for (int64_t j = 0; j < nn; j++) {
+        lam[2 * j] = a[2 * j] = lr[j];
+        lam[2 * j + 1] = a[2 * j + 1] = li[j];
+    }
+    /* Q^H */
+    orc_fft3d(a, nx, ny, nzp, -1, nthreads);
+    for (int iz = 0; iz < nzp; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                if (kz[iz] != 0.0) {
+                    int64_t j = ((int64_t)iz * ny + iy) * nx + ix;
+                    double f = exp(-h * p->beta * (fabs(kx[ix]) + fabs(ky[iy])) /
+                                   (sqrt(fabs(2.0 * kz[iz]))));
+                    a[2 * j] *= f;
+                    a[2 * j + 1] *= f;
+                }
+            }
+    orc_fft3d(a, nx, ny, nzp, +1, nthreads);
+    for (int iz = 0; iz < nzp; iz++)
+        for (int64_t t = 0; t < ntr; t++) {
+            int64_t j = (int64_t)iz * ntr + t;
+            double m = mxy[t] + (mz[iz] * (1.0 - mxy[t]));
+            a[2 * j] = (1.0 - m) * lam[2 * j] + m * (a[2 * j] / ffact);
+            a[2 * j + 1] = (1.0 - m) * lam[2 * j + 1] + m * (a[2 * j + 1] / ffact);
+        }
+    /* P^H */
+    orc_fft3d(a, nx, ny, nzp, -1, nthreads);
+    const double cutoff = p->filt_cutoff;
+    for (int iz = 0; iz < nzp; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                int64_t j = ((int64_t)iz * ny + iy) * nx + ix;
+                if ((kz[iz] > cutoff) || (kz[iz] < -cutoff)) {
+                    if (kz[iz] != 0.0) {
+                        double th = -h * (kx[ix] * kx[ix] + ky[iy] * ky[iy]) / (2.0 * kz[iz]);
+                        double c = cos(th), s = sin(th);
+                        double re = a[2 * j], im = a[2 * j + 1];
+                        a[2 * j] = c * re - s * im;
+                        a[2 * j + 1] = s * re + c * im;
+                    }
+                } else if (p->filter) {
+                    a[2 * j] = 0.0;
+                    a[2 * j + 1] = 0.0;
+                }
+            }
+    orc_fft3d(a, nx, ny, nzp, +1, nthreads);
+    for (int64_t j = 0; j < nn; j++) {
+        lr[j] = a[2 * j] / ffact;
+        li[j] = a[2 * j + 1] / ffact;
+    }
+    free(a);
+    free(lam);
+    free(kx);
+    free(ky);
+    free(kz);
+    free(mxy);
+    free(mz);
+    return PUFF_OK;
 }

@@ -1,0 +1,230 @@
+"""Reverse-mode gradient of the integrated power of one undulator module with
+respect to the per-period undulator strength a_w(z) and the seed amplitude,
+behind a ``torch.autograd.Function``.
+
+No reference counterpart: the reference computes no derivatives.  The
+forward model is the reference's UndSection schedule (undulator.f90:103-333,
+restated in ``undulator.und_section``) with the linear taper replaced by a
+per-period table of n2col values (a_w / aw); the output is
+
+    J = sum_iz wz[iz] power(iz),   power as gPowerP (io/av_write.f90:39-202),
+
+i.e. the integral of |A_perp|^2 over the mesh (trapezoid in z2 on a temporal
+mesh, one period of planes on a periodic mesh).
+
+The backward pass is the discrete adjoint of the implemented steps
+(puff_rk4par_adjoint, puff_diffraction_adjoint) with checkpoint and
+recompute: the forward keeps a device snapshot of the state every ``every``
+RK4 steps; the backward replays each segment from its snapshot, keeping one
+snapshot per step of the segment, and walks the segment's operations in
+reverse.  Snapshots are caller-owned device memory (torch uint8 tensors).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from .gpu import Integrator
+from .params import PI_REF, ModuleParams
+from .undulator import und_section
+
+try:
+    import torch
+except ImportError:   # the gradient engine needs torch for device memory
+    torch = None
+
+
+@dataclass
+class Op:
+    kind: str          # "rk4", "diff" or "wrap"
+    sz: float = 0.0
+    h: float = 0.0
+
+
+class _Recorder:
+    """Stand-in driver that records the schedule's calls instead of running them."""
+
+    def __init__(self):
+        self.ops: list[Op] = []
+        self._sz = 0.0
+
+    def rk4par(self, sz, h):
+        self.ops.append(Op("rk4", float(sz), float(h)))
+        return 0
+
+    def diffraction_step(self, h):
+        self.ops.append(Op("diff", 0.0, float(h)))
+
+    def wrap_z2(self):
+        self.ops.append(Op("wrap"))
+
+
+def module_program(nsteps: int, h: float, steps4diff: int, *, diffraction: bool = True,
+                   periodic: bool = False, redist_every: int = 60, sz_local: float = 0.0) -> list[Op]:
+    """The operations UndSection performs for one module (no writes)."""
+    rec = _Recorder()
+    und_section(rec, nsteps, h, diffraction=diffraction, steps4diff=steps4diff,
+                redist_every=redist_every, sz_local=sz_local, periodic=periodic)
+    return rec.ops
+
+
+def power_weights(mp: ModuleParams) -> np.ndarray:
+    """z2 weights wz of J = sum wz power: trapezoid over the NZ2 nodes of a
+    temporal mesh; dz2 per plane over one period of a periodic mesh (its
+    aliased last plane weighs 0)."""
+    wz = np.full(mp.nz2, mp.dz2)
+    if mp.periodic:
+        wz[-1] = 0.0
+    else:
+        wz[0] *= 0.5
+        wz[-1] *= 0.5
+    return wz
+
+
+def integrated_power(mp: ModuleParams, power: np.ndarray) -> float:
+    return float(np.dot(power_weights(mp), power))
+
+
+def segments(ops: list[Op], every: int) -> list[tuple[int, int]]:
+    """[start, stop) op ranges; a new range starts at every ``every``-th RK4 op."""
+    starts = [0]
+    nrk = 0
+    for i, op in enumerate(ops):
+        if op.kind == "rk4":
+            if nrk > 0 and nrk % every == 0 and i > 0:
+                starts.append(i)
+            nrk += 1
+    bounds = starts + [len(ops)]
+    return [(bounds[k], bounds[k + 1]) for k in range(len(starts)) if bounds[k] < bounds[k + 1]]
+
+
+class ModuleGradient:
+    """Forward run of one module with checkpoints, and its adjoint.
+
+    ``drv`` is an Integrator whose particles hold the module's initial state;
+    ``field_shape`` is the seed field of unit amplitude (split Re / Im);
+    ``taper_period`` / ``taper_z0`` place the per-period table on zbar."""
+
+    def __init__(self, drv: Integrator, ops: list[Op], field_shape: tuple, *, every: int = 30,
+                 taper_period: float | None = None, taper_z0: float = 0.0):
+        if torch is None:
+            raise RuntimeError("the gradient engine keeps its snapshots in torch device memory")
+        self.drv = drv
+        self.mp = drv.mp
+        self.ops = ops
+        self.every = int(every)
+        self.segs = segments(ops, self.every)
+        self.shape_r = np.ascontiguousarray(field_shape[0], dtype=np.float64).reshape(-1)
+        self.shape_i = np.ascontiguousarray(field_shape[1], dtype=np.float64).reshape(-1)
+        self.taper_period = taper_period if taper_period else 4.0 * PI_REF * self.mp.rho
+        self.taper_z0 = taper_z0
+        self.wz = power_weights(self.mp)
+        nb = drv.state_bytes()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self._blob = lambda: torch.empty(nb, dtype=torch.uint8, device=dev)
+        # initial particles (taken from the driver as it is now)
+        self.initial = self._blob()
+        drv.state_save(self.initial.data_ptr())
+        self.ckpts = [self._blob() for _ in self.segs]
+        self.steps = [self._blob() for _ in range(self.every)]
+        self.table = None
+        self.seed = None
+        self.J = None
+        self.rk4_calls = 0
+
+    def memory_bytes(self) -> int:
+        return (1 + len(self.ckpts) + len(self.steps)) * self.initial.numel()
+
+    # ---- forward
+    def _set_inputs(self, table, seed: float):
+        self.table = np.ascontiguousarray(table, dtype=np.float64)
+        self.seed = float(seed)
+        self.drv.state_load(self.initial.data_ptr())
+        self.drv.upload_field(self.seed * self.shape_r, self.seed * self.shape_i)
+        self.drv.set_taper_table(self.table, self.taper_z0, self.taper_period)
+
+    def _run(self, op: Op):
+        if op.kind == "rk4":
+            st = self.drv.rk4par(op.sz, op.h)
+            self.rk4_calls += 1
+            if st != 0:
+                raise RuntimeError(f"rk4par status {st} at zbar {op.sz}")
+        elif op.kind == "diff":
+            self.drv.diffraction_step(op.h)
+        else:
+            self.drv.wrap_z2()
+
+    def forward(self, table, seed: float) -> float:
+        self._set_inputs(table, seed)
+        for k, (a, b) in enumerate(self.segs):
+            self.drv.state_save(self.ckpts[k].data_ptr())
+            for op in self.ops[a:b]:
+                self._run(op)
+        self.J = integrated_power(self.mp, self.drv.power())
+        return self.J
+
+    def evaluate(self, table, seed: float) -> float:
+        """J without keeping checkpoints (finite differences)."""
+        self._set_inputs(table, seed)
+        for op in self.ops:
+            self._run(op)
+        return integrated_power(self.mp, self.drv.power())
+
+    # ---- backward
+    def backward(self, scale: float = 1.0) -> dict:
+        """dJ/d(table) and dJ/d(seed amplitude), times ``scale``.  Must follow
+        forward() (the state must be the module's final state)."""
+        drv = self.drv
+        drv.adjoint_enable()
+        drv.adjoint_zero()
+        drv.adjoint_seed_power(self.wz, scale)
+        g_table = np.zeros(self.table.size)
+        for k in range(len(self.segs) - 1, -1, -1):
+            a, b = self.segs[k]
+            drv.state_load(self.ckpts[k].data_ptr())
+            slots = {}
+            for i in range(a, b):
+                op = self.ops[i]
+                if op.kind == "rk4":
+                    j = len(slots)
+                    drv.state_save(self.steps[j].data_ptr())
+                    slots[i] = j
+                self._run(op)
+            for i in range(b - 1, a - 1, -1):
+                op = self.ops[i]
+                if op.kind == "rk4":
+                    drv.state_load(self.steps[slots[i]].data_ptr())
+                    st, dn, sl = drv.rk4par_adjoint(op.sz, op.h)
+                    if st != 0:
+                        raise RuntimeError(f"adjoint replay status {st} at zbar {op.sz}")
+                    for s in range(4):
+                        if sl[s] >= 0:
+                            g_table[sl[s]] += dn[s]
+                elif op.kind == "diff":
+                    drv.diffraction_adjoint(op.h)
+        _, lar, lai = drv.adjoint_download()
+        g_seed = float(np.dot(lar, self.shape_r) + np.dot(lai, self.shape_i))
+        return {"table": g_table, "seed": g_seed, "field": (lar, lai)}
+
+
+if torch is not None:
+
+    class IntegratedPower(torch.autograd.Function):
+        """J(table, seed) of one module; gradients by the discrete adjoint.
+
+        Inputs: ``table`` (CPU float64 tensor of per-period n2col values),
+        ``seed`` (0-d float64 tensor, seed amplitude) and the ModuleGradient
+        ``engine``.  Output: 0-d float64 tensor J."""
+
+        @staticmethod
+        def forward(ctx, table, seed, engine):
+            J = engine.forward(table.detach().cpu().numpy(), float(seed))
+            ctx.engine = engine
+            return torch.tensor(J, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gJ):
+            g = ctx.engine.backward(float(gJ))
+            return (torch.from_numpy(g["table"]), torch.tensor(g["seed"], dtype=torch.float64), None)

@@ -35,6 +35,12 @@ struct Particles {
     R* chi = nullptr;
 };
 
+template <typename R>
+struct AdjBufs {
+    PartPtrs<R> L;
+    PartPtrs<R> M[3];
+};
+
 struct Ctx {
     puff_params p{};
     int precision = PUFF_FP64;
@@ -93,6 +99,22 @@ struct Ctx {
     std::vector<Pend> pending;
     double prof_ms[PUFF_PROF_KINDS] = {};
     int64_t prof_n[PUFF_PROF_KINDS] = {};
+    // particle order identity: a new id for every sort or full upload
+    int64_t order_id = 0;
+    int64_t order_counter = 0;
+    // discrete adjoint (puff_adjoint_enable): cotangent L of the particles
+    // and LA of the field, stage cotangents M (stages 2..4) and mA (1..4)
+    bool adj = false;
+    AdjBufs<double> ad;
+    AdjBufs<float> af;
+    void* LA = nullptr;
+    void* mA[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* dn2 = nullptr;     // 4 x DN2_COPIES
+    double* h_dn2 = nullptr;   // pinned
+    double* adj_wz = nullptr;  // nz2 power weights
+    void* h_hdr = nullptr;     // pinned state-blob header
+    int* lam_orig = nullptr;   // slot order of the particle cotangent
+    int64_t lam_order_id = 0;
 
     int64_t nmesh() const { return (int64_t)p.nx * p.ny * p.nz2; }
     int nzp() const { return p.periodic ? p.nz2 - 1 : p.nz2; }
@@ -104,6 +126,12 @@ template <>
 Particles<double>& parts<double>(Ctx* c) { return c->pd; }
 template <>
 Particles<float>& parts<float>(Ctx* c) { return c->pf; }
+template <typename R>
+static AdjBufs<R>& adjb(Ctx* c);
+template <>
+AdjBufs<double>& adjb<double>(Ctx* c) { return c->ad; }
+template <>
+AdjBufs<float>& adjb<float>(Ctx* c) { return c->af; }
 
 static int dalloc(Ctx* c, void** p, size_t bytes) {
     if (bytes == 0) bytes = 16;
@@ -348,12 +376,18 @@ static int stage_scalars(const puff_params& p, double z, double n2col, StageScal
 
 // getAlpha with its side effect (wiggler_taper.f90:39-45), or the per-period
 // table when one is set.
+// Slot of the per-period table at zbar z (clamped), -1 without a table.
+static int taper_slot(const Ctx* c, double z) {
+    if (c->taper_table.empty()) return -1;
+    long long k = (long long)std::floor((z - c->taper_z0) / c->taper_period);
+    if (k < 0) k = 0;
+    if (k >= (long long)c->taper_table.size()) k = (long long)c->taper_table.size() - 1;
+    return (int)k;
+}
+
 static double advance_taper(Ctx* c, double z) {
     if (!c->taper_table.empty()) {
-        long long k = (long long)std::floor((z - c->taper_z0) / c->taper_period);
-        if (k < 0) k = 0;
-        if (k >= (long long)c->taper_table.size()) k = (long long)c->taper_table.size() - 1;
-        c->n2col = c->taper_table[(size_t)k];
+        c->n2col = c->taper_table[(size_t)taper_slot(c, z)];
         return c->n2col;
     }
     if (z >= c->p.szfs && z <= c->p.szfe) {
@@ -533,6 +567,8 @@ static void release(Ctx* c) {
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     c->ev_pool.clear();
     if (c->h_flags) (void)hipHostFree(c->h_flags);
+    if (c->h_dn2) (void)hipHostFree(c->h_dn2);
+    if (c->h_hdr) (void)hipHostFree(c->h_hdr);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -562,6 +598,7 @@ static void sort_particles(Ctx* c) {
     c->sorted = true;
     c->identity_order = false;
     c->steps_since_sort = 0;
+    c->order_id = ++c->order_counter;
 }
 
 template <typename R>
@@ -729,6 +766,7 @@ static int upload_particles(Ctx* c, int kind, const double* x, const double* y, 
         PUFF_HIP(hipGetLastError());
         c->identity_order = true;
         c->sorted = false;
+        c->order_id = ++c->order_counter;
     }
     PUFF_TRY(upload_array<R>(c, kind, x, P.y.x));
     PUFF_TRY(upload_array<R>(c, kind, y, P.y.y));
@@ -822,6 +860,447 @@ static int bunching(Ctx* c, int harmonic, int nslices, double slice, double* out
         const double br = acc[is], bi = acc[nslices + is], sd = acc[2 * nslices + is];
         out[is] = (sd > 0.0) ? std::sqrt(br * br + bi * bi) / slice : 0.0;   // av_write.f90:725-749
     }
+    return PUFF_OK;
+}
+
+// ---- discrete adjoint ---------------------------------------------------------
+
+static int need_adj(const Ctx* c) {
+    if (!c->adj) {
+        set_error("adjoint buffers not allocated (puff_adjoint_enable)");
+        return PUFF_ERR_STATE;
+    }
+    return PUFF_OK;
+}
+
+template <typename R>
+static int adjoint_zero(Ctx* c);
+
+template <typename R>
+static int adjoint_enable(Ctx* c) {
+    if (c->adj) return PUFF_OK;
+    AdjBufs<R>& B = adjb<R>(c);
+    auto setp = [&](PartPtrs<R>& q) -> int {
+        PUFF_TRY(dalloc(c, (void**)&q.x, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.y, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.z2, sizeof(double) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.pr, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.pi, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.g, sizeof(double) * c->n));
+        return PUFF_OK;
+    };
+    PUFF_TRY(setp(B.L));
+    for (int k = 0; k < 3; ++k) PUFF_TRY(setp(B.M[k]));
+    const int64_t nm = c->nmesh();
+    PUFF_TRY(dalloc(c, &c->LA, sizeof(cplx<R>) * nm));
+    for (int k = 0; k < 4; ++k) PUFF_TRY(dalloc(c, &c->mA[k], sizeof(cplx<R>) * nm));
+    PUFF_TRY(dalloc(c, (void**)&c->dn2, sizeof(double) * 4 * DN2_COPIES));
+    PUFF_TRY(dalloc(c, (void**)&c->adj_wz, sizeof(double) * c->p.nz2));
+    PUFF_TRY(dalloc(c, (void**)&c->lam_orig, sizeof(int) * c->n));
+    PUFF_HIP(hipHostMalloc((void**)&c->h_dn2, sizeof(double) * 4 * DN2_COPIES, hipHostMallocDefault));
+    c->adj = true;
+    return adjoint_zero<R>(c);
+}
+
+// Zero-fill of a typed array with a fill kernel (no memset nodes).
+template <typename T>
+static void fill_zero_typed(T* p, int64_t n, hipStream_t s) {
+    static_assert(sizeof(T) % 4 == 0, "4-byte multiple");
+    launch_fill_zero_words(reinterpret_cast<uint32_t*>(p), (int64_t)(sizeof(T) / 4) * n, s);
+}
+
+// Re-index the particle cotangent from slot order orig_from to slot order
+// orig_to (reference index of each slot), through the reference order.
+template <typename R>
+static int repermute_cotangent(Ctx* c, const int* orig_from, const int* orig_to) {
+    AdjBufs<R>& B = adjb<R>(c);
+    hipStream_t s = c->stream;
+    R* rr[4] = {B.L.x, B.L.y, B.L.pr, B.L.pi};
+    double* dd[2] = {B.L.z2, B.L.g};
+    for (R* a : rr) {
+        launch_unpermute<R>(a, orig_from, c->n, c->stage_buf, s);
+        launch_permute_in<R>(c->stage_buf, orig_to, c->n, a, s);
+    }
+    for (double* a : dd) {
+        launch_unpermute<double>(a, orig_from, c->n, c->stage_buf, s);
+        launch_permute_in<double>(c->stage_buf, orig_to, c->n, a, s);
+    }
+    PUFF_HIP(hipGetLastError());
+    return PUFF_OK;
+}
+
+// The particle cotangent keeps its own slot order (lam_orig): a sort or a
+// state load changes the state's order only.  Bring it to the state's order.
+template <typename R>
+static int sync_cotangent_order(Ctx* c) {
+    if (c->lam_order_id == c->order_id) return PUFF_OK;
+    PUFF_TRY(repermute_cotangent<R>(c, c->lam_orig, c->orig));
+    PUFF_HIP(hipMemcpyAsync(c->lam_orig, c->orig, sizeof(int) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    c->lam_order_id = c->order_id;
+    return PUFF_OK;
+}
+
+template <typename R>
+static int adjoint_zero(Ctx* c) {
+    PUFF_TRY(need_adj(c));
+    AdjBufs<R>& B = adjb<R>(c);
+    const int64_t n = c->n, nm = c->nmesh();
+    fill_zero_typed<R>(B.L.x, n, c->stream);
+    fill_zero_typed<R>(B.L.y, n, c->stream);
+    fill_zero_typed<R>(B.L.pr, n, c->stream);
+    fill_zero_typed<R>(B.L.pi, n, c->stream);
+    fill_zero_typed<double>(B.L.z2, n, c->stream);
+    fill_zero_typed<double>(B.L.g, n, c->stream);
+    fill_zero_typed<cplx<R>>(reinterpret_cast<cplx<R>*>(c->LA), nm, c->stream);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipMemcpyAsync(c->lam_orig, c->orig, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
+    c->lam_order_id = c->order_id;
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int adjoint_seed_power(Ctx* c, const double* wz, double scale) {
+    PUFF_TRY(need_adj(c));
+    PUFF_HIP(hipMemcpyAsync(c->adj_wz, wz, sizeof(double) * c->p.nz2, hipMemcpyHostToDevice, c->stream));
+    launch_adj_seed_power<R>(reinterpret_cast<cplx<R>*>(c->LA), reinterpret_cast<const cplx<R>*>(c->A),
+                             c->adj_wz, c->p.nx, c->p.ny, c->p.nz2, c->p.periodic, c->p.dx, c->p.dy,
+                             scale, c->stream);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    return PUFF_OK;
+}
+
+// The aliased last plane of a periodic mesh copies plane 1: its cotangent is
+// folded into plane 1.
+template <typename R>
+static int fold_alias(Ctx* c) {
+    if (!c->p.periodic) return PUFF_OK;
+    launch_fold_alias<R>(reinterpret_cast<cplx<R>*>(c->LA), (int64_t)c->p.nx * c->p.ny, c->p.nz2, c->stream);
+    PUFF_HIP(hipGetLastError());
+    return PUFF_OK;
+}
+
+// Adjoint of one rk4par step: the current state is the state at the start of
+// the step (left unchanged), L / LA the cotangent of the end state (replaced
+// by the cotangent of the start state).  dn2col[k] = dJ/d(n2col of stage k),
+// slot[k] its per-period table slot (-1 without a table).
+template <typename R>
+static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* slot) {
+    PUFF_TRY(need_adj(c));
+    Particles<R>& P = parts<R>(c);
+    AdjBufs<R>& B = adjb<R>(c);
+    hipStream_t s = c->stream;
+    const double hh = h * 0.5, h6 = h / 6.0;
+    const double zs[4] = {sz, sz + hh, sz + hh, sz + h};
+    const double saved[3] = {c->n2col, c->n2col0, c->undgrad};
+    StageScalars sc[4];
+    int st_err = PUFF_OK;
+    for (int k = 0; k < 4 && st_err == PUFF_OK; ++k) {
+        const double n2 = advance_taper(c, zs[k]);
+        st_err = stage_scalars(c->p, zs[k], n2, sc[k]);
+        if (slot) slot[k] = taper_slot(c, zs[k]);
+    }
+    c->n2col = saved[0];
+    c->n2col0 = saved[1];
+    if (st_err != PUFF_OK) return st_err;
+    PUFF_TRY(sync_cotangent_order<R>(c));
+    PUFF_TRY(fold_alias<R>(c));
+    launch_fill_zero(c->dn2, 4 * DN2_COPIES, s);
+    int slot_p = prof_start(c, 10);
+    launch_step_begin(c->flags, c->box, s);
+    // forward stages 1-3 again: k_1..k_3 and dA_1..dA_3 of this step
+    StageArgs<R> a{};
+    a.n = c->n;
+    a.y0 = P.y;
+    a.chi = P.chi;
+    a.h6 = h6;
+    a.A0 = reinterpret_cast<const cplx<R>*>(c->A);
+    a.mesh = c->mesh;
+    a.phys = c->phys;
+    a.flags = c->flags;
+    a.box = c->box;
+    const double cin[4] = {0.0, hh, hh, h};
+    const double cnext[4] = {hh, hh, h, 0.0};
+    for (int st = 1; st <= 3; ++st) {
+        a.s = sc[st - 1];
+        a.cin = cin[st - 1];
+        a.cnext = cnext[st - 1];
+        a.kin = (st > 1) ? P.k[st - 2] : PartPtrs<R>{};
+        a.kout = P.k[st - 1];
+        a.dAin = (st > 1) ? reinterpret_cast<const cplx<R>*>(c->dA[st - 2]) : nullptr;
+        a.dAout = reinterpret_cast<cplx<R>*>(c->dA[st - 1]);
+        launch_stage<R>(st, a, s);
+    }
+    prof_stop(c, slot_p);
+    // adjoint stages 4 .. 1
+    const double wts[4] = {1.0, 2.0, 2.0, 1.0};
+    AdjArgs<R> b{};
+    b.n = c->n;
+    b.y0 = P.y;
+    b.chi = P.chi;
+    b.A0 = reinterpret_cast<const cplx<R>*>(c->A);
+    b.L = B.L;
+    b.LA = reinterpret_cast<const cplx<R>*>(c->LA);
+    b.mesh = c->mesh;
+    b.phys = c->phys;
+    b.flags = c->flags;
+    b.box = c->box;
+    slot_p = prof_start(c, 7);
+    for (int st = 4; st >= 1; --st) {
+        b.s = sc[st - 1];
+        b.cin = cin[st - 1];
+        b.kin = (st > 1) ? P.k[st - 2] : PartPtrs<R>{};
+        b.dAin = (st > 1) ? reinterpret_cast<const cplx<R>*>(c->dA[st - 2]) : nullptr;
+        b.ws = wts[st - 1] * h6;
+        b.cnext = (st < 4) ? cin[st] : 0.0;
+        b.mnext = (st < 4) ? B.M[st - 1] : PartPtrs<R>{};
+        b.mAnext = (st < 4) ? reinterpret_cast<const cplx<R>*>(c->mA[st]) : nullptr;
+        b.mout = (st > 1) ? B.M[st - 2] : PartPtrs<R>{};
+        b.m3 = B.M[1];
+        b.m4 = B.M[2];
+        b.mAout = reinterpret_cast<cplx<R>*>(c->mA[st - 1]);
+        b.dn2 = c->dn2 + (st - 1) * DN2_COPIES;
+        launch_adj_stage<R>(st, b, s);
+    }
+    prof_stop(c, slot_p);
+    AdjCombineArgs<R> ca{};
+    ca.LA = reinterpret_cast<cplx<R>*>(c->LA);
+    for (int k = 0; k < 4; ++k) ca.mA[k] = reinterpret_cast<cplx<R>*>(c->mA[k]);
+    for (int k = 0; k < 3; ++k) ca.dA[k] = reinterpret_cast<cplx<R>*>(c->dA[k]);
+    ca.box = c->box;
+    ca.nx = c->p.nx;
+    ca.ny = c->p.ny;
+    ca.nz2 = c->p.nz2;
+    ca.nzp = c->nzp();
+    ca.periodic = c->p.periodic;
+    ca.one_d = c->p.one_d;
+    slot_p = prof_start(c, 8);
+    launch_adj_combine<R>(ca, s);
+    prof_stop(c, slot_p);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    PUFF_HIP(hipMemcpyAsync(c->h_dn2, c->dn2, sizeof(double) * 4 * DN2_COPIES, hipMemcpyDeviceToHost, s));
+    PUFF_HIP(hipStreamSynchronize(s));
+    prof_collect(c);
+    if (dn2col) {
+        for (int k = 0; k < 4; ++k) {
+            double t = 0.0;
+            for (int j = 0; j < DN2_COPIES; ++j) t += c->h_dn2[k * DN2_COPIES + j];
+            dn2col[k] = t;
+        }
+    }
+    const int f = *c->h_flags;
+    if (f) {
+        set_error("a trial state of the replayed step left the mesh");
+        if (f & FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
+        if (f & FLAG_TRANSVERSE) return PUFF_OUT_TRANSVERSE;
+        return PUFF_OUT_Z2_BUFFER;
+    }
+    return PUFF_OK;
+}
+
+template <typename R>
+static int diffraction_adjoint(Ctx* c, double h) {
+    PUFF_TRY(need_adj(c));
+    if (c->p.one_d) {
+        set_error("no diffraction on a 1-D mesh (read_base_input.f90:389)");
+        return PUFF_ERR_STATE;
+    }
+    DiffArgs<R> a{};
+    a.a = reinterpret_cast<cplx<R>*>(c->LA);
+    a.anb = reinterpret_cast<cplx<R>*>(c->anb);
+    a.kx = c->kx;
+    a.ky = c->ky;
+    a.kz = c->kz;
+    a.mxy = c->mxy;
+    a.mz = c->mz;
+    a.nx = c->p.nx;
+    a.ny = c->p.ny;
+    a.nz2 = c->p.nz2;
+    a.nzp = c->nzp();
+    a.periodic = c->p.periodic;
+    a.filter = c->p.filter;
+    a.h = h;
+    a.beta = c->p.beta;
+    a.cutoff = c->p.filt_cutoff;
+    a.ffact = (double)c->p.nx * (double)c->p.ny * (double)a.nzp;
+    a.ax = c->ax;
+    a.ay = c->ay;
+    a.az = c->az;
+    const int slot = prof_start(c, 9);
+    launch_diffraction_adjoint<R>(a, c->stream);
+    prof_stop(c, slot);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    prof_collect(c);
+    return PUFF_OK;
+}
+
+template <typename R>
+static int adjoint_upload(Ctx* c, int kind, const double* x, const double* y, const double* z2,
+                          const double* px, const double* py, const double* g, const double* ar,
+                          const double* ai) {
+    PUFF_TRY(need_adj(c));
+    PUFF_TRY(sync_cotangent_order<R>(c));
+    AdjBufs<R>& B = adjb<R>(c);
+    PUFF_TRY(upload_array<R>(c, kind, x, B.L.x));
+    PUFF_TRY(upload_array<R>(c, kind, y, B.L.y));
+    PUFF_TRY(upload_array<double>(c, kind, z2, B.L.z2));
+    PUFF_TRY(upload_array<R>(c, kind, px, B.L.pr));
+    PUFF_TRY(upload_array<R>(c, kind, py, B.L.pi));
+    PUFF_TRY(upload_array<double>(c, kind, g, B.L.g));
+    if (ar && ai) {
+        const int64_t nm = c->nmesh();
+        const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        PUFF_HIP(hipMemcpyAsync(c->stage_buf, ar, sizeof(double) * nm, k, c->stream));
+        PUFF_HIP(hipMemcpyAsync(c->stage_buf + nm, ai, sizeof(double) * nm, k, c->stream));
+        launch_pack_field<R>(c->stage_buf, c->stage_buf + nm, reinterpret_cast<cplx<R>*>(c->LA), nm, c->stream);
+        PUFF_HIP(hipGetLastError());
+        PUFF_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PUFF_OK;
+}
+
+template <typename R>
+static int adjoint_download(Ctx* c, int kind, double* x, double* y, double* z2, double* px, double* py,
+                            double* g, double* ar, double* ai) {
+    PUFF_TRY(need_adj(c));
+    PUFF_TRY(sync_cotangent_order<R>(c));
+    AdjBufs<R>& B = adjb<R>(c);
+    PUFF_TRY(download_array<R>(c, kind, B.L.x, x));
+    PUFF_TRY(download_array<R>(c, kind, B.L.y, y));
+    PUFF_TRY(download_array<double>(c, kind, B.L.z2, z2));
+    PUFF_TRY(download_array<R>(c, kind, B.L.pr, px));
+    PUFF_TRY(download_array<R>(c, kind, B.L.pi, py));
+    PUFF_TRY(download_array<double>(c, kind, B.L.g, g));
+    if (ar || ai) {
+        const int64_t nm = c->nmesh();
+        const hipMemcpyKind k = kind == PUFF_MEM_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        launch_unpack_field<R>(reinterpret_cast<const cplx<R>*>(c->LA), c->stage_buf, c->stage_buf + nm, nm,
+                               c->stream);
+        PUFF_HIP(hipGetLastError());
+        if (ar) PUFF_HIP(hipMemcpyAsync(ar, c->stage_buf, sizeof(double) * nm, k, c->stream));
+        if (ai) PUFF_HIP(hipMemcpyAsync(ai, c->stage_buf + nm, sizeof(double) * nm, k, c->stream));
+        PUFF_HIP(hipStreamSynchronize(c->stream));
+    }
+    return PUFF_OK;
+}
+
+// ---- state snapshots ------------------------------------------------------------
+// Blob: 256-byte header, then x, y, z2, pr, pi, Gamma, chi, orig (particle slot
+// order) and the field, each section 256-byte aligned.
+
+struct BlobHeader {
+    uint64_t magic;
+    int64_t n, nmesh;
+    int32_t precision, sorted, identity_order, pad;
+    int64_t order_id, steps_since_sort;
+    double n2col, n2col0, undgrad;
+};
+constexpr uint64_t BLOB_MAGIC = 0x31424f4c42465550ull;   // "PUFBLOB1"
+static_assert(sizeof(BlobHeader) <= 256, "blob header");
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+template <typename R>
+static size_t blob_layout(const Ctx* c, size_t off[9]) {
+    const size_t n = (size_t)c->n;
+    const size_t sz[9] = {sizeof(R) * n, sizeof(R) * n, sizeof(double) * n, sizeof(R) * n, sizeof(R) * n,
+                          sizeof(double) * n, sizeof(R) * n, sizeof(int) * n,
+                          sizeof(cplx<R>) * (size_t)c->nmesh()};
+    size_t o = 256;
+    for (int k = 0; k < 9; ++k) {
+        off[k] = o;
+        o = align256(o + sz[k]);
+    }
+    return o;
+}
+
+template <typename R>
+static int state_bytes(Ctx* c, int64_t* bytes) {
+    size_t off[9];
+    *bytes = (int64_t)blob_layout<R>(c, off);
+    return PUFF_OK;
+}
+
+template <typename R>
+static int state_save(Ctx* c, void* dst) {
+    Particles<R>& P = parts<R>(c);
+    size_t off[9];
+    blob_layout<R>(c, off);
+    if (!c->h_hdr) PUFF_HIP(hipHostMalloc(&c->h_hdr, 256, hipHostMallocDefault));
+    BlobHeader* hd = reinterpret_cast<BlobHeader*>(c->h_hdr);
+    *hd = BlobHeader{};
+    hd->magic = BLOB_MAGIC;
+    hd->n = c->n;
+    hd->nmesh = c->nmesh();
+    hd->precision = c->precision;
+    hd->sorted = c->sorted;
+    hd->identity_order = c->identity_order;
+    hd->order_id = c->order_id;
+    hd->steps_since_sort = c->steps_since_sort;
+    hd->n2col = c->n2col;
+    hd->n2col0 = c->n2col0;
+    hd->undgrad = c->undgrad;
+    char* d = reinterpret_cast<char*>(dst);
+    const size_t n = (size_t)c->n;
+    hipStream_t s = c->stream;
+    PUFF_HIP(hipMemcpyAsync(d, c->h_hdr, sizeof(BlobHeader), hipMemcpyHostToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[0], P.y.x, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[1], P.y.y, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[2], P.y.z2, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[3], P.y.pr, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[4], P.y.pi, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[5], P.y.g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[6], P.chi, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[7], c->orig, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(d + off[8], c->A, sizeof(cplx<R>) * (size_t)c->nmesh(), hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipStreamSynchronize(s));
+    return PUFF_OK;
+}
+
+template <typename R>
+static int state_load(Ctx* c, const void* src) {
+    Particles<R>& P = parts<R>(c);
+    size_t off[9];
+    blob_layout<R>(c, off);
+    if (!c->h_hdr) PUFF_HIP(hipHostMalloc(&c->h_hdr, 256, hipHostMallocDefault));
+    const char* d = reinterpret_cast<const char*>(src);
+    hipStream_t s = c->stream;
+    PUFF_HIP(hipMemcpyAsync(c->h_hdr, d, sizeof(BlobHeader), hipMemcpyDeviceToHost, s));
+    PUFF_HIP(hipStreamSynchronize(s));
+    const BlobHeader hd = *reinterpret_cast<const BlobHeader*>(c->h_hdr);
+    if (hd.magic != BLOB_MAGIC || hd.n != c->n || hd.nmesh != c->nmesh() || hd.precision != c->precision) {
+        set_error("state blob does not belong to this context's shape and precision");
+        return PUFF_ERR_ARG;
+    }
+    const size_t n = (size_t)c->n;
+    PUFF_HIP(hipMemcpyAsync(P.y.x, d + off[0], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(P.y.y, d + off[1], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(P.y.z2, d + off[2], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(P.y.pr, d + off[3], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(P.y.pi, d + off[4], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(P.y.g, d + off[5], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(P.chi, d + off[6], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(c->orig, d + off[7], sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipMemcpyAsync(c->A, d + off[8], sizeof(cplx<R>) * (size_t)c->nmesh(), hipMemcpyDeviceToDevice, s));
+    PUFF_HIP(hipStreamSynchronize(s));
+    c->sorted = hd.sorted != 0;
+    c->identity_order = hd.identity_order != 0;
+    c->order_id = hd.order_id;
+    c->steps_since_sort = hd.steps_since_sort;
+    c->n2col = hd.n2col;
+    c->n2col0 = hd.n2col0;
+    c->undgrad = hd.undgrad;
+    return PUFF_OK;
+}
+
+template <typename R>
+static int sort_now(Ctx* c) {
+    sort_particles<R>(c);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipStreamSynchronize(c->stream));
     return PUFF_OK;
 }
 
@@ -1065,6 +1544,70 @@ int puff_particle_count(puff_ctx* ctx, int64_t* n) {
     if (!ctx || !n) return PUFF_ERR_ARG;
     *n = ctx->c.n;
     return PUFF_OK;
+}
+
+int puff_state_bytes(puff_ctx* ctx, int64_t* bytes) {
+    if (!ctx || !bytes) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, state_bytes, bytes);
+}
+
+int puff_state_save(puff_ctx* ctx, void* dst) {
+    CTX_CHECK(ctx);
+    if (!dst) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, state_save, dst);
+}
+
+int puff_state_load(puff_ctx* ctx, const void* src) {
+    CTX_CHECK(ctx);
+    if (!src) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, state_load, src);
+}
+
+int puff_sort_now(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    return ctx->c.precision == PUFF_FP64 ? puff::sort_now<double>(&ctx->c) : puff::sort_now<float>(&ctx->c);
+}
+
+int puff_adjoint_enable(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    return ctx->c.precision == PUFF_FP64 ? puff::adjoint_enable<double>(&ctx->c)
+                                         : puff::adjoint_enable<float>(&ctx->c);
+}
+
+int puff_adjoint_zero(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    return ctx->c.precision == PUFF_FP64 ? puff::adjoint_zero<double>(&ctx->c)
+                                         : puff::adjoint_zero<float>(&ctx->c);
+}
+
+int puff_adjoint_seed_power(puff_ctx* ctx, const double* wz, double scale) {
+    CTX_CHECK(ctx);
+    if (!wz) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, adjoint_seed_power, wz, scale);
+}
+
+int puff_rk4par_adjoint(puff_ctx* ctx, double sz, double h, double* dn2col, int32_t* slot) {
+    CTX_CHECK(ctx);
+    return DISPATCH(ctx, rk4par_adjoint, sz, h, dn2col, slot);
+}
+
+int puff_diffraction_adjoint(puff_ctx* ctx, double h) {
+    CTX_CHECK(ctx);
+    return ctx->c.precision == PUFF_FP64 ? puff::diffraction_adjoint<double>(&ctx->c, h)
+                                         : puff::diffraction_adjoint<float>(&ctx->c, h);
+}
+
+int puff_adjoint_upload(puff_ctx* ctx, int32_t kind, const double* x, const double* y, const double* z2,
+                        const double* px, const double* py, const double* gam, const double* ar,
+                        const double* ai) {
+    CTX_CHECK(ctx);
+    return DISPATCH(ctx, adjoint_upload, kind, x, y, z2, px, py, gam, ar, ai);
+}
+
+int puff_adjoint_download(puff_ctx* ctx, int32_t kind, double* x, double* y, double* z2, double* px,
+                          double* py, double* gam, double* ar, double* ai) {
+    CTX_CHECK(ctx);
+    return DISPATCH(ctx, adjoint_download, kind, x, y, z2, px, py, gam, ar, ai);
 }
 
 }  // extern "C"

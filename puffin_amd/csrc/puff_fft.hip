@@ -122,14 +122,51 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
     const int nb = (int)min<int64_t>(B, nlines - L0);
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.ax.tw);
     cplx<R>* base = a.a + L0 * N;
+    const int64_t ntr = (int64_t)a.nx * a.ny;
     for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
         const int b = w / N;
-        s0[w] = (b < nb) ? base[w] : cplx<R>{0, 0};
+        cplx<R> v = (b < nb) ? base[w] : cplx<R>{0, 0};
+        if (op == XOP_ADJ_FOLD_SAVE_FWD && b < nb) {
+            const int64_t L = L0 + b;
+            if (a.periodic && L < a.ny) {
+                // the output's aliased plane copies plane 1: fold its cotangent
+                cplx<R>* al = a.a + (int64_t)(a.nz2 - 1) * ntr + L * N + (w % N);
+                const cplx<R> t = *al;
+                v.x += t.x;
+                v.y += t.y;
+                *al = cplx<R>{0, 0};
+            }
+            a.anb[L0 * N + w] = v;
+        }
+        s0[w] = v;
     }
     __syncthreads();
-    const int first_sign = (op == XOP_FORWARD) ? sign : +1;
+    const int first_sign = (op == XOP_FORWARD) ? sign : (op == XOP_ADJ_FOLD_SAVE_FWD ? -1 : +1);
     cplx<R>* res = lds_fft<R, false>(s0, s1, B, a.ax, tw, first_sign);
-    if (op == XOP_INV_SPLIT_FWD) {
+    if (op == XOP_ADJ_INV_MIX_FWD) {
+        // adjoint of the absorption split: (1 - m) cotangent + m W^H G W cotangent / ffact
+        const cplx<R>* lam = a.anb + L0 * N;
+        for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
+            const int b = w / N, j = w % N;
+            const int64_t L = L0 + b;
+            const int iy = (int)(L % a.ny), iz = (int)(L / a.ny);
+            const double mxy = a.mxy[(int64_t)iy * a.nx + j];
+            const double m = mxy + (a.mz[iz] * (1.0 - mxy));
+            const cplx<R> v = res[w];
+            const cplx<R> l = lam[w];
+            res[w] = {(R)((1.0 - m) * (double)l.x + m * ((double)v.x / a.ffact)),
+                      (R)((1.0 - m) * (double)l.y + m * ((double)v.y / a.ffact))};
+        }
+        __syncthreads();
+        cplx<R>* other = (res == s0) ? s1 : s0;
+        res = lds_fft<R, false>(res, other, B, a.ax, tw, -1);
+    } else if (op == XOP_ADJ_INV_SCALE) {
+        for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
+            const cplx<R> v = res[w];
+            res[w] = {(R)((double)v.x / a.ffact), (R)((double)v.y / a.ffact)};
+        }
+        __syncthreads();
+    } else if (op == XOP_INV_SPLIT_FWD) {
         // /ffact then AbsorptionStep's split (diffraction.f90:310, 454-470)
         cplx<R>* anb = a.anb + L0 * N;
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
@@ -163,7 +200,6 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
     for (int w = threadIdx.x; w < nb * N; w += blockDim.x) base[w] = res[w];
     if (op == XOP_INV_RECOMBINE && a.periodic) {
         // keep the aliased plane NZ2 equal to plane 1 (para_field.f90:3197-3244)
-        const int64_t ntr = (int64_t)a.nx * a.ny;
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
             const int64_t L = L0 + w / N;
             if (L < a.ny) a.a[(int64_t)(a.nz2 - 1) * ntr + L * N + (w % N)] = res[w];
@@ -305,6 +341,30 @@ void launch_diffraction(const DiffArgs<R>& a, hipStream_t s) {
     run_x(a, XOP_INV_RECOMBINE, 0, s);
 }
 
+// Adjoint of launch_diffraction.  With W the unnormalised forward transform,
+// E the multiplyexp factor (with its filter), G the real damping and M the
+// real mask, the forward step is D = Q P with P = W^H E W / ffact and
+// Q = (1 - M) + W^H G W M / ffact, so D^H = P^H Q^H with
+// Q^H = (1 - M) + M W^H G W / ffact and P^H = W^H conj(E) W / ffact:
+//   x fwd (fold alias, keep copy) | y fwd | z fwd . damping . z inv | y inv |
+//   x inv . /ffact . mix with copy . x fwd | y fwd | z fwd . conj(E) . z inv |
+//   y inv | x inv . /ffact
+template <typename R>
+void launch_diffraction_adjoint(const DiffArgs<R>& a, hipStream_t s) {
+    DiffArgs<R> c = a;
+    run_x(c, XOP_ADJ_FOLD_SAVE_FWD, 0, s);
+    run_y(c, -1, s);
+    run_z(c, SPEC_ABSORB, 0, s);
+    run_y(c, +1, s);
+    run_x(c, XOP_ADJ_INV_MIX_FWD, 0, s);
+    run_y(c, -1, s);
+    c.h = -a.h;   // conj(E): the propagator phase with -h (the filter is real)
+    run_z(c, SPEC_PROPAGATE, 0, s);
+    c.h = a.h;
+    run_y(c, +1, s);
+    run_x(c, XOP_ADJ_INV_SCALE, 0, s);
+}
+
 template <typename R>
 void launch_fft3d(cplx<R>* data, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
                   const FftAxis& az, int sign, hipStream_t s) {
@@ -324,6 +384,8 @@ void launch_fft3d(cplx<R>* data, int nx, int ny, int nz, const FftAxis& ax, cons
 
 template void launch_diffraction<double>(const DiffArgs<double>&, hipStream_t);
 template void launch_diffraction<float>(const DiffArgs<float>&, hipStream_t);
+template void launch_diffraction_adjoint<double>(const DiffArgs<double>&, hipStream_t);
+template void launch_diffraction_adjoint<float>(const DiffArgs<float>&, hipStream_t);
 template void launch_fft3d<double>(cplx<double>*, int, int, int, const FftAxis&, const FftAxis&,
                                    const FftAxis&, int, hipStream_t);
 template void launch_fft3d<float>(cplx<float>*, int, int, int, const FftAxis&, const FftAxis&,

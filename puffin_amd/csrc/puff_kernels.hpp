@@ -73,6 +73,63 @@ template <typename R>
 void launch_combine(const CombineArgs<R>& a, hipStream_t s);
 void launch_wrap(double* z2, int64_t n, double lenz2, hipStream_t s);
 
+// ---- discrete adjoint of one RK4 step -----------------------------------------
+// Stage s (4 .. 1) pulls the cotangent a_s = ws L + cnext mu_{s+1} of its
+// derivative k_s back through f at the trial state y0 + cin k_{s-1} and trial
+// field A0 + cin dA_{s-1}: mu_s = J_f^T a_s (particles into mout, field into
+// mAout).  Stage 1 then forms L <- L + mu_1 + mu_2 + mu_3 + mu_4.
+constexpr int DN2_COPIES = 64;   // spread accumulators of dJ/dn2col per stage
+
+template <typename R>
+struct AdjArgs {
+    int64_t n;
+    PartPtrs<R> y0;
+    PartPtrs<R> kin;       // k_{s-1} (stages 2-4)
+    const R* chi;
+    double cin;
+    const cplx<R>* A0;
+    const cplx<R>* dAin;   // dA_{s-1} (stages 2-4)
+    PartPtrs<R> L;         // cotangent of the step's end state
+    const cplx<R>* LA;
+    PartPtrs<R> mnext;     // mu_{s+1} (stages 1-3)
+    const cplx<R>* mAnext;
+    double cnext;
+    double ws;             // w_s h / 6, w = (1, 2, 2, 1)
+    PartPtrs<R> mout;      // mu_s (stages 2-4)
+    PartPtrs<R> m3, m4;    // stage 1: mu_3, mu_4 (mu_2 = mnext)
+    cplx<R>* mAout;
+    MeshGeom mesh;
+    Physics phys;
+    StageScalars s;
+    int* flags;
+    Box* box;
+    double* dn2;           // DN2_COPIES accumulators of this stage
+};
+
+template <typename R>
+struct AdjCombineArgs {
+    cplx<R>* LA;
+    cplx<R>* mA[4];
+    cplx<R>* dA[3];        // forward sources of stages 1-3, cleared here
+    const Box* box;
+    int nx, ny, nz2, nzp, periodic, one_d;
+};
+
+template <typename R>
+void launch_adj_stage(int stage, const AdjArgs<R>& a, hipStream_t s);
+template <typename R>
+void launch_adj_combine(const AdjCombineArgs<R>& a, hipStream_t s);
+// L_A += scale 2 wz[iz] wxy |A|^2 gradient, i.e. the cotangent of
+// sum_iz wz[iz] power(iz) with power the transverse trapezoid of |A|^2.
+template <typename R>
+void launch_adj_seed_power(cplx<R>* LA, const cplx<R>* A, const double* wz, int nx, int ny,
+                           int nz2, int periodic, double dx, double dy, double scale,
+                           hipStream_t s);
+
+template <typename R>
+void launch_fold_alias(cplx<R>* LA, int64_t ntr, int nz2, hipStream_t s);
+void launch_fill_zero_words(uint32_t* p, int64_t n, hipStream_t s);
+
 // ---- particle sort by cell --------------------------------------------------
 template <typename R>
 struct SortArgs {
@@ -109,7 +166,11 @@ enum SpecOp { SPEC_PROPAGATE = 0, SPEC_ABSORB = 1 };
 enum XOp {
     XOP_FORWARD = 0,        // forward transform only
     XOP_INV_SPLIT_FWD = 1,  // inverse, /ffact, split into boundary and interior, forward
-    XOP_INV_RECOMBINE = 2   // inverse, /ffact, + interior
+    XOP_INV_RECOMBINE = 2,  // inverse, /ffact, + interior
+    // adjoint pipeline
+    XOP_ADJ_FOLD_SAVE_FWD = 3,  // fold the aliased periodic plane, keep a copy, forward
+    XOP_ADJ_INV_MIX_FWD = 4,    // inverse, /ffact, (1 - m) copy + m value, forward
+    XOP_ADJ_INV_SCALE = 5       // inverse, /ffact
 };
 
 template <typename R>
@@ -128,6 +189,9 @@ struct DiffArgs {
 
 template <typename R>
 void launch_diffraction(const DiffArgs<R>& a, hipStream_t s);
+// Adjoint of launch_diffraction applied to the cotangent held in a.a.
+template <typename R>
+void launch_diffraction_adjoint(const DiffArgs<R>& a, hipStream_t s);
 
 // Plain 3-D transform (tests): sign -1 forward, +1 backward, unnormalised.
 template <typename R>

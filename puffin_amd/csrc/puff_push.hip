@@ -14,87 +14,9 @@
 //   equations       interaction/puffin_equations.f90:17-176
 //   b-fields        undulator/bfields.f90:36-881 (per-stage coefficients are
 //                   computed on the host, see puff_ctx.hip)
-#include "puff_common.hpp"
-#include "puff_kernels.hpp"
+#include "puff_stage.hpp"
 
 namespace puff {
-
-// Cell of one particle and its interpolation fractions.
-template <typename R>
-struct Cell {
-    int ix, iy, iz0, iz1;
-    R fx, fy, fz;   // x_in2, y_in2, z2_in2
-};
-
-// Node and fractions of a particle (getInterps_3D / _1D).  Returns flag bits;
-// 0 means every node index is inside the mesh.
-template <typename R>
-__device__ __forceinline__ int locate(const MeshGeom& m, R x, R y, double z2,
-                                      Cell<R>& c) {
-    int flag = 0;
-    const double zq = floor(z2 * m.inv_dz2);
-    if (m.periodic) {
-        if (!(fabs(zq) < 1.0e9)) return FLAG_Z2MESH;
-        long long q = (long long)zq;
-        long long r = q % m.nzp;
-        if (r < 0) r += m.nzp;
-        c.iz0 = (int)r;
-        c.iz1 = (c.iz0 + 1 == m.nzp) ? 0 : c.iz0 + 1;
-    } else {
-        // temporal mesh: z2node >= NZ2 stops the reference
-        // (system_interpolation.f90:70-75); z2 < 0 is refused too.
-        if (!(zq >= 0.0 && zq <= (double)(m.nzp - 2))) return FLAG_Z2MESH;
-        c.iz0 = (int)zq;
-        c.iz1 = c.iz0 + 1;
-    }
-    const double locz2 = z2 - zq * m.dz2;
-    c.fz = (R)(locz2 * m.inv_dz2);
-    if (m.one_d) {
-        c.ix = 0;
-        c.iy = 0;
-        c.fx = 0;
-        c.fy = 0;
-        return 0;
-    }
-    const R tx = (x + (R)m.halfx) * (R)m.inv_dx;
-    const R ty = (y + (R)m.halfy) * (R)m.inv_dy;
-    const R xq = floor(tx), yq = floor(ty);
-    // xnode >= nspinDX .or. xnode < 1 (system_interpolation.f90:60-68)
-    if (!(xq >= (R)0 && xq <= (R)(m.nx - 2))) flag |= FLAG_TRANSVERSE;
-    if (!(yq >= (R)0 && yq <= (R)(m.ny - 2))) flag |= FLAG_TRANSVERSE;
-    if (flag) return flag;
-    c.ix = (int)xq;
-    c.iy = (int)yq;
-    c.fx = (x + (R)m.halfx - xq * (R)m.dx) * (R)m.inv_dx;
-    c.fy = (y + (R)m.halfy - yq * (R)m.dy) * (R)m.inv_dy;
-    return 0;
-}
-
-// Scaled undulator field at one particle from the stage coefficients
-// (bfields.f90): see stage_scalars() on the host for the coefficient map.
-template <typename R>
-__device__ __forceinline__ void bfield(const StageScalars& s, R x, R y, R& bx,
-                                       R& by, R& bz) {
-    if (s.und_type == PUFF_UND_PLANEPOLE) {
-        bx = (R)s.c[3] * y;
-        by = (R)s.c[1] + (R)s.c[4] * x;
-        bz = (R)s.c[2] * sinh((R)s.c[7] * y);
-    } else if (s.und_type == PUFF_UND_CURVED) {
-        const R kx = (R)s.c[5], ky = (R)s.c[6];
-        bx = (R)s.c[0] * sinh(kx * x) * sinh(ky * y) + (R)s.c[3] * y;
-        by = (R)s.c[1] * cosh(kx * x) * cosh(ky * y) + (R)s.c[4] * x;
-        bz = (R)s.c[2] * cosh(kx * x) * sinh(ky * y);
-    } else {
-        bx = (R)s.c[0] + (R)s.c[3] * y;
-        by = (R)s.c[1] + (R)s.c[4] * x;
-        bz = (R)s.c[5] * x + (R)s.c[6] * y;
-    }
-}
-
-template <typename R>
-__device__ __forceinline__ void atomic_add(R* p, R v) {
-    unsafeAtomicAdd(p, v);
-}
 
 // One RK4 stage.  STAGE = 1..4 (rk4par's four derivs calls).
 template <typename R, int STAGE>
@@ -151,85 +73,17 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     const bool ok = in && flag == 0;
     if (flag) atomicOr(a.flags, flag);
 
-    // ---- bounding box of this workgroup's cells (wavefront, then workgroup
-    // reduction); max values are reduced as min of their negatives
-    __shared__ int s_red[6][STAGE_THREADS / 64];
-    __shared__ int s_bb[6];
+    // ---- node box of this workgroup's cells and the step's active box
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
-    {
-        int v[6] = {ok ? c.ix : 0x7fffffff, ok ? -c.ix : 1,
-                    ok ? c.iy : 0x7fffffff, ok ? -c.iy : 1,
-                    ok ? c.iz0 : 0x7fffffff, ok ? -c.iz0 : 1};
-#pragma unroll
-        for (int d = 0; d < 6; ++d) {
-            v[d] = wave_min(v[d]);
-            if (lane == 0) s_red[d][wv] = v[d];
-        }
-        __syncthreads();
-        if (threadIdx.x < 6) {
-            const int d = threadIdx.x;
-            int r = s_red[d][0];
-            for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = min(r, s_red[d][k]);
-            s_bb[d] = (d & 1) ? -r : r;
-            // active box of the step: one atomic per workgroup into one of
-            // BOX_COPIES spread copies (same-address atomics would serialise)
-            const bool empty = (d & 1) ? (r == 1) : (r == 0x7fffffff);
-            if (!empty) {
-                Box* b = a.box + (blockIdx.x % BOX_COPIES);
-                if (d & 1) atomicMax(&b->hi[d >> 1], -r);
-                else atomicMin(&b->lo[d >> 1], r);
-            }
-        }
-        __syncthreads();
-    }
-    const bool any_ok = s_bb[1] >= 0;
-    const int bx0 = s_bb[0], by0 = s_bb[2], bz0 = s_bb[4];
-    const int tx = any_ok ? (m.one_d ? 1 : s_bb[1] - bx0 + 2) : 0;
-    const int ty = any_ok ? (m.one_d ? 1 : s_bb[3] - by0 + 2) : 0;
-    const int tz = any_ok ? s_bb[5] - bz0 + 2 : 0;
-    // tile-private deposit when the workgroup's node box fits in LDS and does
-    // not wrap around a periodic mesh
-    const bool use_tile = ph.field_evolve && any_ok && (int64_t)tx * ty * tz <= TILE_NODES &&
-                          !(m.periodic && s_bb[5] + 1 >= m.nzp);
+    const TileBox tb = workgroup_box<R>(m, ok, c, a.box);
+    const bool use_tile = ph.field_evolve && tile_fits(m, tb);
 
     // ---- nodes and tri-linear weights (zero weights on lanes without a cell)
+    const int nnb = m.one_d ? 2 : 8;
     R w[8];
     int nd[8];
-    int lt[8];    // tile index of each corner
-    int nnb, key;
-    if (m.one_d) {
-        // lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1)
-        nnb = 2;
-        w[0] = ok ? (R)1 - c.fz : (R)0;
-        w[1] = ok ? (R)1 - ((R)1 - c.fz) : (R)0;
-        nd[0] = ok ? c.iz0 : 0;
-        nd[1] = ok ? c.iz1 : 0;
-        lt[0] = ok ? c.iz0 - bz0 : 0;
-        lt[1] = lt[0] + 1;
-        key = ok ? c.iz0 : -1 - lane;
-    } else {
-        nnb = 8;
-        const R x1 = (R)1 - c.fx, x2 = c.fx;
-        const R y1 = (R)1 - c.fy, y2 = c.fy;
-        const R z1 = (R)1 - c.fz, z2w = c.fz;
-        w[0] = x1 * y1 * z1;  w[1] = x2 * y1 * z1;  w[2] = x1 * y2 * z1;  w[3] = x2 * y2 * z1;
-        w[4] = x1 * y1 * z2w; w[5] = x2 * y1 * z2w; w[6] = x1 * y2 * z2w; w[7] = x2 * y2 * z2w;
-        const int ntr = m.nx * m.ny;
-        const int b0 = ok ? c.iz0 * ntr + c.iy * m.nx + c.ix : 0;
-        const int b1 = ok ? c.iz1 * ntr + c.iy * m.nx + c.ix : 0;
-        nd[0] = b0; nd[1] = b0 + 1; nd[2] = b0 + m.nx; nd[3] = b0 + m.nx + 1;
-        nd[4] = b1; nd[5] = b1 + 1; nd[6] = b1 + m.nx; nd[7] = b1 + m.nx + 1;
-        const int t0 = ok ? ((c.iz0 - bz0) * ty + (c.iy - by0)) * tx + (c.ix - bx0) : 0;
-        const int txy = tx * ty;
-        lt[0] = t0; lt[1] = t0 + 1; lt[2] = t0 + tx; lt[3] = t0 + tx + 1;
-        lt[4] = t0 + txy; lt[5] = t0 + txy + 1; lt[6] = t0 + txy + tx; lt[7] = t0 + txy + tx + 1;
-        key = ok ? b0 : -1 - lane;
-        if (!ok) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) w[k] = (R)0;
-        }
-    }
+    int t0, key;
+    cell_weights<R>(m, ok, c, tb, lane, w, nd, t0, key);
 
     // ---- gather F = sum_k w_k (A0 + c dA_{s-1})[node_k]  (getFFelecs)
     R fr = 0, fi = 0;
@@ -254,78 +108,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     if (ph.field_evolve) {
         const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
         const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
-        if (use_tile) {
-            // FP64 accumulation in an LDS tile, then one global atomic per
-            // touched node and component.  Each lane visits the corners in a
-            // rotated order so that lanes of one cell do not hit the same LDS
-            // address in the same instruction.
-            __shared__ double s_tile[2 * TILE_NODES];
-            const int nt = tx * ty * tz;
-            for (int t = threadIdx.x; t < 2 * nt; t += blockDim.x) s_tile[t] = 0.0;
-            __syncthreads();
-            if (ok) {
-                const int txy = tx * ty;
-                const int t0 = lt[0];
-                const double dsr = (double)sr, dsi = (double)si;
-                const double wx[2] = {(double)((R)1 - c.fx), (double)c.fx};
-                const double wy[2] = {(double)((R)1 - c.fy), (double)c.fy};
-                const double wz[2] = {(double)((R)1 - c.fz), (double)c.fz};
-                for (int q = 0; q < nnb; ++q) {
-                    const int k = (q + lane) & (nnb - 1);
-                    double wk;
-                    int tk;
-                    if (m.one_d) {
-                        wk = (double)w[0];
-                        if (k) wk = (double)w[1];
-                        tk = t0 + k;
-                    } else {
-                        wk = (double)((R)wx[k & 1] * (R)wy[(k >> 1) & 1] * (R)wz[k >> 2]);
-                        tk = t0 + (k & 1) + ((k & 2) ? tx : 0) + ((k & 4) ? txy : 0);
-                    }
-                    atomicAdd(&s_tile[2 * tk], wk * dsr);
-                    atomicAdd(&s_tile[2 * tk + 1], wk * dsi);
-                }
-            }
-            __syncthreads();
-            const int ntr = m.nx * m.ny;
-            for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-                const double tr = s_tile[2 * t], ti = s_tile[2 * t + 1];
-                if (tr != 0.0 || ti != 0.0) {
-                    const int lx = t % tx, r = t / tx;
-                    const int ly = r % ty, lz = r / ty;
-                    const int node = (bz0 + lz) * ntr + (by0 + ly) * m.nx + (bx0 + lx);
-                    atomic_add(&a.dAout[node].x, (R)tr);
-                    atomic_add(&a.dAout[node].y, (R)ti);
-                }
-            }
-        } else {
-            // Lanes of a wavefront that share a cell are summed first (runs of
-            // equal keys), then one global atomic per run and corner.
-            const int prev = __shfl_up(key, 1, 64);
-            const bool head = (lane == 0) || (prev != key);
-            const unsigned long long heads = __ballot(head);
-            const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
-            const int seg_end = after ? lane + __builtin_ctzll(after) : 63;
-            const int maxlen = wave_max(head ? seg_end - lane + 1 : 1);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k < nnb) {
-                    R vr = w[k] * sr, vi = w[k] * si;
-                    for (int o = 1; o < maxlen; o <<= 1) {
-                        const R tr = __shfl_down(vr, o, 64);
-                        const R ti = __shfl_down(vi, o, 64);
-                        if (lane + o <= seg_end) {
-                            vr += tr;
-                            vi += ti;
-                        }
-                    }
-                    if (ok && head) {
-                        atomic_add(&a.dAout[nd[k]].x, vr);
-                        atomic_add(&a.dAout[nd[k]].y, vi);
-                    }
-                }
-            }
-        }
+        scatter_cells<R>(m, tb, use_tile, ok, c, nnb, w, nd, t0, key, sr, si, a.dAout);
     }
     if (!in) return;
     if (!ph.coupling) { fr = 0; fi = 0; }   // rhs.f90:221-224
@@ -382,21 +165,6 @@ __global__ void k_step_begin(int* flags, Box* box) {
             box[k].hi[d] = -1;
         }
     }
-}
-
-// Union of the spread box copies.
-__device__ __forceinline__ Box merged_box(const Box* box) {
-    Box b;
-    for (int d = 0; d < 3; ++d) {
-        b.lo[d] = 0x7fffffff;
-        b.hi[d] = -1;
-    }
-    for (int k = 0; k < BOX_COPIES; ++k)
-        for (int d = 0; d < 3; ++d) {
-            b.lo[d] = min(b.lo[d], box[k].lo[d]);
-            b.hi[d] = max(b.hi[d], box[k].hi[d]);
-        }
-    return b;
 }
 
 // A += h/6 (dA1 + dA4 + 2 (dA2 + dA3)) over the active box

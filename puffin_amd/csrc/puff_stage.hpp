@@ -1,0 +1,310 @@
+// Device helpers shared by the forward RK4 stage kernels (puff_push.hip) and
+// their adjoints (puff_adjoint.hip): cell location, undulator fields, the
+// workgroup's cell box and the weighted scatter onto the mesh.
+//
+// Reference arithmetic restated (UKFELs/Puffin):
+//   getInterps_3D   interaction/system_interpolation.f90:25-98
+//   1-D branch      interaction/system_interpolation_1D.f90:24-179
+//   getSource_3D    interaction/system_interpolation.f90:150-248
+//   b-fields        undulator/bfields.f90:36-881 (per-stage coefficients are
+//                   computed on the host, see stage_scalars in puff_ctx.hip)
+#pragma once
+
+#include "puff_common.hpp"
+#include "puff_kernels.hpp"
+
+namespace puff {
+
+// Cell of one particle and its interpolation fractions.
+template <typename R>
+struct Cell {
+    int ix, iy, iz0, iz1;
+    R fx, fy, fz;   // x_in2, y_in2, z2_in2
+};
+
+// Node and fractions of a particle (getInterps_3D / _1D).  Returns flag bits;
+// 0 means every node index is inside the mesh.
+template <typename R>
+__device__ __forceinline__ int locate(const MeshGeom& m, R x, R y, double z2,
+                                      Cell<R>& c) {
+    int flag = 0;
+    const double zq = floor(z2 * m.inv_dz2);
+    if (m.periodic) {
+        if (!(fabs(zq) < 1.0e9)) return FLAG_Z2MESH;
+        long long q = (long long)zq;
+        long long r = q % m.nzp;
+        if (r < 0) r += m.nzp;
+        c.iz0 = (int)r;
+        c.iz1 = (c.iz0 + 1 == m.nzp) ? 0 : c.iz0 + 1;
+    } else {
+        // temporal mesh: z2node >= NZ2 stops the reference
+        // (system_interpolation.f90:70-75); z2 < 0 is refused too.
+        if (!(zq >= 0.0 && zq <= (double)(m.nzp - 2))) return FLAG_Z2MESH;
+        c.iz0 = (int)zq;
+        c.iz1 = c.iz0 + 1;
+    }
+    const double locz2 = z2 - zq * m.dz2;
+    c.fz = (R)(locz2 * m.inv_dz2);
+    if (m.one_d) {
+        c.ix = 0;
+        c.iy = 0;
+        c.fx = 0;
+        c.fy = 0;
+        return 0;
+    }
+    const R tx = (x + (R)m.halfx) * (R)m.inv_dx;
+    const R ty = (y + (R)m.halfy) * (R)m.inv_dy;
+    const R xq = floor(tx), yq = floor(ty);
+    // xnode >= nspinDX .or. xnode < 1 (system_interpolation.f90:60-68)
+    if (!(xq >= (R)0 && xq <= (R)(m.nx - 2))) flag |= FLAG_TRANSVERSE;
+    if (!(yq >= (R)0 && yq <= (R)(m.ny - 2))) flag |= FLAG_TRANSVERSE;
+    if (flag) return flag;
+    c.ix = (int)xq;
+    c.iy = (int)yq;
+    c.fx = (x + (R)m.halfx - xq * (R)m.dx) * (R)m.inv_dx;
+    c.fy = (y + (R)m.halfy - yq * (R)m.dy) * (R)m.inv_dy;
+    return 0;
+}
+
+// Scaled undulator field at one particle from the stage coefficients
+// (bfields.f90): see stage_scalars() on the host for the coefficient map.
+template <typename R>
+__device__ __forceinline__ void bfield(const StageScalars& s, R x, R y, R& bx,
+                                       R& by, R& bz) {
+    if (s.und_type == PUFF_UND_PLANEPOLE) {
+        bx = (R)s.c[3] * y;
+        by = (R)s.c[1] + (R)s.c[4] * x;
+        bz = (R)s.c[2] * sinh((R)s.c[7] * y);
+    } else if (s.und_type == PUFF_UND_CURVED) {
+        const R kx = (R)s.c[5], ky = (R)s.c[6];
+        bx = (R)s.c[0] * sinh(kx * x) * sinh(ky * y) + (R)s.c[3] * y;
+        by = (R)s.c[1] * cosh(kx * x) * cosh(ky * y) + (R)s.c[4] * x;
+        bz = (R)s.c[2] * cosh(kx * x) * sinh(ky * y);
+    } else {
+        bx = (R)s.c[0] + (R)s.c[3] * y;
+        by = (R)s.c[1] + (R)s.c[4] * x;
+        bz = (R)s.c[5] * x + (R)s.c[6] * y;
+    }
+}
+
+// Transverse derivatives of bfield(): g = {dbx/dx, dbx/dy, dby/dx, dby/dy,
+// dbz/dx, dbz/dy}.
+__device__ __forceinline__ void bfield_grad(const StageScalars& s, double x, double y,
+                                            double g[6]) {
+    g[0] = 0.0;
+    g[1] = s.c[3];
+    g[2] = s.c[4];
+    g[3] = 0.0;
+    if (s.und_type == PUFF_UND_PLANEPOLE) {
+        g[4] = 0.0;
+        g[5] = s.c[2] * s.c[7] * cosh(s.c[7] * y);
+    } else if (s.und_type == PUFF_UND_CURVED) {
+        const double kx = s.c[5], ky = s.c[6];
+        const double shx = sinh(kx * x), chx = cosh(kx * x), shy = sinh(ky * y), chy = cosh(ky * y);
+        g[0] = s.c[0] * kx * chx * shy;
+        g[1] += s.c[0] * ky * shx * chy;
+        g[2] += s.c[1] * kx * shx * chy;
+        g[3] = s.c[1] * ky * chx * shy;
+        g[4] = s.c[2] * kx * shx * shy;
+        g[5] = s.c[2] * ky * chx * chy;
+    } else {
+        g[4] = s.c[5];
+        g[5] = s.c[6];
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void atomic_add(R* p, R v) {
+    unsafeAtomicAdd(p, v);
+}
+
+// Node box of the cells of one workgroup (origin and extent in nodes, i.e.
+// cells + 1), and the step's active box: one atomic per workgroup into one
+// of BOX_COPIES spread copies (same-address atomics would serialise).  Must
+// be reached by every thread of the workgroup.
+struct TileBox {
+    int bx0, by0, bz0;
+    int tx, ty, tz;
+    int hz;        // last cell plane
+    bool any;
+};
+
+template <typename R>
+__device__ __forceinline__ TileBox workgroup_box(const MeshGeom& m, bool ok, const Cell<R>& c,
+                                                 Box* box) {
+    __shared__ int s_red[6][STAGE_THREADS / 64];
+    __shared__ int s_bb[6];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    // max values are reduced as min of their negatives
+    int v[6] = {ok ? c.ix : 0x7fffffff, ok ? -c.ix : 1,
+                ok ? c.iy : 0x7fffffff, ok ? -c.iy : 1,
+                ok ? c.iz0 : 0x7fffffff, ok ? -c.iz0 : 1};
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        v[d] = wave_min(v[d]);
+        if (lane == 0) s_red[d][wv] = v[d];
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        int r = s_red[d][0];
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = min(r, s_red[d][k]);
+        s_bb[d] = (d & 1) ? -r : r;
+        const bool empty = (d & 1) ? (r == 1) : (r == 0x7fffffff);
+        if (!empty) {
+            Box* b = box + (blockIdx.x % BOX_COPIES);
+            if (d & 1) atomicMax(&b->hi[d >> 1], -r);
+            else atomicMin(&b->lo[d >> 1], r);
+        }
+    }
+    __syncthreads();
+    TileBox t;
+    t.any = s_bb[1] >= 0;
+    t.bx0 = s_bb[0];
+    t.by0 = s_bb[2];
+    t.bz0 = s_bb[4];
+    t.hz = s_bb[5];
+    t.tx = t.any ? (m.one_d ? 1 : s_bb[1] - t.bx0 + 2) : 0;
+    t.ty = t.any ? (m.one_d ? 1 : s_bb[3] - t.by0 + 2) : 0;
+    t.tz = t.any ? s_bb[5] - t.bz0 + 2 : 0;
+    return t;
+}
+
+// Union of the spread box copies.
+__device__ __forceinline__ Box merged_box(const Box* box) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.lo[d] = 0x7fffffff;
+        b.hi[d] = -1;
+    }
+    for (int k = 0; k < BOX_COPIES; ++k)
+        for (int d = 0; d < 3; ++d) {
+            b.lo[d] = min(b.lo[d], box[k].lo[d]);
+            b.hi[d] = max(b.hi[d], box[k].hi[d]);
+        }
+    return b;
+}
+
+// Whether the workgroup's node box fits the LDS tile and does not wrap
+// around a periodic mesh (block-uniform).
+__device__ __forceinline__ bool tile_fits(const MeshGeom& m, const TileBox& t) {
+    return t.any && (int64_t)t.tx * t.ty * t.tz <= TILE_NODES && !(m.periodic && t.hz + 1 >= m.nzp);
+}
+
+// Tri-linear (or 1-D linear) weights, node indices and the key of a lane's
+// cell (unique negative key on lanes without a cell).
+template <typename R>
+__device__ __forceinline__ void cell_weights(const MeshGeom& m, bool ok, const Cell<R>& c,
+                                             const TileBox& t, int lane, R* w, int* nd,
+                                             int& t0, int& key) {
+    if (m.one_d) {
+        // lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1)
+        w[0] = ok ? (R)1 - c.fz : (R)0;
+        w[1] = ok ? (R)1 - ((R)1 - c.fz) : (R)0;
+        nd[0] = ok ? c.iz0 : 0;
+        nd[1] = ok ? c.iz1 : 0;
+        t0 = ok ? c.iz0 - t.bz0 : 0;
+        key = ok ? c.iz0 : -1 - lane;
+        return;
+    }
+    const R x1 = (R)1 - c.fx, x2 = c.fx;
+    const R y1 = (R)1 - c.fy, y2 = c.fy;
+    const R z1 = (R)1 - c.fz, z2w = c.fz;
+    w[0] = x1 * y1 * z1;  w[1] = x2 * y1 * z1;  w[2] = x1 * y2 * z1;  w[3] = x2 * y2 * z1;
+    w[4] = x1 * y1 * z2w; w[5] = x2 * y1 * z2w; w[6] = x1 * y2 * z2w; w[7] = x2 * y2 * z2w;
+    const int ntr = m.nx * m.ny;
+    const int b0 = ok ? c.iz0 * ntr + c.iy * m.nx + c.ix : 0;
+    const int b1 = ok ? c.iz1 * ntr + c.iy * m.nx + c.ix : 0;
+    nd[0] = b0; nd[1] = b0 + 1; nd[2] = b0 + m.nx; nd[3] = b0 + m.nx + 1;
+    nd[4] = b1; nd[5] = b1 + 1; nd[6] = b1 + m.nx; nd[7] = b1 + m.nx + 1;
+    t0 = ok ? ((c.iz0 - t.bz0) * t.ty + (c.iy - t.by0)) * t.tx + (c.ix - t.bx0) : 0;
+    key = ok ? b0 : -1 - lane;
+    if (!ok) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = (R)0;
+    }
+}
+
+// out[node_k] += w_k (sr, si) for every lane with a cell.
+//   use_tile: FP64 accumulation in an LDS tile over the workgroup's node box,
+//             then one global atomic per touched node and component.  Each
+//             lane visits the corners in a rotated order so that lanes of one
+//             cell do not hit the same LDS address in the same instruction.
+//   else:     lanes of a wavefront that share a cell are summed first (runs
+//             of equal keys), then one global atomic per run and corner.
+// use_tile must be block-uniform; every thread of the workgroup must call.
+template <typename R>
+__device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& t, bool use_tile,
+                                              bool ok, const Cell<R>& c, int nnb, const R* w,
+                                              const int* nd, int t0, int key, R sr, R si,
+                                              cplx<R>* out) {
+    const int lane = threadIdx.x & 63;
+    if (use_tile) {
+        __shared__ double s_tile[2 * TILE_NODES];
+        const int nt = t.tx * t.ty * t.tz;
+        for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) s_tile[q] = 0.0;
+        __syncthreads();
+        if (ok) {
+            const int txy = t.tx * t.ty;
+            const double dsr = (double)sr, dsi = (double)si;
+            const double wx[2] = {(double)((R)1 - c.fx), (double)c.fx};
+            const double wy[2] = {(double)((R)1 - c.fy), (double)c.fy};
+            const double wz[2] = {(double)((R)1 - c.fz), (double)c.fz};
+            for (int q = 0; q < nnb; ++q) {
+                const int k = (q + lane) & (nnb - 1);
+                double wk;
+                int tk;
+                if (m.one_d) {
+                    wk = (double)w[0];
+                    if (k) wk = (double)w[1];
+                    tk = t0 + k;
+                } else {
+                    wk = (double)((R)wx[k & 1] * (R)wy[(k >> 1) & 1] * (R)wz[k >> 2]);
+                    tk = t0 + (k & 1) + ((k & 2) ? t.tx : 0) + ((k & 4) ? txy : 0);
+                }
+                atomicAdd(&s_tile[2 * tk], wk * dsr);
+                atomicAdd(&s_tile[2 * tk + 1], wk * dsi);
+            }
+        }
+        __syncthreads();
+        const int ntr = m.nx * m.ny;
+        for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+            const double tr = s_tile[2 * q], ti = s_tile[2 * q + 1];
+            if (tr != 0.0 || ti != 0.0) {
+                const int lx = q % t.tx, r = q / t.tx;
+                const int ly = r % t.ty, lz = r / t.ty;
+                const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
+                atomic_add(&out[node].x, (R)tr);
+                atomic_add(&out[node].y, (R)ti);
+            }
+        }
+    } else {
+        const int prev = __shfl_up(key, 1, 64);
+        const bool head = (lane == 0) || (prev != key);
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+        const int seg_end = after ? lane + __builtin_ctzll(after) : 63;
+        const int maxlen = wave_max(head ? seg_end - lane + 1 : 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k < nnb) {
+                R vr = w[k] * sr, vi = w[k] * si;
+                for (int o = 1; o < maxlen; o <<= 1) {
+                    const R tr = __shfl_down(vr, o, 64);
+                    const R ti = __shfl_down(vi, o, 64);
+                    if (lane + o <= seg_end) {
+                        vr += tr;
+                        vi += ti;
+                    }
+                }
+                if (ok && head) {
+                    atomic_add(&out[nd[k]].x, vr);
+                    atomic_add(&out[nd[k]].y, vi);
+                }
+            }
+        }
+    }
+}
+
+}  // namespace puff

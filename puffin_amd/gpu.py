@@ -49,9 +49,22 @@ SIGNATURES = {
     "puff_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
                                          ctypes.c_int32]),
     "puff_active_box": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int32)]),
+    "puff_state_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "puff_state_save": (ctypes.c_int, [_vp, _vp]),
+    "puff_state_load": (ctypes.c_int, [_vp, _vp]),
+    "puff_sort_now": (ctypes.c_int, [_vp]),
+    "puff_adjoint_enable": (ctypes.c_int, [_vp]),
+    "puff_adjoint_zero": (ctypes.c_int, [_vp]),
+    "puff_adjoint_seed_power": (ctypes.c_int, [_vp, _dp, ctypes.c_double]),
+    "puff_rk4par_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _dp,
+                                           ctypes.POINTER(ctypes.c_int32)]),
+    "puff_diffraction_adjoint": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "puff_adjoint_upload": (ctypes.c_int, [_vp, ctypes.c_int32] + [_vp] * 8),
+    "puff_adjoint_download": (ctypes.c_int, [_vp, ctypes.c_int32] + [_vp] * 8),
 }
 
-PROFILE_KINDS = ("stage1", "stage2", "stage3", "stage4", "combine", "diffraction", "step_begin")
+PROFILE_KINDS = ("stage1", "stage2", "stage3", "stage4", "combine", "diffraction", "step_begin",
+                 "adj_stages", "adj_combine", "adj_diffraction", "adj_replay")
 
 
 class PuffinError(RuntimeError):
@@ -243,3 +256,59 @@ class Integrator:
         _check(lib().puff_bunching(self._h, harmonic, nslices, slice_len, out.ctypes.data_as(_dp)),
                "puff_bunching")
         return out
+
+    # ---- state snapshots (device blobs in the slot order)
+    def state_bytes(self) -> int:
+        b = ctypes.c_int64()
+        _check(lib().puff_state_bytes(self._h, ctypes.byref(b)), "puff_state_bytes")
+        return int(b.value)
+
+    def state_save(self, dev_ptr: int):
+        _check(lib().puff_state_save(self._h, dev_ptr), "puff_state_save")
+
+    def state_load(self, dev_ptr: int):
+        _check(lib().puff_state_load(self._h, dev_ptr), "puff_state_load")
+
+    def sort_now(self):
+        _check(lib().puff_sort_now(self._h), "puff_sort_now")
+
+    # ---- discrete adjoint
+    def adjoint_enable(self):
+        _check(lib().puff_adjoint_enable(self._h), "puff_adjoint_enable")
+
+    def adjoint_zero(self):
+        _check(lib().puff_adjoint_zero(self._h), "puff_adjoint_zero")
+
+    def adjoint_seed_power(self, wz: np.ndarray, scale: float = 1.0):
+        w = np.ascontiguousarray(wz, dtype=np.float64)
+        assert w.size == self.mp.nz2
+        _check(lib().puff_adjoint_seed_power(self._h, w.ctypes.data_as(_dp), scale),
+               "puff_adjoint_seed_power")
+
+    def rk4par_adjoint(self, sz: float, h: float):
+        """Pull the cotangent back through one step from the current state.
+        Returns (status, dJ/dn2col per stage, taper-table slot per stage)."""
+        dn = np.zeros(4)
+        slot = (ctypes.c_int32 * 4)()
+        st = _check(lib().puff_rk4par_adjoint(self._h, sz, h, dn.ctypes.data_as(_dp), slot),
+                    "puff_rk4par_adjoint")
+        return st, dn, np.array(list(slot), dtype=np.int64)
+
+    def diffraction_adjoint(self, h: float):
+        _check(lib().puff_diffraction_adjoint(self._h, h), "puff_diffraction_adjoint")
+
+    def adjoint_upload(self, lam: dict | None, lar=None, lai=None):
+        arrs = [np.ascontiguousarray(lam[k], dtype=np.float64) if lam is not None else None
+                for k in PARTICLE_KEYS]
+        fr = None if lar is None else np.ascontiguousarray(lar, dtype=np.float64).reshape(-1)
+        fi = None if lai is None else np.ascontiguousarray(lai, dtype=np.float64).reshape(-1)
+        _check(lib().puff_adjoint_upload(self._h, MEM_HOST, *[_ptr(a) for a in arrs], _ptr(fr), _ptr(fi)),
+               "puff_adjoint_upload")
+
+    def adjoint_download(self):
+        out = {k: np.empty(self.n) for k in PARTICLE_KEYS}
+        lar = np.empty(self.mp.nmesh)
+        lai = np.empty(self.mp.nmesh)
+        _check(lib().puff_adjoint_download(self._h, MEM_HOST, *[_ptr(out[k]) for k in PARTICLE_KEYS],
+                                           _ptr(lar), _ptr(lai)), "puff_adjoint_download")
+        return out, lar, lai
