@@ -180,6 +180,131 @@ def cpu_baseline(case, nsample: int, steps: int):
                         f"(no diffraction), FP64 C oracle with OpenMP atomics"))
 
 
+def accuracy_run(case, precision: int, steps: int):
+    """The module's schedule (rk4par, diffraction every iSteps4Diff steps) for
+    ``steps`` steps; returns power(z2), slice bunching |b| and the on-axis
+    spectrum of the final field."""
+    from puffin_amd import gpu
+    mp = case.mp
+    g = gpu.Integrator(mp, case.n, precision)
+    try:
+        g.upload_particles(case.particles)
+        g.upload_field(*case.field)
+        g.set_taper(case.n2col0, case.n2col0, case.undgrad)
+        szl = 0.0
+        t0 = time.perf_counter()
+        for istep in range(1, steps + 1):
+            st = g.rk4par(szl, case.h)
+            if st != 0:
+                raise RuntimeError(f"rk4par status {st} at step {istep}")
+            szl += case.h
+            if istep % case.steps4diff == 0:
+                g.diffraction_step(case.steps4diff * case.h)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        power = g.power()
+        lam = 4.0 * math.pi * mp.rho
+        nsl = int(math.ceil((mp.nz2 - 1) * mp.dz2 / lam))
+        bunch = g.bunching(1, nsl, lam)
+        ar, ai = g.download_field()
+    finally:
+        g.close()
+    ntr = mp.nx * mp.ny
+    c = (mp.ny // 2) * mp.nx + mp.nx // 2
+    axis = ar[c::ntr][: mp.nzp] + 1j * ai[c::ntr][: mp.nzp]
+    spec = np.abs(np.fft.fft(axis)) ** 2
+    spec = spec[: mp.nzp // 2 + 1]                  # one-sided (plotSpecInt.py:79-100)
+    return dict(power=power, bunching=bunch, spectrum=spec, wall_s=wall,
+                integrated_power=integrated_power(power, mp.dz2))
+
+
+def precision_report(case, steps: int):
+    """Error of the FP32-mixed form against FP64 per output after ``steps``."""
+    log(f"accuracy: {steps} steps FP64 and mixed")
+    r64 = accuracy_run(case, 0, steps)
+    r32 = accuracy_run(case, 1, steps)
+
+    def err(k):
+        a, b = r32[k], r64[k]
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+    p0 = integrated_power(np.asarray(case_power0(case)), case.mp.dz2)
+    out = dict(steps=steps, periods=steps * case.h / (4.0 * math.pi * case.mp.rho),
+               integrated_power_rel_err=abs(r32["integrated_power"] - r64["integrated_power"]) /
+               abs(r64["integrated_power"]),
+               power_z_max_rel_err=err("power"), bunching_max_rel_err=err("bunching"),
+               spectrum_max_rel_err=err("spectrum"),
+               power_gain=r64["integrated_power"] / p0 if p0 > 0 else None,
+               wall_s=dict(fp64=r64["wall_s"], mixed=r32["wall_s"]))
+    log(f"accuracy: integrated power rel err {out['integrated_power_rel_err']:.3g}, "
+        f"power(z) {out['power_z_max_rel_err']:.3g}, |b| {out['bunching_max_rel_err']:.3g}, "
+        f"spectrum {out['spectrum_max_rel_err']:.3g}")
+    return out
+
+
+def case_power0(case):
+    """power(z2) of the initial field (transverse trapezoid of |A|^2)."""
+    mp = case.mp
+    ar = case.field[0].reshape(mp.nz2, mp.ny, mp.nx)
+    ai = case.field[1].reshape(mp.nz2, mp.ny, mp.nx)
+    f = ar * ar + ai * ai
+    wx = np.full(mp.nx, mp.dx)
+    wx[[0, -1]] *= 0.5
+    wy = np.full(mp.ny, mp.dy)
+    wy[[0, -1]] *= 0.5
+    return np.einsum("zyx,y,x->z", f, wy, wx)
+
+
+def adjoint_bench(steps_check: bool = True):
+    """d(integrated power)/d(per-period a_w, seed amplitude) on the adjoint
+    workload (helical, periodic 33 x 33 x 256, 2^20 macroparticles, 16
+    periods, checkpoints every 30 steps): forward and gradient wall time and
+    central differences (h = 1e-6) of two components."""
+    from puffin_amd import gpu
+    from puffin_amd.adjoint import ModuleGradient, module_program
+    from puffin_amd.configs import case_c3
+    c = case_c3()
+    mp = c.mp
+    ops = module_program(c.nsteps, c.h, c.steps4diff, diffraction=True, periodic=True, redist_every=60)
+    drv = gpu.Integrator(mp, c.n, 0)
+    try:
+        drv.upload_particles(c.particles)
+        drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+        shape = (c.field[0] / c.info["seed_amp"], c.field[1] / c.info["seed_amp"])
+        eng = ModuleGradient(drv, ops, shape, every=30, taper_period=c.taper_period)
+        table, seed = c.taper_table, c.info["seed_amp"]
+        eng.evaluate(table, seed)             # warm-up
+        drv.synchronize()
+        t0 = time.perf_counter()
+        J = eng.evaluate(table, seed)
+        t1 = time.perf_counter()
+        eng.forward(table, seed)
+        t2 = time.perf_counter()
+        g = eng.backward()
+        t3 = time.perf_counter()
+        out = dict(workload=WORKLOADS["c3"] + ", 16 periods of 30 steps", J=J,
+                   forward_s=t1 - t0, forward_checkpointed_s=t2 - t1, backward_s=t3 - t2,
+                   gradient_cost_vs_forward=(t3 - t1) / (t1 - t0), checkpoint_every=30,
+                   snapshot_bytes=eng.memory_bytes(), grad_seed=g["seed"],
+                   grad_table=[float(v) for v in g["table"]])
+        if steps_check:
+            eps = 1e-6
+            fd_s = (eng.evaluate(table, seed * (1 + eps)) - eng.evaluate(table, seed * (1 - eps))) / (2 * eps * seed)
+            k = int(np.argmax(np.abs(g["table"])))
+            tp, tm = table.copy(), table.copy()
+            tp[k] += eps
+            tm[k] -= eps
+            fd_t = (eng.evaluate(tp, seed) - eng.evaluate(tm, seed)) / (2 * eps)
+            out["fd_check"] = dict(h=eps, seed_rel_err=abs(fd_s - g["seed"]) / abs(g["seed"]),
+                                   table_index=k,
+                                   table_rel_err=abs(fd_t - g["table"][k]) / np.abs(g["table"]).max())
+        log(f"adjoint: forward {out['forward_s']:.3f} s, gradient {t3 - t1:.3f} s "
+            f"({out['gradient_cost_vs_forward']:.2f}x)")
+        return out
+    finally:
+        drv.close()
+
+
 def calibrate_copy(mib: int):
     """A plain device-to-device copy of known size (reads and writes mib MiB
     each, 5 times), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -207,6 +332,9 @@ def main():
                     help="workload (the JSON line is quoted on c1)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--accuracy-periods", type=int, default=10,
+                    help="undulator periods of the FP64-vs-mixed accuracy run (0: skip)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient measurement")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -278,6 +406,10 @@ def main():
                              step_roofline=mx["step_roofline"], kernels=mx["kernels"],
                              integrated_power_rel_err=perr,
                              power_err_after_steps=args.warmup + args.steps)
+    if "mixed" in line and args.case == "c1" and args.accuracy_periods > 0:
+        line["mixed"]["accuracy"] = precision_report(case, 30 * args.accuracy_periods)
+    if not args.no_adjoint and args.case == "c1":
+        line["adjoint"] = adjoint_bench()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_sample, args.cpu_steps)
     else:

@@ -117,7 +117,7 @@ def _helical_like(name, idx, nx, nz2, n, periods, periodic, seed_amp=0.0, aw_vec
     if seed_amp:
         ar, ai = gaussian_seed(mp, seed_amp / math.sqrt(2.0), seed_amp / math.sqrt(2.0), w0=2.0)
     c = Case(name, mp, p, (ar, ai), h, nsteps, steps_for_diffraction(rho, h), n2col0=n2col0,
-             info=dict(gamma_r=gamma_r, sig_x=sig_x, n_per_macro=n_per_macro))
+             info=dict(gamma_r=gamma_r, sig_x=sig_x, n_per_macro=n_per_macro, seed_amp=seed_amp))
     if aw_vec is not None:
         c.taper_table = np.asarray(aw_vec, dtype=np.float64)
         c.taper_period = 4.0 * PI_REF * rho

@@ -55,9 +55,19 @@ struct Box {
 constexpr int BOX_COPIES = 64;
 
 // Workgroup of the RK4 stage kernels (one macroparticle per thread) and the
-// node capacity of its LDS deposit tile (2 FP64 per node: 32 KiB).
+// node capacity of its LDS deposit tile (2 FP64 per node: 64 KiB of the
+// CU's 160 KiB).
 constexpr int STAGE_THREADS = 1024;
-constexpr int TILE_NODES = 2048;
+#ifndef PUFF_TILE_NODES
+#define PUFF_TILE_NODES 4096
+#endif
+constexpr int TILE_NODES = PUFF_TILE_NODES;
+
+// flags[FLAG_WORD_FALLBACK] counts the stage workgroups of a step whose node
+// box did not fit the LDS tile (global-atomic deposit): the host re-sorts the
+// particles when that share grows.
+constexpr int FLAG_WORDS = 2;
+constexpr int FLAG_WORD_FALLBACK = 1;
 
 // Wavefront (64-lane) reductions.
 __device__ __forceinline__ int wave_min(int v) {

@@ -88,6 +88,10 @@ struct Ctx {
     bool sorted = false;
     int sort_every = 30;
     int64_t steps_since_sort = 0;
+    // adaptive re-sort: share of stage workgroups that fell back to global
+    // atomics on the first step after the last sort, and the due flag
+    double fallback_base = 0.0;
+    bool sort_due = false;
     // kernel timing (puff_profile_*)
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -496,9 +500,9 @@ static int create(Ctx* c) {
     if (c->precision == PUFF_FP64) PUFF_TRY(alloc_particles<double>(c));
     else PUFF_TRY(alloc_particles<float>(c));
 
-    PUFF_TRY(dalloc(c, (void**)&c->flags, sizeof(int)));
+    PUFF_TRY(dalloc(c, (void**)&c->flags, sizeof(int) * FLAG_WORDS));
     PUFF_TRY(dalloc(c, (void**)&c->box, sizeof(Box) * BOX_COPIES));
-    PUFF_HIP(hipHostMalloc((void**)&c->h_flags, sizeof(int), hipHostMallocDefault));
+    PUFF_HIP(hipHostMalloc((void**)&c->h_flags, sizeof(int) * FLAG_WORDS, hipHostMallocDefault));
     c->stage_len = 2 * std::max<int64_t>(c->n, c->nmesh());
     PUFF_TRY(dalloc(c, (void**)&c->stage_buf, sizeof(double) * c->stage_len));
     c->diag_len = 3 * 4096 + c->p.nz2;
@@ -618,7 +622,7 @@ static int rk4par(Ctx* c, double sz, double h) {
             return st;
         }
     }
-    if (c->sort_every > 0 && (!c->sorted || c->steps_since_sort >= c->sort_every)) {
+    if (c->sort_every > 0 && (!c->sorted || c->sort_due || c->steps_since_sort >= c->sort_every)) {
         sort_particles<R>(c);
     }
     c->steps_since_sort++;
@@ -669,10 +673,19 @@ static int rk4par(Ctx* c, double sz, double h) {
     launch_combine<R>(ca, s);
     prof_stop(c, slot);
     PUFF_HIP(hipGetLastError());
-    PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int) * FLAG_WORDS, hipMemcpyDeviceToHost, s));
     PUFF_HIP(hipStreamSynchronize(s));
     prof_collect(c);
-    const int f = *c->h_flags;
+    const int f = c->h_flags[0];
+    if (c->sort_every > 0 && c->p.field_evolve) {
+        // particles drift out of their sorted runs (faster once the beam's
+        // energy spread has grown): re-sort when the share of workgroups that
+        // could not use the LDS tile exceeds twice its value after the sort
+        const double groups = 4.0 * (double)((c->n + STAGE_THREADS - 1) / STAGE_THREADS);
+        const double share = groups > 0 ? (double)c->h_flags[FLAG_WORD_FALLBACK] / groups : 0.0;
+        if (c->steps_since_sort == 1) c->fallback_base = share;
+        c->sort_due = share > std::max(0.02, 2.0 * c->fallback_base);
+    }
     if (f) {
         c->n2col = saved[0];
         c->n2col0 = saved[1];
@@ -1079,7 +1092,7 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
     launch_adj_combine<R>(ca, s);
     prof_stop(c, slot_p);
     PUFF_HIP(hipGetLastError());
-    PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int), hipMemcpyDeviceToHost, s));
+    PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int) * FLAG_WORDS, hipMemcpyDeviceToHost, s));
     PUFF_HIP(hipMemcpyAsync(c->h_dn2, c->dn2, sizeof(double) * 4 * DN2_COPIES, hipMemcpyDeviceToHost, s));
     PUFF_HIP(hipStreamSynchronize(s));
     prof_collect(c);
@@ -1090,7 +1103,7 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
             dn2col[k] = t;
         }
     }
-    const int f = *c->h_flags;
+    const int f = c->h_flags[0];
     if (f) {
         set_error("a trial state of the replayed step left the mesh");
         if (f & FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
@@ -1194,7 +1207,7 @@ static int adjoint_download(Ctx* c, int kind, double* x, double* y, double* z2, 
 struct BlobHeader {
     uint64_t magic;
     int64_t n, nmesh;
-    int32_t precision, sorted, identity_order, pad;
+    int32_t precision, sorted, identity_order, sort_due;
     int64_t order_id, steps_since_sort;
     double n2col, n2col0, undgrad;
 };
@@ -1238,6 +1251,7 @@ static int state_save(Ctx* c, void* dst) {
     hd->precision = c->precision;
     hd->sorted = c->sorted;
     hd->identity_order = c->identity_order;
+    hd->sort_due = c->sort_due;
     hd->order_id = c->order_id;
     hd->steps_since_sort = c->steps_since_sort;
     hd->n2col = c->n2col;
@@ -1288,6 +1302,7 @@ static int state_load(Ctx* c, const void* src) {
     PUFF_HIP(hipStreamSynchronize(s));
     c->sorted = hd.sorted != 0;
     c->identity_order = hd.identity_order != 0;
+    c->sort_due = hd.sort_due != 0;
     c->order_id = hd.order_id;
     c->steps_since_sort = hd.steps_since_sort;
     c->n2col = hd.n2col;

@@ -77,6 +77,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     const int lane = threadIdx.x & 63;
     const TileBox tb = workgroup_box<R>(m, ok, c, a.box);
     const bool use_tile = ph.field_evolve && tile_fits(m, tb);
+    if (threadIdx.x == 0 && ph.field_evolve && tb.any && !use_tile) atomicAdd(a.flags + FLAG_WORD_FALLBACK, 1);
 
     // ---- nodes and tri-linear weights (zero weights on lanes without a cell)
     const int nnb = m.one_d ? 2 : 8;
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
 
 // Start of a step: clear flags and the active box.
 __global__ void k_step_begin(int* flags, Box* box) {
-    if (threadIdx.x == 0) *flags = 0;
+    if (threadIdx.x < FLAG_WORDS) flags[threadIdx.x] = 0;
     for (int k = threadIdx.x; k < BOX_COPIES; k += blockDim.x) {
         for (int d = 0; d < 3; ++d) {
             box[k].lo[d] = 0x7fffffff;

@@ -7,8 +7,8 @@
 
 namespace puff {
 
-// Linear cell index (node index of the cell's first corner) of each particle;
-// particles outside the mesh go to the last bin.
+// Bin of each particle's cell in the slot order (planes, then rows, then
+// cells along x); particles outside the mesh go to the last bin.
 template <typename R>
 __global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const double* z2,
                                                    int64_t n, MeshGeom m, int ncell, int* key) {
@@ -32,8 +32,13 @@ __global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const
         } else {
             const R xq = floor((x[i] + (R)m.halfx) * (R)m.inv_dx);
             const R yq = floor((y[i] + (R)m.halfy) * (R)m.inv_dy);
-            if (xq >= (R)0 && xq <= (R)(m.nx - 2) && yq >= (R)0 && yq <= (R)(m.ny - 2))
-                k = iz * m.nx * m.ny + (int)yq * m.nx + (int)xq;
+            if (xq >= (R)0 && xq <= (R)(m.nx - 2) && yq >= (R)0 && yq <= (R)(m.ny - 2)) {
+                // rows run forward on even planes and backward on odd ones, so
+                // that a run of slots crossing into the next plane stays within
+                // a few rows (a compact deposit tile)
+                const int iy = (iz & 1) ? (m.ny - 2 - (int)yq) : (int)yq;
+                k = iz * m.nx * m.ny + iy * m.nx + (int)xq;
+            }
         }
     }
     key[i] = k;

@@ -83,6 +83,14 @@ def lib() -> ctypes.CDLL:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built: run `make -C puffin_amd` "
                                "or __graft_entry__.build()")
+        # torch bundles its own HIP runtime under the same soname
+        # (libamdhip64.so.7); whichever loads first serves the whole process.
+        # Load torch's first when torch is present, so that the library and
+        # torch's device memory and streams share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(l, name)
