@@ -216,11 +216,19 @@ int puff_active_box(puff_ctx* ctx, int32_t* box6);
  * blob holds the particles in the device's slot order, the field and the
  * taper and sort state; it lives in caller-owned device memory of
  * puff_state_bytes bytes (e.g. a torch uint8 CUDA tensor).  Used to take
- * checkpoints for the adjoint's recomputation.
+ * checkpoints for the adjoint's recomputation.  Save and load are ordered on
+ * the context's stream and return before the copies finish (puff_synchronize
+ * waits for them); load reads the blob's header synchronously.
  */
 int puff_state_bytes(puff_ctx* ctx, int64_t* bytes);
 int puff_state_save(puff_ctx* ctx, void* device_dst);
 int puff_state_load(puff_ctx* ctx, const void* device_src);
+/* As above, also writing (save) or taking (load) a host copy of the blob's
+   header, PUFF_STATE_HEADER_BYTES bytes: a load given the header of the same
+   save does not wait for the device.  host_header may be null. */
+#define PUFF_STATE_HEADER_BYTES 256
+int puff_state_save_ex(puff_ctx* ctx, void* device_dst, void* host_header);
+int puff_state_load_ex(puff_ctx* ctx, const void* device_src, const void* host_header);
 /* Sort the macroparticles by mesh cell now (otherwise done every
    puff_set_sort_interval steps inside puff_rk4par).  The order is internal:
    downloads always return the caller's order. */
@@ -250,11 +258,41 @@ int puff_sort_now(puff_ctx* ctx);
  * puff_adjoint_upload / _download exchange the cotangent in the caller's
  * particle order and split field layout (either field pointer may be null).
  */
+/*
+ * Deferred status (no reference counterpart).  With puff_set_deferred(ctx,
+ * 1), puff_rk4par and puff_rk4par_adjoint return PUFF_OK without waiting for
+ * the step: a flagged step still leaves the state unchanged, its flags are
+ * kept, and puff_deferred_status waits for the stream and returns the status
+ * of all steps since its last call (as puff_rk4par).  In this mode the
+ * adjoint step accumulates d(output)/d(n2col) per taper-table slot on the
+ * device (slot -1, no table, into entry ntable) instead of returning dn2col;
+ * puff_adjoint_taper_gradient copies the first n entries of that table
+ * (zeroed by puff_adjoint_zero).  The re-sort between steps then follows only
+ * puff_set_sort_interval.
+ */
+int puff_set_deferred(puff_ctx* ctx, int32_t on);
+int puff_deferred_status(puff_ctx* ctx);
+int puff_adjoint_taper_gradient(puff_ctx* ctx, double* out, int32_t n);
+
 int puff_adjoint_enable(puff_ctx* ctx);
 int puff_adjoint_zero(puff_ctx* ctx);
 int puff_adjoint_seed_power(puff_ctx* ctx, const double* wz_host, double scale);
 int puff_rk4par_adjoint(puff_ctx* ctx, double sz, double h, double* dn2col, int32_t* slot);
 int puff_diffraction_adjoint(puff_ctx* ctx, double h);
+
+/*
+ * Step capture: puff_rk4par_capture is puff_rk4par that also keeps, in
+ * caller-owned device memory of puff_capture_bytes bytes, the step's start
+ * state (after the step's own re-sort) and its stage derivatives and sources,
+ * with a host copy of the capture's header (PUFF_STATE_HEADER_BYTES).
+ * puff_rk4par_adjoint_captured is puff_rk4par_adjoint for that step from the
+ * capture: it neither reads nor changes the context's state and does not
+ * recompute the forward stages.
+ */
+int puff_capture_bytes(puff_ctx* ctx, int64_t* bytes);
+int puff_rk4par_capture(puff_ctx* ctx, double sz, double h, void* capture, void* host_header);
+int puff_rk4par_adjoint_captured(puff_ctx* ctx, double sz, double h, const void* capture,
+                                 const void* host_header, double* dn2col, int32_t* slot);
 int puff_adjoint_upload(puff_ctx* ctx, int32_t mem_kind, const double* x, const double* y,
                         const double* z2, const double* px, const double* py,
                         const double* gam, const double* ar, const double* ai);

@@ -18,6 +18,9 @@ recompute: the forward keeps a device snapshot of the state every ``every``
 RK4 steps; the backward replays each segment from its snapshot, keeping one
 snapshot per step of the segment, and walks the segment's operations in
 reverse.  Snapshots are caller-owned device memory (torch uint8 tensors).
+When memory allows, the segment's replay keeps each step's stage data
+(puff_rk4par_capture), so that the adjoint steps do not recompute forward
+stages.
 """
 from __future__ import annotations
 
@@ -26,7 +29,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .gpu import Integrator
+from .gpu import STATE_HEADER_BYTES, Integrator
 from .params import PI_REF, ModuleParams
 from .undulator import und_section
 
@@ -108,7 +111,8 @@ class ModuleGradient:
     ``taper_period`` / ``taper_z0`` place the per-period table on zbar."""
 
     def __init__(self, drv: Integrator, ops: list[Op], field_shape: tuple, *, every: int = 30,
-                 taper_period: float | None = None, taper_z0: float = 0.0):
+                 taper_period: float | None = None, taper_z0: float = 0.0,
+                 capture: bool | None = None, capture_budget: int = 32 << 30):
         if torch is None:
             raise RuntimeError("the gradient engine keeps its snapshots in torch device memory")
         self.drv = drv
@@ -125,23 +129,35 @@ class ModuleGradient:
         dev = torch.device("cuda", torch.cuda.current_device())
         self._blob = lambda: torch.empty(nb, dtype=torch.uint8, device=dev)
         # initial particles (taken from the driver as it is now)
+        hdr = lambda: np.zeros(STATE_HEADER_BYTES, dtype=np.uint8)
         self.initial = self._blob()
-        drv.state_save(self.initial.data_ptr())
+        self.initial_hdr = hdr()
+        drv.state_save(self.initial.data_ptr(), self.initial_hdr)
         self.ckpts = [self._blob() for _ in self.segs]
-        self.steps = [self._blob() for _ in range(self.every)]
+        self.ckpt_hdr = [hdr() for _ in self.segs]
+        # per-step data of the segment being reversed: step captures (start
+        # state and stage data, no forward stage recomputed by the adjoint
+        # step) when they fit the budget, else start-state snapshots
+        nc = drv.capture_bytes()
+        self.capture = (self.every * nc <= capture_budget) if capture is None else bool(capture)
+        if self.capture:
+            self.steps = [torch.empty(nc, dtype=torch.uint8, device=dev) for _ in range(self.every)]
+        else:
+            self.steps = [self._blob() for _ in range(self.every)]
+        self.step_hdr = [hdr() for _ in range(self.every)]
         self.table = None
         self.seed = None
         self.J = None
         self.rk4_calls = 0
 
     def memory_bytes(self) -> int:
-        return (1 + len(self.ckpts) + len(self.steps)) * self.initial.numel()
+        return (1 + len(self.ckpts)) * self.initial.numel() + sum(t.numel() for t in self.steps)
 
     # ---- forward
     def _set_inputs(self, table, seed: float):
         self.table = np.ascontiguousarray(table, dtype=np.float64)
         self.seed = float(seed)
-        self.drv.state_load(self.initial.data_ptr())
+        self.drv.state_load(self.initial.data_ptr(), self.initial_hdr)
         self.drv.upload_field(self.seed * self.shape_r, self.seed * self.shape_i)
         self.drv.set_taper_table(self.table, self.taper_z0, self.taper_period)
 
@@ -156,20 +172,39 @@ class ModuleGradient:
         else:
             self.drv.wrap_z2()
 
+    def _deferred(self, on: bool):
+        """Steps without a host wait each (the status is checked per segment)."""
+        self.drv.set_deferred(on)
+
+    def _check_status(self, where: str):
+        st = self.drv.deferred_status()
+        if st != 0:
+            raise RuntimeError(f"{where}: a step raised status {st}")
+
     def forward(self, table, seed: float) -> float:
         self._set_inputs(table, seed)
-        for k, (a, b) in enumerate(self.segs):
-            self.drv.state_save(self.ckpts[k].data_ptr())
-            for op in self.ops[a:b]:
-                self._run(op)
+        self._deferred(True)
+        try:
+            for k, (a, b) in enumerate(self.segs):
+                self.drv.state_save(self.ckpts[k].data_ptr(), self.ckpt_hdr[k])
+                for op in self.ops[a:b]:
+                    self._run(op)
+            self._check_status("forward")
+        finally:
+            self._deferred(False)
         self.J = integrated_power(self.mp, self.drv.power())
         return self.J
 
     def evaluate(self, table, seed: float) -> float:
         """J without keeping checkpoints (finite differences)."""
         self._set_inputs(table, seed)
-        for op in self.ops:
-            self._run(op)
+        self._deferred(True)
+        try:
+            for op in self.ops:
+                self._run(op)
+            self._check_status("forward")
+        finally:
+            self._deferred(False)
         return integrated_power(self.mp, self.drv.power())
 
     # ---- backward
@@ -180,30 +215,38 @@ class ModuleGradient:
         drv.adjoint_enable()
         drv.adjoint_zero()
         drv.adjoint_seed_power(self.wz, scale)
-        g_table = np.zeros(self.table.size)
-        for k in range(len(self.segs) - 1, -1, -1):
-            a, b = self.segs[k]
-            drv.state_load(self.ckpts[k].data_ptr())
-            slots = {}
-            for i in range(a, b):
-                op = self.ops[i]
-                if op.kind == "rk4":
-                    j = len(slots)
-                    drv.state_save(self.steps[j].data_ptr())
-                    slots[i] = j
-                self._run(op)
-            for i in range(b - 1, a - 1, -1):
-                op = self.ops[i]
-                if op.kind == "rk4":
-                    drv.state_load(self.steps[slots[i]].data_ptr())
-                    st, dn, sl = drv.rk4par_adjoint(op.sz, op.h)
-                    if st != 0:
-                        raise RuntimeError(f"adjoint replay status {st} at zbar {op.sz}")
-                    for s in range(4):
-                        if sl[s] >= 0:
-                            g_table[sl[s]] += dn[s]
-                elif op.kind == "diff":
-                    drv.diffraction_adjoint(op.h)
+        self._deferred(True)
+        try:
+            for k in range(len(self.segs) - 1, -1, -1):
+                a, b = self.segs[k]
+                drv.state_load(self.ckpts[k].data_ptr(), self.ckpt_hdr[k])
+                slots = {}
+                for i in range(a, b):
+                    op = self.ops[i]
+                    if op.kind == "rk4":
+                        j = len(slots)
+                        slots[i] = j
+                        if self.capture:
+                            drv.rk4par_capture(op.sz, op.h, self.steps[j].data_ptr(), self.step_hdr[j])
+                            continue
+                        drv.state_save(self.steps[j].data_ptr(), self.step_hdr[j])
+                    self._run(op)
+                for i in range(b - 1, a - 1, -1):
+                    op = self.ops[i]
+                    if op.kind == "rk4":
+                        j = slots[i]
+                        if self.capture:
+                            drv.rk4par_adjoint_captured(op.sz, op.h, self.steps[j].data_ptr(), self.step_hdr[j])
+                        else:
+                            drv.state_load(self.steps[j].data_ptr(), self.step_hdr[j])
+                            drv.rk4par_adjoint(op.sz, op.h)
+                    elif op.kind == "diff":
+                        drv.diffraction_adjoint(op.h)
+                self._check_status("adjoint replay")
+            # per-slot dJ/dn2col accumulated on the device by the adjoint steps
+            g_table = drv.adjoint_taper_gradient(self.table.size + 1)[: self.table.size]
+        finally:
+            self._deferred(False)
         _, lar, lai = drv.adjoint_download()
         g_seed = float(np.dot(lar, self.shape_r) + np.dot(lai, self.shape_i))
         return {"table": g_table, "seed": g_seed, "field": (lar, lai)}

@@ -29,7 +29,7 @@
 namespace puff {
 
 template <typename R, int STAGE>
-__global__ __launch_bounds__(STAGE_THREADS) void k_adj_stage(AdjArgs<R> a) {
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     __shared__ int s_skip;
     if (threadIdx.x == 0) s_skip = *a.flags;
     __syncthreads();
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_adj_stage(AdjArgs<R> a) {
     const int wv = threadIdx.x >> 6;
     const TileBox tb = workgroup_box<R>(m, ok, c, a.box);
     const bool scat = ph.coupling && ph.electrons_evolve;
-    const bool use_tile = scat && tile_fits(m, tb);
+    const bool use_tile = scat && tile_fits<ADJ_TILE_NODES>(m, tb);
     const int nnb = m.one_d ? 2 : 8;
     R w[8];
     int nd[8];
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_adj_stage(AdjArgs<R> a) {
 
     // ---- field cotangent of this stage: w_k (dL/dFr, dL/dFi) at the nodes
     if (scat) {
-        scatter_cells<R>(m, tb, use_tile, ok, c, nnb, w, nd, t0, key, ok ? (R)Lfr : (R)0,
+        scatter_cells<R, ADJ_TILE_NODES>(m, tb, use_tile, ok, c, nnb, w, nd, t0, key, ok ? (R)Lfr : (R)0,
                          ok ? (R)Lfi : (R)0, a.mAout);
     }
 
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_adj_stage(AdjArgs<R> a) {
             dn = apr * (ph.inv2rho * by + ph.kappa * pid * fac * bz) +
                  api * (ph.inv2rho * bx - ph.kappa * prd * fac * bz);
         dn = wave_sum(dn);
-        __shared__ double s_dn[STAGE_THREADS / 64];
+        __shared__ double s_dn[ADJ_THREADS / 64];
         if (lane == 0) s_dn[wv] = dn;
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -246,6 +246,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_adj_stage(AdjArgs<R> a) {
 // last plane of a periodic mesh is never written (its cotangent stays 0).
 template <typename R>
 __global__ __launch_bounds__(256) void k_adj_combine(AdjCombineArgs<R> a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.flags[0]) atomicOr(a.flags + FLAG_WORD_STICKY, a.flags[0]);
     const Box b = merged_box(a.box);
     if (b.hi[0] < 0) return;
     const int x0 = b.lo[0], x1 = min(b.hi[0] + 1, a.nx - 1);
@@ -278,9 +279,11 @@ __global__ __launch_bounds__(256) void k_adj_combine(AdjCombineArgs<R> a) {
         a.mA[1][j] = zero;
         a.mA[2][j] = zero;
         a.mA[3][j] = zero;
-        a.dA[0][j] = zero;
-        a.dA[1][j] = zero;
-        a.dA[2][j] = zero;
+        if (a.clear_dA) {
+            a.dA[0][j] = zero;
+            a.dA[1][j] = zero;
+            a.dA[2][j] = zero;
+        }
     }
 }
 
@@ -322,6 +325,23 @@ __global__ __launch_bounds__(256) void k_adj_seed_power(cplx<R>* LA, const cplx<
     }
 }
 
+// Per-stage dJ/dn2col into the per-slot gradient table (one wavefront).
+__global__ void k_dn2_accumulate(const double* dn2, Slots4 slots, int ntab, double* table) {
+    const int lane = threadIdx.x;
+    for (int st = 0; st < 4; ++st) {
+        double v = (lane < DN2_COPIES) ? dn2[st * DN2_COPIES + lane] : 0.0;
+        v = wave_sum(v);
+        if (lane == st) {
+            const int k = slots.s[st] < 0 ? ntab : slots.s[st];
+            table[k] += v;
+        }
+    }
+}
+
+void launch_dn2_accumulate(const double* dn2, Slots4 slots, int ntab, double* dn2_table, hipStream_t st) {
+    hipLaunchKernelGGL(k_dn2_accumulate, dim3(1), dim3(64), 0, st, dn2, slots, ntab, dn2_table);
+}
+
 // Periodic mesh: fold the aliased last plane's cotangent into plane 1.
 template <typename R>
 __global__ __launch_bounds__(256) void k_fold_alias(cplx<R>* LA, int64_t ntr, int nz2) {
@@ -359,7 +379,7 @@ void launch_fill_zero_words(uint32_t* p, int64_t n, hipStream_t s) {
 
 template <typename R>
 void launch_adj_stage(int stage, const AdjArgs<R>& a, hipStream_t s) {
-    const int threads = STAGE_THREADS;
+    const int threads = ADJ_THREADS;
     const int64_t blocks = (a.n + threads - 1) / threads;
     dim3 grid((unsigned)(blocks > 0 ? blocks : 1));
     switch (stage) {

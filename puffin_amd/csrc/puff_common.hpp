@@ -63,11 +63,19 @@ constexpr int STAGE_THREADS = 1024;
 #endif
 constexpr int TILE_NODES = PUFF_TILE_NODES;
 
-// flags[FLAG_WORD_FALLBACK] counts the stage workgroups of a step whose node
-// box did not fit the LDS tile (global-atomic deposit): the host re-sorts the
-// particles when that share grows.
-constexpr int FLAG_WORDS = 2;
+// Workgroup and LDS tile of the adjoint stage kernels: more live values per
+// particle than the forward, so smaller workgroups with a larger VGPR budget.
+constexpr int ADJ_THREADS = 256;
+constexpr int ADJ_TILE_NODES = 1024;
+
+// Flag words of the step kernels: [0] flag bits of the current step;
+// [FLAG_WORD_FALLBACK] the stage workgroups of the step whose node box did not
+// fit the LDS tile (global-atomic deposit): the host re-sorts the particles
+// when that share grows; [FLAG_WORD_STICKY] the OR of the step flags since
+// the host last read them (deferred status).
+constexpr int FLAG_WORDS = 3;
 constexpr int FLAG_WORD_FALLBACK = 1;
+constexpr int FLAG_WORD_STICKY = 2;
 
 // Wavefront (64-lane) reductions.
 __device__ __forceinline__ int wave_min(int v) {

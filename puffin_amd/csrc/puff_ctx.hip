@@ -35,6 +35,8 @@ struct Particles {
     R* chi = nullptr;
 };
 
+constexpr int DN2_TABLE_CAP = 4096;   // taper-table slots of the deferred adjoint
+
 template <typename R>
 struct AdjBufs {
     PartPtrs<R> L;
@@ -119,6 +121,9 @@ struct Ctx {
     void* h_hdr = nullptr;     // pinned state-blob header
     int* lam_orig = nullptr;   // slot order of the particle cotangent
     int64_t lam_order_id = 0;
+    double* dn2_table = nullptr;   // per-slot dJ/dn2col (deferred adjoint steps)
+    // deferred status: steps return without waiting for their flags
+    bool deferred = false;
 
     int64_t nmesh() const { return (int64_t)p.nx * p.ny * p.nz2; }
     int nzp() const { return p.periodic ? p.nz2 - 1 : p.nz2; }
@@ -576,6 +581,50 @@ static void release(Ctx* c) {
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
 
+// ---- step capture ---------------------------------------------------------------
+// A step's start state and stage data kept in caller-owned device memory for
+// the adjoint: 256-byte header, x y z2 pr pi Gamma, chi, orig, A, then k1, k2,
+// k3 (six arrays each) and dA1, dA2, dA3, each section 256-byte aligned.
+
+template <typename R>
+struct CaptureView {
+    PartPtrs<R> y;
+    R* chi;
+    int* orig;
+    cplx<R>* A0;
+    PartPtrs<R> k[3];
+    cplx<R>* dA[3];
+};
+
+static size_t align256c(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Pointers into the capture at base (null base: sizes only); returns its bytes.
+template <typename R>
+static size_t capture_view(const Ctx* c, char* base, CaptureView<R>* v) {
+    size_t o = 256;
+    const size_t n = (size_t)c->n, nm = (size_t)c->nmesh();
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + o : nullptr;
+        o = align256c(o + bytes);
+        return p;
+    };
+    auto take_parts = [&](PartPtrs<R>& q) {
+        q.x = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.y = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.z2 = reinterpret_cast<double*>(take(sizeof(double) * n));
+        q.pr = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.pi = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.g = reinterpret_cast<double*>(take(sizeof(double) * n));
+    };
+    take_parts(v->y);
+    v->chi = reinterpret_cast<R*>(take(sizeof(R) * n));
+    v->orig = reinterpret_cast<int*>(take(sizeof(int) * n));
+    v->A0 = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * nm));
+    for (int k = 0; k < 3; ++k) take_parts(v->k[k]);
+    for (int k = 0; k < 3; ++k) v->dA[k] = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * nm));
+    return o;
+}
+
 // ---- the integrator step ----------------------------------------------------
 
 template <typename R>
@@ -605,8 +654,12 @@ static void sort_particles(Ctx* c) {
     c->order_id = ++c->order_counter;
 }
 
+struct BlobHeader;
 template <typename R>
-static int rk4par(Ctx* c, double sz, double h) {
+static void capture_start(Ctx* c, char* cap, void* cap_hdr, CaptureView<R>& v, const double* taper0);
+
+template <typename R>
+static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hdr = nullptr) {
     Particles<R>& P = parts<R>(c);
     hipStream_t s = c->stream;
     const double hh = h * 0.5, h6 = h / 6.0;   // puffin_mpi_RK4.f90:89-91
@@ -626,6 +679,8 @@ static int rk4par(Ctx* c, double sz, double h) {
         sort_particles<R>(c);
     }
     c->steps_since_sort++;
+    CaptureView<R> v{};
+    if (cap) capture_start<R>(c, cap, cap_hdr, v, saved);
     int slot = prof_start(c, 6);
     launch_step_begin(c->flags, c->box, s);
     prof_stop(c, slot);
@@ -641,25 +696,36 @@ static int rk4par(Ctx* c, double sz, double h) {
     a.box = c->box;
     const double cin[4] = {0.0, hh, hh, h};
     const double cnext[4] = {hh, hh, h, 0.0};
+    // stage derivatives and sources: the context's buffers, or the capture's
+    PartPtrs<R> K[3] = {P.k[0], P.k[1], P.k[2]};
+    cplx<R>* D[4];
+    for (int k = 0; k < 4; ++k) D[k] = reinterpret_cast<cplx<R>*>(c->dA[k]);
+    if (cap) {
+        for (int k = 0; k < 3; ++k) {
+            K[k] = v.k[k];
+            D[k] = v.dA[k];
+        }
+    }
     for (int st = 1; st <= 4; ++st) {
         a.s = sc[st - 1];
         a.cin = cin[st - 1];
         a.cnext = cnext[st - 1];
-        a.kin = (st > 1) ? P.k[st - 2] : PartPtrs<R>{};
-        a.kout = (st < 4) ? P.k[st - 1] : PartPtrs<R>{};
+        a.kin = (st > 1) ? K[st - 2] : PartPtrs<R>{};
+        a.kout = (st < 4) ? K[st - 1] : PartPtrs<R>{};
         if (st == 4) {
-            a.k1 = P.k[0];
-            a.k2 = P.k[1];
+            a.k1 = K[0];
+            a.k2 = K[1];
         }
-        a.dAin = (st > 1) ? reinterpret_cast<const cplx<R>*>(c->dA[st - 2]) : nullptr;
-        a.dAout = reinterpret_cast<cplx<R>*>(c->dA[st - 1]);
+        a.dAin = (st > 1) ? D[st - 2] : nullptr;
+        a.dAout = D[st - 1];
         slot = prof_start(c, st - 1);
         launch_stage<R>(st, a, s);
         prof_stop(c, slot);
     }
     CombineArgs<R> ca{};
     ca.A = reinterpret_cast<cplx<R>*>(c->A);
-    for (int k = 0; k < 4; ++k) ca.d[k] = reinterpret_cast<cplx<R>*>(c->dA[k]);
+    for (int k = 0; k < 4; ++k) ca.d[k] = D[k];
+    ca.clear_from = cap ? 3 : 0;   // the capture keeps dA1..dA3
     ca.h6 = h6;
     ca.box = c->box;
     ca.flags = c->flags;
@@ -673,6 +739,7 @@ static int rk4par(Ctx* c, double sz, double h) {
     launch_combine<R>(ca, s);
     prof_stop(c, slot);
     PUFF_HIP(hipGetLastError());
+    if (c->deferred) return PUFF_OK;   // flags collected by puff_deferred_status
     PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int) * FLAG_WORDS, hipMemcpyDeviceToHost, s));
     PUFF_HIP(hipStreamSynchronize(s));
     prof_collect(c);
@@ -910,6 +977,7 @@ static int adjoint_enable(Ctx* c) {
     PUFF_TRY(dalloc(c, (void**)&c->dn2, sizeof(double) * 4 * DN2_COPIES));
     PUFF_TRY(dalloc(c, (void**)&c->adj_wz, sizeof(double) * c->p.nz2));
     PUFF_TRY(dalloc(c, (void**)&c->lam_orig, sizeof(int) * c->n));
+    PUFF_TRY(dalloc(c, (void**)&c->dn2_table, sizeof(double) * (DN2_TABLE_CAP + 1)));
     PUFF_HIP(hipHostMalloc((void**)&c->h_dn2, sizeof(double) * 4 * DN2_COPIES, hipHostMallocDefault));
     c->adj = true;
     return adjoint_zero<R>(c);
@@ -965,6 +1033,7 @@ static int adjoint_zero(Ctx* c) {
     fill_zero_typed<double>(B.L.z2, n, c->stream);
     fill_zero_typed<double>(B.L.g, n, c->stream);
     fill_zero_typed<cplx<R>>(reinterpret_cast<cplx<R>*>(c->LA), nm, c->stream);
+    fill_zero_typed<double>(c->dn2_table, DN2_TABLE_CAP + 1, c->stream);
     PUFF_HIP(hipGetLastError());
     PUFF_HIP(hipMemcpyAsync(c->lam_orig, c->orig, sizeof(int) * n, hipMemcpyDeviceToDevice, c->stream));
     c->lam_order_id = c->order_id;
@@ -998,31 +1067,13 @@ static int fold_alias(Ctx* c) {
 // the step (left unchanged), L / LA the cotangent of the end state (replaced
 // by the cotangent of the start state).  dn2col[k] = dJ/d(n2col of stage k),
 // slot[k] its per-period table slot (-1 without a table).
+// Forward stages 1-3 of the step starting at the current state again: k_1..k_3
+// and dA_1..dA_3 in the context's buffers (the adjoint step without a capture).
 template <typename R>
-static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* slot) {
-    PUFF_TRY(need_adj(c));
+static int replay_stages(Ctx* c, const StageScalars* sc, const double* cin, const double* cnext, double h6) {
     Particles<R>& P = parts<R>(c);
-    AdjBufs<R>& B = adjb<R>(c);
     hipStream_t s = c->stream;
-    const double hh = h * 0.5, h6 = h / 6.0;
-    const double zs[4] = {sz, sz + hh, sz + hh, sz + h};
-    const double saved[3] = {c->n2col, c->n2col0, c->undgrad};
-    StageScalars sc[4];
-    int st_err = PUFF_OK;
-    for (int k = 0; k < 4 && st_err == PUFF_OK; ++k) {
-        const double n2 = advance_taper(c, zs[k]);
-        st_err = stage_scalars(c->p, zs[k], n2, sc[k]);
-        if (slot) slot[k] = taper_slot(c, zs[k]);
-    }
-    c->n2col = saved[0];
-    c->n2col0 = saved[1];
-    if (st_err != PUFF_OK) return st_err;
-    PUFF_TRY(sync_cotangent_order<R>(c));
-    PUFF_TRY(fold_alias<R>(c));
-    launch_fill_zero(c->dn2, 4 * DN2_COPIES, s);
-    int slot_p = prof_start(c, 10);
-    launch_step_begin(c->flags, c->box, s);
-    // forward stages 1-3 again: k_1..k_3 and dA_1..dA_3 of this step
+    const int slot_p = prof_start(c, 10);
     StageArgs<R> a{};
     a.n = c->n;
     a.y0 = P.y;
@@ -1033,8 +1084,6 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
     a.phys = c->phys;
     a.flags = c->flags;
     a.box = c->box;
-    const double cin[4] = {0.0, hh, hh, h};
-    const double cnext[4] = {hh, hh, h, 0.0};
     for (int st = 1; st <= 3; ++st) {
         a.s = sc[st - 1];
         a.cin = cin[st - 1];
@@ -1046,13 +1095,92 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
         launch_stage<R>(st, a, s);
     }
     prof_stop(c, slot_p);
+    PUFF_HIP(hipGetLastError());
+    return PUFF_OK;
+}
+
+template <typename R>
+static int capture_check(Ctx* c, const void* cap_hdr, double taper[3], int64_t* order_id);
+
+template <typename R>
+static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* slot,
+                          const char* cap = nullptr, const void* cap_hdr = nullptr) {
+    PUFF_TRY(need_adj(c));
+    Particles<R>& P = parts<R>(c);
+    AdjBufs<R>& B = adjb<R>(c);
+    hipStream_t s = c->stream;
+    const double hh = h * 0.5, h6 = h / 6.0;
+    const double zs[4] = {sz, sz + hh, sz + hh, sz + h};
+    const double saved[3] = {c->n2col, c->n2col0, c->undgrad};
+    CaptureView<R> v{};
+    int64_t cap_order = 0;
+    if (cap) {
+        // the captured step's taper state replaces the context's for the
+        // stage coefficients (restored below)
+        double tp[3];
+        PUFF_TRY(capture_check<R>(c, cap_hdr, tp, &cap_order));
+        capture_view<R>(c, const_cast<char*>(cap), &v);
+        c->n2col = tp[0];
+        c->n2col0 = tp[1];
+        c->undgrad = tp[2];
+    }
+    StageScalars sc[4];
+    int st_err = PUFF_OK;
+    for (int k = 0; k < 4 && st_err == PUFF_OK; ++k) {
+        const double n2 = advance_taper(c, zs[k]);
+        st_err = stage_scalars(c->p, zs[k], n2, sc[k]);
+        if (slot) slot[k] = taper_slot(c, zs[k]);
+    }
+    c->n2col = saved[0];
+    c->n2col0 = saved[1];
+    c->undgrad = saved[2];
+    if (st_err != PUFF_OK) return st_err;
+    if (c->deferred && c->taper_table.size() > (size_t)DN2_TABLE_CAP) {
+        set_error("taper table longer than the deferred adjoint's gradient table");
+        return PUFF_ERR_ARG;
+    }
+    if (cap) {
+        // the cotangent follows the captured step's slot order
+        if (c->lam_order_id != cap_order) {
+            PUFF_TRY(repermute_cotangent<R>(c, c->lam_orig, v.orig));
+            PUFF_HIP(hipMemcpyAsync(c->lam_orig, v.orig, sizeof(int) * c->n, hipMemcpyDeviceToDevice, s));
+            c->lam_order_id = cap_order;
+        }
+    } else {
+        PUFF_TRY(sync_cotangent_order<R>(c));
+    }
+    PUFF_TRY(fold_alias<R>(c));
+    launch_fill_zero(c->dn2, 4 * DN2_COPIES, s);
+    launch_step_begin(c->flags, c->box, s);
+    const double cin[4] = {0.0, hh, hh, h};
+    const double cnext[4] = {hh, hh, h, 0.0};
+    // start state and stage data: the capture's, or the context's state with
+    // forward stages 1-3 replayed (k_1..k_3, dA_1..dA_3)
+    PartPtrs<R> Y0 = P.y;
+    const R* CHI = P.chi;
+    const cplx<R>* A0 = reinterpret_cast<const cplx<R>*>(c->A);
+    PartPtrs<R> K[3] = {P.k[0], P.k[1], P.k[2]};
+    const cplx<R>* D[3];
+    for (int k = 0; k < 3; ++k) D[k] = reinterpret_cast<const cplx<R>*>(c->dA[k]);
+    if (cap) {
+        Y0 = v.y;
+        CHI = v.chi;
+        A0 = v.A0;
+        for (int k = 0; k < 3; ++k) {
+            K[k] = v.k[k];
+            D[k] = v.dA[k];
+        }
+    } else {
+        PUFF_TRY(replay_stages<R>(c, sc, cin, cnext, h6));
+    }
+    int slot_p;
     // adjoint stages 4 .. 1
     const double wts[4] = {1.0, 2.0, 2.0, 1.0};
     AdjArgs<R> b{};
     b.n = c->n;
-    b.y0 = P.y;
-    b.chi = P.chi;
-    b.A0 = reinterpret_cast<const cplx<R>*>(c->A);
+    b.y0 = Y0;
+    b.chi = CHI;
+    b.A0 = A0;
     b.L = B.L;
     b.LA = reinterpret_cast<const cplx<R>*>(c->LA);
     b.mesh = c->mesh;
@@ -1063,8 +1191,8 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
     for (int st = 4; st >= 1; --st) {
         b.s = sc[st - 1];
         b.cin = cin[st - 1];
-        b.kin = (st > 1) ? P.k[st - 2] : PartPtrs<R>{};
-        b.dAin = (st > 1) ? reinterpret_cast<const cplx<R>*>(c->dA[st - 2]) : nullptr;
+        b.kin = (st > 1) ? K[st - 2] : PartPtrs<R>{};
+        b.dAin = (st > 1) ? D[st - 2] : nullptr;
         b.ws = wts[st - 1] * h6;
         b.cnext = (st < 4) ? cin[st] : 0.0;
         b.mnext = (st < 4) ? B.M[st - 1] : PartPtrs<R>{};
@@ -1082,16 +1210,26 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
     for (int k = 0; k < 4; ++k) ca.mA[k] = reinterpret_cast<cplx<R>*>(c->mA[k]);
     for (int k = 0; k < 3; ++k) ca.dA[k] = reinterpret_cast<cplx<R>*>(c->dA[k]);
     ca.box = c->box;
+    ca.flags = c->flags;
     ca.nx = c->p.nx;
     ca.ny = c->p.ny;
     ca.nz2 = c->p.nz2;
     ca.nzp = c->nzp();
     ca.periodic = c->p.periodic;
     ca.one_d = c->p.one_d;
+    ca.clear_dA = cap ? 0 : 1;
     slot_p = prof_start(c, 8);
     launch_adj_combine<R>(ca, s);
     prof_stop(c, slot_p);
     PUFF_HIP(hipGetLastError());
+    if (c->deferred) {
+        // dJ/dn2col stays on the device (puff_adjoint_taper_gradient)
+        Slots4 sl;
+        for (int k = 0; k < 4; ++k) sl.s[k] = taper_slot(c, zs[k]);
+        launch_dn2_accumulate(c->dn2, sl, (int)c->taper_table.size(), c->dn2_table, s);
+        PUFF_HIP(hipGetLastError());
+        return PUFF_OK;
+    }
     PUFF_HIP(hipMemcpyAsync(c->h_flags, c->flags, sizeof(int) * FLAG_WORDS, hipMemcpyDeviceToHost, s));
     PUFF_HIP(hipMemcpyAsync(c->h_dn2, c->dn2, sizeof(double) * 4 * DN2_COPIES, hipMemcpyDeviceToHost, s));
     PUFF_HIP(hipStreamSynchronize(s));
@@ -1212,7 +1350,7 @@ struct BlobHeader {
     double n2col, n2col0, undgrad;
 };
 constexpr uint64_t BLOB_MAGIC = 0x31424f4c42465550ull;   // "PUFBLOB1"
-static_assert(sizeof(BlobHeader) <= 256, "blob header");
+static_assert(sizeof(BlobHeader) <= PUFF_STATE_HEADER_BYTES && sizeof(BlobHeader) % 4 == 0, "blob header");
 
 static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
@@ -1237,14 +1375,32 @@ static int state_bytes(Ctx* c, int64_t* bytes) {
     return PUFF_OK;
 }
 
+// Sections of a blob and the context arrays they hold.
 template <typename R>
-static int state_save(Ctx* c, void* dst) {
+static CopySections blob_sections(Ctx* c, char* blob, bool to_blob) {
     Particles<R>& P = parts<R>(c);
     size_t off[9];
     blob_layout<R>(c, off);
-    if (!c->h_hdr) PUFF_HIP(hipHostMalloc(&c->h_hdr, 256, hipHostMallocDefault));
-    BlobHeader* hd = reinterpret_cast<BlobHeader*>(c->h_hdr);
-    *hd = BlobHeader{};
+    const size_t n = (size_t)c->n;
+    void* arr[9] = {P.y.x, P.y.y, P.y.z2, P.y.pr, P.y.pi, P.y.g, P.chi, c->orig, c->A};
+    const size_t bytes[9] = {sizeof(R) * n, sizeof(R) * n, sizeof(double) * n, sizeof(R) * n, sizeof(R) * n,
+                             sizeof(double) * n, sizeof(R) * n, sizeof(int) * n,
+                             sizeof(cplx<R>) * (size_t)c->nmesh()};
+    CopySections cs{};
+    cs.n = 9;
+    for (int k = 0; k < 9; ++k) {
+        uint8_t* b = reinterpret_cast<uint8_t*>(blob + off[k]);
+        cs.src[k] = to_blob ? reinterpret_cast<const uint8_t*>(arr[k]) : b;
+        cs.dst[k] = to_blob ? b : reinterpret_cast<uint8_t*>(arr[k]);
+        cs.bytes[k] = (int64_t)bytes[k];
+    }
+    return cs;
+}
+
+template <typename R>
+static int state_save(Ctx* c, void* dst, void* host_header) {
+    Words64 words{};
+    BlobHeader* hd = reinterpret_cast<BlobHeader*>(words.w);
     hd->magic = BLOB_MAGIC;
     hd->n = c->n;
     hd->nmesh = c->nmesh();
@@ -1258,48 +1414,36 @@ static int state_save(Ctx* c, void* dst) {
     hd->n2col0 = c->n2col0;
     hd->undgrad = c->undgrad;
     char* d = reinterpret_cast<char*>(dst);
-    const size_t n = (size_t)c->n;
     hipStream_t s = c->stream;
-    PUFF_HIP(hipMemcpyAsync(d, c->h_hdr, sizeof(BlobHeader), hipMemcpyHostToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[0], P.y.x, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[1], P.y.y, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[2], P.y.z2, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[3], P.y.pr, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[4], P.y.pi, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[5], P.y.g, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[6], P.chi, sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[7], c->orig, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(d + off[8], c->A, sizeof(cplx<R>) * (size_t)c->nmesh(), hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipStreamSynchronize(s));
+    launch_store_words(reinterpret_cast<uint32_t*>(d), words, (int)(sizeof(BlobHeader) / 4), s);
+    launch_copy_sections(blob_sections<R>(c, d, true), s);
+    PUFF_HIP(hipGetLastError());
+    if (host_header) {
+        std::memset(host_header, 0, PUFF_STATE_HEADER_BYTES);
+        std::memcpy(host_header, hd, sizeof(BlobHeader));
+    }
     return PUFF_OK;
 }
 
 template <typename R>
-static int state_load(Ctx* c, const void* src) {
-    Particles<R>& P = parts<R>(c);
-    size_t off[9];
-    blob_layout<R>(c, off);
-    if (!c->h_hdr) PUFF_HIP(hipHostMalloc(&c->h_hdr, 256, hipHostMallocDefault));
+static int state_load(Ctx* c, const void* src, const void* host_header) {
     const char* d = reinterpret_cast<const char*>(src);
     hipStream_t s = c->stream;
-    PUFF_HIP(hipMemcpyAsync(c->h_hdr, d, sizeof(BlobHeader), hipMemcpyDeviceToHost, s));
-    PUFF_HIP(hipStreamSynchronize(s));
-    const BlobHeader hd = *reinterpret_cast<const BlobHeader*>(c->h_hdr);
+    BlobHeader hd;
+    if (host_header) {
+        std::memcpy(&hd, host_header, sizeof(BlobHeader));
+    } else {
+        if (!c->h_hdr) PUFF_HIP(hipHostMalloc(&c->h_hdr, 256, hipHostMallocDefault));
+        PUFF_HIP(hipMemcpyAsync(c->h_hdr, d, sizeof(BlobHeader), hipMemcpyDeviceToHost, s));
+        PUFF_HIP(hipStreamSynchronize(s));
+        hd = *reinterpret_cast<const BlobHeader*>(c->h_hdr);
+    }
     if (hd.magic != BLOB_MAGIC || hd.n != c->n || hd.nmesh != c->nmesh() || hd.precision != c->precision) {
         set_error("state blob does not belong to this context's shape and precision");
         return PUFF_ERR_ARG;
     }
-    const size_t n = (size_t)c->n;
-    PUFF_HIP(hipMemcpyAsync(P.y.x, d + off[0], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(P.y.y, d + off[1], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(P.y.z2, d + off[2], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(P.y.pr, d + off[3], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(P.y.pi, d + off[4], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(P.y.g, d + off[5], sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(P.chi, d + off[6], sizeof(R) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(c->orig, d + off[7], sizeof(int) * n, hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipMemcpyAsync(c->A, d + off[8], sizeof(cplx<R>) * (size_t)c->nmesh(), hipMemcpyDeviceToDevice, s));
-    PUFF_HIP(hipStreamSynchronize(s));
+    launch_copy_sections(blob_sections<R>(c, const_cast<char*>(d), false), s);
+    PUFF_HIP(hipGetLastError());
     c->sorted = hd.sorted != 0;
     c->identity_order = hd.identity_order != 0;
     c->sort_due = hd.sort_due != 0;
@@ -1308,6 +1452,77 @@ static int state_load(Ctx* c, const void* src) {
     c->n2col = hd.n2col;
     c->n2col0 = hd.n2col0;
     c->undgrad = hd.undgrad;
+    return PUFF_OK;
+}
+
+constexpr uint64_t CAPTURE_MAGIC = 0x3150414346465550ull;   // "PUFFCAP1"
+
+// Start of a captured step (after its sort): header, start state and cleared
+// stage sources in the capture; a host copy of the header to cap_hdr.
+template <typename R>
+static void capture_start(Ctx* c, char* cap, void* cap_hdr, CaptureView<R>& v, const double* taper0) {
+    Particles<R>& P = parts<R>(c);
+    capture_view<R>(c, cap, &v);
+    Words64 words{};
+    BlobHeader* hd = reinterpret_cast<BlobHeader*>(words.w);
+    hd->magic = CAPTURE_MAGIC;
+    hd->n = c->n;
+    hd->nmesh = c->nmesh();
+    hd->precision = c->precision;
+    hd->sorted = c->sorted;
+    hd->identity_order = c->identity_order;
+    hd->sort_due = c->sort_due;
+    hd->order_id = c->order_id;
+    hd->steps_since_sort = c->steps_since_sort;
+    hd->n2col = taper0[0];
+    hd->n2col0 = taper0[1];
+    hd->undgrad = taper0[2];
+    hipStream_t s = c->stream;
+    launch_store_words(reinterpret_cast<uint32_t*>(cap), words, (int)(sizeof(BlobHeader) / 4), s);
+    const size_t n = (size_t)c->n;
+    CopySections cs{};
+    const void* src[9] = {P.y.x, P.y.y, P.y.z2, P.y.pr, P.y.pi, P.y.g, P.chi, c->orig, c->A};
+    void* dst[9] = {v.y.x, v.y.y, v.y.z2, v.y.pr, v.y.pi, v.y.g, v.chi, v.orig, v.A0};
+    const size_t bytes[9] = {sizeof(R) * n, sizeof(R) * n, sizeof(double) * n, sizeof(R) * n, sizeof(R) * n,
+                             sizeof(double) * n, sizeof(R) * n, sizeof(int) * n,
+                             sizeof(cplx<R>) * (size_t)c->nmesh()};
+    cs.n = 9;
+    for (int k = 0; k < 9; ++k) {
+        cs.src[k] = reinterpret_cast<const uint8_t*>(src[k]);
+        cs.dst[k] = reinterpret_cast<uint8_t*>(dst[k]);
+        cs.bytes[k] = (int64_t)bytes[k];
+    }
+    launch_copy_sections(cs, s);
+    for (int k = 0; k < 3; ++k) fill_zero_typed<cplx<R>>(v.dA[k], c->nmesh(), s);
+    if (cap_hdr) {
+        std::memset(cap_hdr, 0, PUFF_STATE_HEADER_BYTES);
+        std::memcpy(cap_hdr, hd, sizeof(BlobHeader));
+    }
+}
+
+template <typename R>
+static int capture_check(Ctx* c, const void* cap_hdr, double taper[3], int64_t* order_id) {
+    if (!cap_hdr) {
+        set_error("the adjoint of a captured step needs the capture's host header");
+        return PUFF_ERR_ARG;
+    }
+    BlobHeader hd;
+    std::memcpy(&hd, cap_hdr, sizeof(BlobHeader));
+    if (hd.magic != CAPTURE_MAGIC || hd.n != c->n || hd.nmesh != c->nmesh() || hd.precision != c->precision) {
+        set_error("step capture does not belong to this context's shape and precision");
+        return PUFF_ERR_ARG;
+    }
+    taper[0] = hd.n2col;
+    taper[1] = hd.n2col0;
+    taper[2] = hd.undgrad;
+    *order_id = hd.order_id;
+    return PUFF_OK;
+}
+
+template <typename R>
+static int capture_bytes(Ctx* c, int64_t* bytes) {
+    CaptureView<R> v{};
+    *bytes = (int64_t)capture_view<R>(c, nullptr, &v);
     return PUFF_OK;
 }
 
@@ -1524,6 +1739,8 @@ int puff_profile_enable(puff_ctx* ctx, int32_t on) {
 
 int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds, int32_t reset) {
     CTX_CHECK(ctx);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    puff::prof_collect(&ctx->c);
     if (nkinds < 0 || nkinds > PUFF_PROF_KINDS) return PUFF_ERR_ARG;
     for (int k = 0; k < nkinds; ++k) {
         if (ms) ms[k] = ctx->c.prof_ms[k];
@@ -1561,21 +1778,61 @@ int puff_particle_count(puff_ctx* ctx, int64_t* n) {
     return PUFF_OK;
 }
 
+int puff_set_deferred(puff_ctx* ctx, int32_t on) {
+    CTX_CHECK(ctx);
+    Ctx& c = ctx->c;
+    if (hipStreamSynchronize(c.stream) != hipSuccess) return PUFF_ERR_HIP;
+    puff::prof_collect(&c);
+    c.deferred = on != 0;
+    return PUFF_OK;
+}
+
+int puff_deferred_status(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    Ctx& c = ctx->c;
+    PUFF_HIP(hipMemcpyAsync(c.h_flags, c.flags, sizeof(int) * puff::FLAG_WORDS, hipMemcpyDeviceToHost, c.stream));
+    puff::launch_fill_zero_words(reinterpret_cast<uint32_t*>(c.flags + puff::FLAG_WORD_STICKY), 1, c.stream);
+    PUFF_HIP(hipStreamSynchronize(c.stream));
+    puff::prof_collect(&c);
+    const int f = c.h_flags[puff::FLAG_WORD_STICKY];
+    if (f & puff::FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
+    if (f & puff::FLAG_TRANSVERSE) return PUFF_OUT_TRANSVERSE;
+    if (f) return PUFF_OUT_Z2_BUFFER;
+    return PUFF_OK;
+}
+
+int puff_adjoint_taper_gradient(puff_ctx* ctx, double* out, int32_t n) {
+    CTX_CHECK(ctx);
+    Ctx& c = ctx->c;
+    if (!c.adj) {
+        puff::set_error("adjoint buffers not allocated (puff_adjoint_enable)");
+        return PUFF_ERR_STATE;
+    }
+    if (!out || n < 0 || n > puff::DN2_TABLE_CAP + 1) return PUFF_ERR_ARG;
+    PUFF_HIP(hipMemcpyAsync(out, c.dn2_table, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    PUFF_HIP(hipStreamSynchronize(c.stream));
+    return PUFF_OK;
+}
+
 int puff_state_bytes(puff_ctx* ctx, int64_t* bytes) {
     if (!ctx || !bytes) return PUFF_ERR_ARG;
     return DISPATCH(ctx, state_bytes, bytes);
 }
 
-int puff_state_save(puff_ctx* ctx, void* dst) {
+int puff_state_save(puff_ctx* ctx, void* dst) { return puff_state_save_ex(ctx, dst, nullptr); }
+
+int puff_state_load(puff_ctx* ctx, const void* src) { return puff_state_load_ex(ctx, src, nullptr); }
+
+int puff_state_save_ex(puff_ctx* ctx, void* dst, void* host_header) {
     CTX_CHECK(ctx);
     if (!dst) return PUFF_ERR_ARG;
-    return DISPATCH(ctx, state_save, dst);
+    return DISPATCH(ctx, state_save, dst, host_header);
 }
 
-int puff_state_load(puff_ctx* ctx, const void* src) {
+int puff_state_load_ex(puff_ctx* ctx, const void* src, const void* host_header) {
     CTX_CHECK(ctx);
     if (!src) return PUFF_ERR_ARG;
-    return DISPATCH(ctx, state_load, src);
+    return DISPATCH(ctx, state_load, src, host_header);
 }
 
 int puff_sort_now(puff_ctx* ctx) {
@@ -1604,6 +1861,29 @@ int puff_adjoint_seed_power(puff_ctx* ctx, const double* wz, double scale) {
 int puff_rk4par_adjoint(puff_ctx* ctx, double sz, double h, double* dn2col, int32_t* slot) {
     CTX_CHECK(ctx);
     return DISPATCH(ctx, rk4par_adjoint, sz, h, dn2col, slot);
+}
+
+int puff_capture_bytes(puff_ctx* ctx, int64_t* bytes) {
+    if (!ctx || !bytes) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, capture_bytes, bytes);
+}
+
+int puff_rk4par_capture(puff_ctx* ctx, double sz, double h, void* capture, void* host_header) {
+    CTX_CHECK(ctx);
+    if (!capture || !host_header) return PUFF_ERR_ARG;
+    return ctx->c.precision == PUFF_FP64
+               ? puff::rk4par<double>(&ctx->c, sz, h, reinterpret_cast<char*>(capture), host_header)
+               : puff::rk4par<float>(&ctx->c, sz, h, reinterpret_cast<char*>(capture), host_header);
+}
+
+int puff_rk4par_adjoint_captured(puff_ctx* ctx, double sz, double h, const void* capture,
+                                 const void* host_header, double* dn2col, int32_t* slot) {
+    CTX_CHECK(ctx);
+    if (!capture || !host_header) return PUFF_ERR_ARG;
+    const char* cap = reinterpret_cast<const char*>(capture);
+    return ctx->c.precision == PUFF_FP64
+               ? puff::rk4par_adjoint<double>(&ctx->c, sz, h, dn2col, slot, cap, host_header)
+               : puff::rk4par_adjoint<float>(&ctx->c, sz, h, dn2col, slot, cap, host_header);
 }
 
 int puff_diffraction_adjoint(puff_ctx* ctx, double h) {

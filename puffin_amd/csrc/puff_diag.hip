@@ -112,6 +112,39 @@ __global__ void k_fill_zero(double* p, int64_t n) {
 
 static dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256 > 0 ? (n + 255) / 256 : 1)); }
 
+__global__ void k_store_words(uint32_t* dst, Words64 src, int n) {
+    const int t = threadIdx.x;
+    if (t < n) dst[t] = src.w[t];
+}
+
+__global__ __launch_bounds__(256) void k_copy_sections(CopySections c) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nth = (int64_t)gridDim.x * blockDim.x;
+    for (int s = 0; s < c.n; ++s) {
+        const int64_t n16 = c.bytes[s] / 16;
+        const uint4* sp = reinterpret_cast<const uint4*>(c.src[s]);
+        uint4* dp = reinterpret_cast<uint4*>(c.dst[s]);
+        for (int64_t i = tid; i < n16; i += nth) dp[i] = sp[i];
+        const int64_t rem = (c.bytes[s] - n16 * 16) / 4;
+        const uint32_t* s4 = reinterpret_cast<const uint32_t*>(c.src[s] + n16 * 16);
+        uint32_t* d4 = reinterpret_cast<uint32_t*>(c.dst[s] + n16 * 16);
+        for (int64_t i = tid; i < rem; i += nth) d4[i] = s4[i];
+    }
+}
+
+void launch_copy_sections(const CopySections& c, hipStream_t s) {
+    int64_t total = 0;
+    for (int k = 0; k < c.n; ++k) total += c.bytes[k] / 16;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy_sections, dim3((unsigned)blocks), dim3(256), 0, s, c);
+}
+
+void launch_store_words(uint32_t* dst, const Words64& src, int nwords, hipStream_t s) {
+    hipLaunchKernelGGL(k_store_words, dim3(1), dim3(64), 0, s, dst, src, nwords);
+}
+
 void launch_fill_zero(double* p, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_zero, grid_for(n), dim3(256), 0, s, p, n);
 }

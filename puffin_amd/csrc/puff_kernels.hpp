@@ -62,8 +62,9 @@ struct CombineArgs {
     cplx<R>* d[4];
     double h6;
     const Box* box;
-    const int* flags;
+    int* flags;
     int nx, ny, nz2, nzp, periodic, one_d;
+    int clear_from;        // d[k] with k >= clear_from are cleared after the update
 };
 
 template <typename R>
@@ -112,8 +113,17 @@ struct AdjCombineArgs {
     cplx<R>* mA[4];
     cplx<R>* dA[3];        // forward sources of stages 1-3, cleared here
     const Box* box;
+    int* flags;
     int nx, ny, nz2, nzp, periodic, one_d;
+    int clear_dA;          // clear dA[0..2] (replayed forward sources)
 };
+
+// dn2_table[slot[s]] += sum of the DN2_COPIES accumulators of stage s
+// (slot -1: the entry after the table), s = 0..3.
+struct Slots4 {
+    int s[4];
+};
+void launch_dn2_accumulate(const double* dn2, Slots4 slots, int ntab, double* dn2_table, hipStream_t st);
 
 template <typename R>
 void launch_adj_stage(int stage, const AdjArgs<R>& a, hipStream_t s);
@@ -207,6 +217,21 @@ void launch_bunching(const double* z2, const R* chi, int64_t n, double rho, int 
                      int nslices, double slice, int periodic, double* acc, hipStream_t s);
 
 // ---- data movement ---------------------------------------------------------
+// Up to 64 words written to device memory by a kernel (stream-ordered, no
+// host staging buffer to keep alive).
+struct Words64 {
+    uint32_t w[64];
+};
+void launch_store_words(uint32_t* dst, const Words64& src, int nwords, hipStream_t s);
+// Several device-to-device copies in one launch (byte counts multiple of 4,
+// pointers 16-byte aligned).
+struct CopySections {
+    const uint8_t* src[12];
+    uint8_t* dst[12];
+    int64_t bytes[12];
+    int n;
+};
+void launch_copy_sections(const CopySections& c, hipStream_t s);
 void launch_fill_zero(double* p, int64_t n, hipStream_t s);
 template <typename R>
 void launch_pack_field(const double* ar, const double* ai, cplx<R>* a, int64_t n, hipStream_t s);

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
 
 // Start of a step: clear flags and the active box.
 __global__ void k_step_begin(int* flags, Box* box) {
-    if (threadIdx.x < FLAG_WORDS) flags[threadIdx.x] = 0;
+    if (threadIdx.x < FLAG_WORD_STICKY) flags[threadIdx.x] = 0;
     for (int k = threadIdx.x; k < BOX_COPIES; k += blockDim.x) {
         for (int d = 0; d < 3; ++d) {
             box[k].lo[d] = 0x7fffffff;
@@ -169,10 +169,12 @@ __global__ void k_step_begin(int* flags, Box* box) {
 }
 
 // A += h/6 (dA1 + dA4 + 2 (dA2 + dA3)) over the active box
-// (puffin_mpi_RK4.f90:281-282, 314-315), then clear the four deposits there.
+// (puffin_mpi_RK4.f90:281-282, 314-315), then clear the deposits there (all
+// four, or only the last when the first three are kept for the adjoint).
 // The periodic mesh keeps its aliased last plane equal to plane 0.
 template <typename R>
 __global__ __launch_bounds__(256) void k_field_combine(CombineArgs<R> a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.flags[0]) atomicOr(a.flags + FLAG_WORD_STICKY, a.flags[0]);
     const Box b = merged_box(a.box);
     if (b.hi[0] < 0) return;   // no particle deposited
     const int x0 = b.lo[0], x1 = min(b.hi[0] + 1, a.nx - 1);
@@ -207,10 +209,7 @@ __global__ __launch_bounds__(256) void k_field_combine(CombineArgs<R> a) {
             if (a.periodic && iz == 0) a.A[(int64_t)(a.nz2 - 1) * ntr + (int64_t)iy * a.nx + ix] = v;
         }
         const cplx<R> zero = {(R)0, (R)0};
-        a.d[0][j] = zero;
-        a.d[1][j] = zero;
-        a.d[2][j] = zero;
-        a.d[3][j] = zero;
+        for (int k = a.clear_from; k < 4; ++k) a.d[k][j] = zero;
     }
 }
 

@@ -188,8 +188,9 @@ __device__ __forceinline__ Box merged_box(const Box* box) {
 
 // Whether the workgroup's node box fits the LDS tile and does not wrap
 // around a periodic mesh (block-uniform).
+template <int TILE = TILE_NODES>
 __device__ __forceinline__ bool tile_fits(const MeshGeom& m, const TileBox& t) {
-    return t.any && (int64_t)t.tx * t.ty * t.tz <= TILE_NODES && !(m.periodic && t.hz + 1 >= m.nzp);
+    return t.any && (int64_t)t.tx * t.ty * t.tz <= TILE && !(m.periodic && t.hz + 1 >= m.nzp);
 }
 
 // Tri-linear (or 1-D linear) weights, node indices and the key of a lane's
@@ -234,14 +235,14 @@ __device__ __forceinline__ void cell_weights(const MeshGeom& m, bool ok, const C
 //   else:     lanes of a wavefront that share a cell are summed first (runs
 //             of equal keys), then one global atomic per run and corner.
 // use_tile must be block-uniform; every thread of the workgroup must call.
-template <typename R>
+template <typename R, int TILE = TILE_NODES>
 __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& t, bool use_tile,
                                               bool ok, const Cell<R>& c, int nnb, const R* w,
                                               const int* nd, int t0, int key, R sr, R si,
                                               cplx<R>* out) {
     const int lane = threadIdx.x & 63;
     if (use_tile) {
-        __shared__ double s_tile[2 * TILE_NODES];
+        __shared__ double s_tile[2 * TILE];
         const int nt = t.tx * t.ty * t.tz;
         for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) s_tile[q] = 0.0;
         __syncthreads();

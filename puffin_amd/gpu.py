@@ -52,6 +52,8 @@ SIGNATURES = {
     "puff_state_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "puff_state_save": (ctypes.c_int, [_vp, _vp]),
     "puff_state_load": (ctypes.c_int, [_vp, _vp]),
+    "puff_state_save_ex": (ctypes.c_int, [_vp, _vp, _vp]),
+    "puff_state_load_ex": (ctypes.c_int, [_vp, _vp, _vp]),
     "puff_sort_now": (ctypes.c_int, [_vp]),
     "puff_adjoint_enable": (ctypes.c_int, [_vp]),
     "puff_adjoint_zero": (ctypes.c_int, [_vp]),
@@ -59,6 +61,13 @@ SIGNATURES = {
     "puff_rk4par_adjoint": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _dp,
                                            ctypes.POINTER(ctypes.c_int32)]),
     "puff_diffraction_adjoint": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "puff_capture_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "puff_rk4par_capture": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "puff_rk4par_adjoint_captured": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _dp,
+                                                    ctypes.POINTER(ctypes.c_int32)]),
+    "puff_set_deferred": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "puff_deferred_status": (ctypes.c_int, [_vp]),
+    "puff_adjoint_taper_gradient": (ctypes.c_int, [_vp, _dp, ctypes.c_int32]),
     "puff_adjoint_upload": (ctypes.c_int, [_vp, ctypes.c_int32] + [_vp] * 8),
     "puff_adjoint_download": (ctypes.c_int, [_vp, ctypes.c_int32] + [_vp] * 8),
 }
@@ -122,6 +131,7 @@ def _ptr(a: np.ndarray | None):
 
 
 PARTICLE_KEYS = ("x", "y", "z2", "px", "py", "gam")
+STATE_HEADER_BYTES = 256   # PUFF_STATE_HEADER_BYTES
 
 
 class Integrator:
@@ -271,11 +281,22 @@ class Integrator:
         _check(lib().puff_state_bytes(self._h, ctypes.byref(b)), "puff_state_bytes")
         return int(b.value)
 
-    def state_save(self, dev_ptr: int):
-        _check(lib().puff_state_save(self._h, dev_ptr), "puff_state_save")
+    def state_save(self, dev_ptr: int, header: np.ndarray | None = None):
+        """Snapshot into device memory at dev_ptr; ``header`` (uint8[256]),
+        when given, receives a host copy of the blob's header."""
+        hp = None
+        if header is not None:
+            assert header.dtype == np.uint8 and header.size >= STATE_HEADER_BYTES
+            hp = header.ctypes.data
+        _check(lib().puff_state_save_ex(self._h, dev_ptr, hp), "puff_state_save")
 
-    def state_load(self, dev_ptr: int):
-        _check(lib().puff_state_load(self._h, dev_ptr), "puff_state_load")
+    def state_load(self, dev_ptr: int, header: np.ndarray | None = None):
+        """Restore a snapshot; with the header of its save, without a wait."""
+        hp = None
+        if header is not None:
+            assert header.dtype == np.uint8 and header.size >= STATE_HEADER_BYTES
+            hp = header.ctypes.data
+        _check(lib().puff_state_load_ex(self._h, dev_ptr, hp), "puff_state_load")
 
     def sort_now(self):
         _check(lib().puff_sort_now(self._h), "puff_sort_now")
@@ -320,3 +341,37 @@ class Integrator:
         _check(lib().puff_adjoint_download(self._h, MEM_HOST, *[_ptr(out[k]) for k in PARTICLE_KEYS],
                                            _ptr(lar), _ptr(lai)), "puff_adjoint_download")
         return out, lar, lai
+
+    # ---- deferred status
+    def set_deferred(self, on: bool):
+        _check(lib().puff_set_deferred(self._h, int(on)), "puff_set_deferred")
+
+    def deferred_status(self) -> int:
+        return _check(lib().puff_deferred_status(self._h), "puff_deferred_status")
+
+    def adjoint_taper_gradient(self, n: int) -> np.ndarray:
+        out = np.zeros(n)
+        _check(lib().puff_adjoint_taper_gradient(self._h, out.ctypes.data_as(_dp), n),
+               "puff_adjoint_taper_gradient")
+        return out
+
+    # ---- step capture
+    def capture_bytes(self) -> int:
+        b = ctypes.c_int64()
+        _check(lib().puff_capture_bytes(self._h, ctypes.byref(b)), "puff_capture_bytes")
+        return int(b.value)
+
+    def rk4par_capture(self, sz: float, h: float, dev_ptr: int, header: np.ndarray) -> int:
+        """rk4par that keeps the step's start state and stage data at dev_ptr
+        (capture_bytes() bytes) and its header in ``header`` (uint8[256])."""
+        assert header.dtype == np.uint8 and header.size >= STATE_HEADER_BYTES
+        return _check(lib().puff_rk4par_capture(self._h, sz, h, dev_ptr, header.ctypes.data),
+                      "puff_rk4par_capture")
+
+    def rk4par_adjoint_captured(self, sz: float, h: float, dev_ptr: int, header: np.ndarray):
+        dn = np.zeros(4)
+        slot = (ctypes.c_int32 * 4)()
+        st = _check(lib().puff_rk4par_adjoint_captured(self._h, sz, h, dev_ptr, header.ctypes.data,
+                                                       dn.ctypes.data_as(_dp), slot),
+                    "puff_rk4par_adjoint_captured")
+        return st, dn, np.array(list(slot), dtype=np.int64)

@@ -123,25 +123,28 @@ def test_state_snapshot_round_trip():
             assert np.abs(q1[k] - q2[k]).max() <= 1e-12 * np.abs(q1[k]).max() + 1e-300, k
 
 
-def _engine(gpu, c, precision, every):
+def _engine(gpu, c, precision, every, capture=None):
     from puffin_amd.adjoint import ModuleGradient
     mp = c["mp"]
     drv = gpu.Integrator(mp, c["parts"]["x"].size, precision)
     drv.upload_particles(c["parts"])
     drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
-    eng = ModuleGradient(drv, c["ops"], c["shape"], every=every, taper_period=c["period"])
+    eng = ModuleGradient(drv, c["ops"], c["shape"], every=every, taper_period=c["period"],
+                         capture=capture)
     return drv, eng
 
 
+@pytest.mark.parametrize("capture", [False, True])
 @pytest.mark.parametrize("periodic", [1, 0])
-def test_module_gradient_matches_oracle(periodic):
+def test_module_gradient_matches_oracle(periodic, capture):
     """Checkpoints every 10 steps (segments end between diffractions and
-    wraps); the default sort interval (30 steps) sorts inside segments."""
-    print(f"[gpu] module gradient periodic={periodic}", flush=True)
+    wraps); the default sort interval (30 steps) sorts inside segments; the
+    adjoint steps replay their forward stages or use the step captures."""
+    print(f"[gpu] module gradient periodic={periodic} capture={capture}", flush=True)
     gpu = _gpu()
     c = small_module(periodic=periodic)
     J0, gt0, gs0, _ = OracleModule(c).gradient(c["table"], c["seed"])
-    drv, eng = _engine(gpu, c, FP64, every=10)
+    drv, eng = _engine(gpu, c, FP64, every=10, capture=capture)
     try:
         J = eng.forward(c["table"], c["seed"])
         g = eng.backward()
@@ -181,7 +184,7 @@ def test_mixed_gradient_tracks_fp64():
     c = small_module(periodic=1, n=4000)
     out = {}
     for prec in (FP64, MIXED):
-        drv, eng = _engine(gpu, c, prec, every=12)
+        drv, eng = _engine(gpu, c, prec, every=12, capture=(prec == MIXED))
         try:
             J = eng.forward(c["table"], c["seed"])
             out[prec] = (J, eng.backward())
