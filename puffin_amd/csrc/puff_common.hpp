@@ -55,11 +55,11 @@ struct Box {
 constexpr int BOX_COPIES = 64;
 
 // Workgroup of the RK4 stage kernels (one macroparticle per thread) and the
-// node capacity of its LDS deposit tile (2 FP64 per node: 64 KiB of the
-// CU's 160 KiB).
-constexpr int STAGE_THREADS = 1024;
+// node capacity of its LDS deposit tile (2 FP64 per node: 32 KiB, so that
+// three 8-wave workgroups share a CU's 160 KiB).
+constexpr int STAGE_THREADS = 512;
 #ifndef PUFF_TILE_NODES
-#define PUFF_TILE_NODES 4096
+#define PUFF_TILE_NODES 2048
 #endif
 constexpr int TILE_NODES = PUFF_TILE_NODES;
 

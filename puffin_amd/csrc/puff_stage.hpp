@@ -228,8 +228,9 @@ __device__ __forceinline__ void cell_weights(const MeshGeom& m, bool ok, const C
 }
 
 // out[node_k] += w_k (sr, si) for every lane with a cell.
-//   use_tile: FP64 accumulation in an LDS tile over the workgroup's node box,
-//             then one global atomic per touched node and component.  Each
+//   use_tile: FP64 accumulation in an LDS tile over the workgroup's node box
+//             (real parts, then imaginary parts), then one global atomic per
+//             touched node and component.  Each
 //             lane visits the corners in a rotated order so that lanes of one
 //             cell do not hit the same LDS address in the same instruction.
 //   else:     lanes of a wavefront that share a cell are summed first (runs
@@ -264,14 +265,14 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
                     wk = (double)((R)wx[k & 1] * (R)wy[(k >> 1) & 1] * (R)wz[k >> 2]);
                     tk = t0 + (k & 1) + ((k & 2) ? t.tx : 0) + ((k & 4) ? txy : 0);
                 }
-                atomicAdd(&s_tile[2 * tk], wk * dsr);
-                atomicAdd(&s_tile[2 * tk + 1], wk * dsi);
+                atomicAdd(&s_tile[tk], wk * dsr);
+                atomicAdd(&s_tile[nt + tk], wk * dsi);
             }
         }
         __syncthreads();
         const int ntr = m.nx * m.ny;
         for (int q = threadIdx.x; q < nt; q += blockDim.x) {
-            const double tr = s_tile[2 * q], ti = s_tile[2 * q + 1];
+            const double tr = s_tile[q], ti = s_tile[nt + q];
             if (tr != 0.0 || ti != 0.0) {
                 const int lx = q % t.tx, r = q / t.tx;
                 const int ly = r % t.ty, lz = r / t.ty;
