@@ -335,9 +335,14 @@ def main():
     ap.add_argument("--accuracy-periods", type=int, default=10,
                     help="undulator periods of the FP64-vs-mixed accuracy run (0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient measurement")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the gradient measurement (its JSON object on stdout)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_bench()), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
