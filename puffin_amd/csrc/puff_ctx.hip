@@ -180,10 +180,16 @@ static void factorize(int n, FftAxis& ax) {
 
 template <typename R>
 static int make_axis(Ctx* c, int n, FftAxis& ax) {
+    // one line, its ping-pong buffer and the twiddle table share a
+    // workgroup's LDS (160 KiB on gfx950)
+    if (3 * (size_t)n * sizeof(cplx<R>) > 160 * 1024) {
+        set_error("FFT line of " + std::to_string(n) + " points does not fit one workgroup's LDS");
+        return PUFF_ERR_ARG;
+    }
     factorize(n, ax);
     std::vector<cplx<R>> tw(n);
     for (int e = 0; e < n; ++e) {
-        // exp(-2 pi i e / n), argument reduced to the nearer quadrant
+        // exp(-2 pi i e / n) in long double, rounded once
         const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
         tw[e] = {(R)std::cos(a), (R)std::sin(a)};
     }

@@ -79,26 +79,38 @@ __device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax
                 }
             }
         } else {
-            // direct DFT pass: work item (b, i, q) -> one output
+            // direct DFT pass for an odd prime r: the outputs q and r - q share
+            // the pre-twiddled inputs y_j = x_j W^(j k twst), with
+            // X_q = sum y_j w^(jq) and X_(r-q) = sum y_j conj(w^(jq)).
             const int nr = N / r;
-            for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+            const int hq = (r + 1) / 2;
+            for (int w = threadIdx.x; w < B * T * hq; w += blockDim.x) {
                 int b, t;
-                if (IL) { b = w % B; t = w / B; } else { t = w % N; b = w / N; }
+                if (IL) { b = w % B; t = w / B; } else { t = w % (T * hq); b = w / (T * hq); }
                 const int q = t / T;
                 const int i = t % T;
                 const int k = i % p;
-                const int de = (k * twst + q * nr) % N;   // exponent step per input
-                int e = 0;
-                R sr = 0, si = 0;
+                const int dk = k * twst;
+                const int dq = q * nr;
+                int e = 0, f = 0;
+                R s0r = 0, s0i = 0, s1r = 0, s1i = 0;
                 for (int j = 0; j < r; ++j) {
-                    const cplx<R> x = src[lidx<IL>(b, i + j * T, B, N)];
-                    const cplx<R> wv = tw_get(tw, e, sign);
-                    sr += x.x * wv.x - x.y * wv.y;
-                    si += x.x * wv.y + x.y * wv.x;
-                    e += de;
+                    cplx<R> x = src[lidx<IL>(b, i + j * T, B, N)];
+                    if (e) x = cmul(x, tw_get(tw, e, sign));
+                    const cplx<R> om = tw_get(tw, f, sign);
+                    const R ac = x.x * om.x, bs = x.y * om.y, as = x.x * om.y, bc = x.y * om.x;
+                    s0r += ac - bs;
+                    s0i += as + bc;
+                    s1r += ac + bs;
+                    s1i += bc - as;
+                    e += dk;
                     if (e >= N) e -= N;
+                    f += dq;
+                    if (f >= N) f -= N;
                 }
-                dst[lidx<IL>(b, (i / p) * p * r + k + q * p, B, N)] = {sr, si};
+                const int ob = (i / p) * p * r + k;
+                dst[lidx<IL>(b, ob + q * p, B, N)] = {s0r, s0i};
+                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, N)] = {s1r, s1i};
             }
         }
         __syncthreads();
@@ -120,7 +132,9 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
     const int64_t nlines = (int64_t)a.ny * a.nzp;
     const int64_t L0 = (int64_t)blockIdx.x * B;
     const int nb = (int)min<int64_t>(B, nlines - L0);
-    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.ax.tw);
+    const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(a.ax.tw);
+    cplx<R>* tw = s1 + B * N;   // twiddle table staged in LDS
+    for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
     cplx<R>* base = a.a + L0 * N;
     const int64_t ntr = (int64_t)a.nx * a.ny;
     for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
@@ -217,7 +231,9 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int B, int sign) {
     const int x0 = blockIdx.x * B;
     const int iz = blockIdx.y;
     const int nb = min(B, a.nx - x0);
-    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.ay.tw);
+    const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(a.ay.tw);
+    cplx<R>* tw = s1 + B * N;   // twiddle table staged in LDS
+    for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
     cplx<R>* base = a.a + (int64_t)iz * a.nx * a.ny + x0;
     for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
         const int b = w % B, j = w / B;
@@ -243,6 +259,7 @@ __global__ __launch_bounds__(256) void k_fft_z(DiffArgs<R> a, int B, int op, int
     const int iy = blockIdx.y;
     const int nb = min(B, a.nx - x0);
     const int64_t ntr = (int64_t)a.nx * a.ny;
+    // few lines per workgroup here: the twiddles stay in global memory (L2)
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.az.tw);
     cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
     for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
@@ -296,9 +313,24 @@ __global__ __launch_bounds__(256) void k_fft_z(DiffArgs<R> a, int B, int op, int
 
 // ---- host side -----------------------------------------------------------------
 
-static int lines_per_block(int n, size_t elem, int cap) {
-    const size_t budget = 64 * 1024;   // two LDS buffers per block
-    int b = (int)(budget / (2 * (size_t)n * elem));
+
+// Dynamic LDS above 64 KiB needs the kernel's opt-in (up to 160 KiB).
+static void lds_opt_in(const void* kernel, size_t bytes) {
+    if (bytes > 64 * 1024)
+        (void)hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+// Lines per workgroup: two LDS line buffers and the twiddle table within
+// `budget` bytes.
+#ifndef PUFF_Z_LDS_BUDGET
+#define PUFF_Z_LDS_BUDGET (32 * 1024)
+#endif
+#ifndef PUFF_XY_LDS_BUDGET
+#define PUFF_XY_LDS_BUDGET (24 * 1024)
+#endif
+static int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_LDS_BUDGET) {
+    const size_t tw = (size_t)n * elem;
+    int b = budget > tw ? (int)((budget - tw) / (2 * (size_t)n * elem)) : 0;
     if (b < 1) b = 1;
     if (b > cap) b = cap;
     return b;
@@ -308,22 +340,28 @@ template <typename R>
 static void run_x(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
     const int B = lines_per_block(a.nx, sizeof(cplx<R>), 64);
     const int64_t nlines = (int64_t)a.ny * a.nzp;
-    const size_t sh = 2 * (size_t)B * a.nx * sizeof(cplx<R>);
+    const size_t sh = (2 * (size_t)B + 1) * a.nx * sizeof(cplx<R>);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_x<R>), sh);
     hipLaunchKernelGGL((k_fft_x<R>), dim3((unsigned)((nlines + B - 1) / B)), dim3(256), sh, s, a, B, op, sign);
 }
 
 template <typename R>
 static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
     const int B = lines_per_block(a.ny, sizeof(cplx<R>), 16);
-    const size_t sh = 2 * (size_t)B * a.ny * sizeof(cplx<R>);
+    const size_t sh = (2 * (size_t)B + 1) * a.ny * sizeof(cplx<R>);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_y<R>), sh);
     hipLaunchKernelGGL((k_fft_y<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(256), sh, s,
                        a, B, sign);
 }
 
 template <typename R>
 static void run_z(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
-    const int B = lines_per_block(a.nzp, sizeof(cplx<R>), 16);
+    // two line buffers, no twiddle table (wider workgroups with longer runs
+    // along x measured slower: occupancy matters more than coalescing here)
+    int B = (int)((PUFF_Z_LDS_BUDGET) / (2 * (size_t)a.nzp * sizeof(cplx<R>)));
+    B = B < 1 ? 1 : (B > 16 ? 16 : B);
     const size_t sh = 2 * (size_t)B * a.nzp * sizeof(cplx<R>);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R>), sh);
     hipLaunchKernelGGL((k_fft_z<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.ny), dim3(256), sh, s,
                        a, B, op, sign);
 }
