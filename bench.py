@@ -305,6 +305,96 @@ def adjoint_bench(steps_check: bool = True):
         drv.close()
 
 
+def device_memory_used() -> int:
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    return int(total - free)
+
+
+def _kinds(prof: dict) -> dict:
+    return {k: dict(ms=round(v[0], 3), calls=v[1]) for k, v in prof.items() if v[1]}
+
+
+def c4_bench(steps: int = 30, adjoint_steps: int = 30, every: int = 10):
+    """The large FP32-mixed workload (257 x 257 x 4096 temporal mesh, 2^27
+    macroparticles, per-period taper table): throughput over `steps` steps
+    (one diffraction), device time per kernel kind over as many more, device
+    memory, and the gradient of a module of `adjoint_steps` steps (snapshots
+    every `every` steps) with its time and memory."""
+    from puffin_amd import gpu
+    from puffin_amd.adjoint import ModuleGradient, module_program
+    from puffin_amd.configs import case_c4
+    t = time.perf_counter()
+    c = case_c4()
+    mp = c.mp
+    log(f"c4: workload generated in {time.perf_counter() - t:.1f} s")
+    base = device_memory_used()
+    drv = gpu.Integrator(mp, c.n, 1)
+    out = dict(workload="planar FEL, temporal mesh 257x257x4096, 2^27 macroparticles, FP32-mixed",
+               macroparticles=c.n, mesh=[mp.nx, mp.ny, mp.nz2], steps_per_diffraction=c.steps4diff)
+
+    def run(szl, k):
+        for istep in range(1, k + 1):
+            st = drv.rk4par(szl, c.h)
+            if st != 0:
+                raise RuntimeError(f"c4 rk4par status {st}")
+            szl += c.h
+            if istep % c.steps4diff == 0:
+                drv.diffraction_step(c.steps4diff * c.h)
+        return szl
+
+    try:
+        drv.upload_particles(c.particles)
+        drv.upload_field(*c.field)
+        drv.set_taper_table(c.taper_table, 0.0, c.taper_period)
+        out["context_bytes"] = device_memory_used() - base
+        szl = run(0.0, 3)
+        drv.synchronize()
+        t0 = time.perf_counter()
+        szl = run(szl, steps)
+        drv.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        out.update(ms_per_step=ms, value=c.n / (ms * 1e-3), unit="macroparticle-steps/s", steps=steps)
+        drv.profile(True)
+        drv.profile_read(reset=True)
+        szl = run(szl, steps)
+        out["device_ms_by_kind"] = _kinds(drv.profile_read(reset=True))
+        drv.profile(False)
+        log(f"c4: {ms:.2f} ms/step, context {out['context_bytes'] / 2**30:.1f} GiB")
+        # gradient of a module from the initial beam
+        drv.upload_particles(c.particles)
+        ops = module_program(adjoint_steps, c.h, c.steps4diff, diffraction=True, periodic=False)
+        shape = (c.field[0] / c.info["seed_amp"], c.field[1] / c.info["seed_amp"])
+        eng = ModuleGradient(drv, ops, shape, every=every, taper_period=c.taper_period)
+        table, seed = c.taper_table, c.info["seed_amp"]
+        eng.evaluate(table, seed)   # first use of the per-module inputs
+        drv.synchronize()
+        t0 = time.perf_counter()
+        eng.evaluate(table, seed)
+        t1 = time.perf_counter()
+        J = eng.forward(table, seed)
+        t2 = time.perf_counter()
+        drv.profile(True)
+        drv.profile_read(reset=True)
+        g = eng.backward()
+        t3 = time.perf_counter()
+        prof = _kinds(drv.profile_read(reset=True))
+        drv.profile(False)
+        out["adjoint"] = dict(steps=adjoint_steps, checkpoint_every=every, J=J,
+                              forward_s=t1 - t0, forward_checkpointed_s=t2 - t1, backward_s=t3 - t2,
+                              gradient_cost_vs_forward=(t3 - t1) / (t1 - t0),
+                              snapshot_bytes=eng.memory_bytes(), captures=eng.capture,
+                              total_device_bytes=device_memory_used() - base,
+                              backward_device_ms_by_kind=prof,
+                              grad_seed=g["seed"], grad_table_first=float(g["table"][0]))
+        log(f"c4 adjoint: forward {t1 - t0:.2f} s, checkpointed {t2 - t1:.2f} s, backward {t3 - t2:.2f} s, "
+            f"device {out['adjoint']['total_device_bytes'] / 2**30:.1f} GiB")
+        del eng
+    finally:
+        drv.close()
+    return out
+
+
 def calibrate_copy(mib: int):
     """A plain device-to-device copy of known size (reads and writes mib MiB
     each, 5 times), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -337,12 +427,18 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient measurement")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the gradient measurement (its JSON object on stdout)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the large FP32-mixed workload")
+    ap.add_argument("--c4-only", action="store_true",
+                    help="only the large FP32-mixed workload (its JSON object on stdout)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
     if args.adjoint_only:
         print(json.dumps(adjoint_bench()), flush=True)
+        return
+    if args.c4_only:
+        print(json.dumps(c4_bench()), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -415,6 +511,12 @@ def main():
         line["mixed"]["accuracy"] = precision_report(case, 30 * args.accuracy_periods)
     if not args.no_adjoint and args.case == "c1":
         line["adjoint"] = adjoint_bench()
+    if not args.no_c4 and args.case == "c1":
+        try:
+            line["c4"] = c4_bench()
+        except Exception as e:   # reported in the line, the C1 measurement stands
+            log(f"c4 failed: {e!r}")
+            line["c4"] = dict(error=repr(e))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case, args.cpu_sample, args.cpu_steps)
     else:

@@ -195,4 +195,49 @@ def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
                 n2col0=1.0, undgrad=undgrad, info=dict(gamma_r=gamma_r, periods=periods))
 
 
-CASES = {"c0": case_c0, "c1": case_c1, "c3": case_c3}
+def case_c4(n: int = 1 << 27, nx: int = 257, nz2: int = 4096, periods: int = 128,
+            run_periods: int | None = None) -> Case:
+    """Large planar (ux = 1, uy = 0) FEL for the FP32-mixed form: temporal
+    mesh 257 x 257 x 4096, 2^27 macroparticles, Gaussian transverse sigma = 2
+    (cut at 4 sigma), flat-top current over 2900 z2 nodes with Gaussian edges
+    (sigma 12 nodes), Gamma spread 1e-3, rho = 0.005, 8 nodes per resonant
+    wavelength, a per-period taper table of `periods` values
+    n2col_k = 1 - 0.01 (k / (periods - 1))^2, Gaussian seed (w0 = 2,
+    amplitude 1e-3).  128 periods slip the beam by 1024 nodes: the flat top
+    spans nodes 40..2940 (edges to 3 sigma beyond), so the whole beam stays
+    on the mesh for the whole module."""
+    rng = np.random.Generator(np.random.Philox(SEED_BASE + 4))
+    rho = 0.005
+    ux, uy = 1.0, 0.0
+    aw = 1.0
+    gamma_r = gamma_r_for_eta(1.0e-4, aw, ux, uy)
+    eta, kappa, fx, fy = scaling(rho, aw, gamma_r, ux, uy)
+    dz2 = 4.0 * PI_REF * rho / 8.0
+    dx = 0.1
+    h = step_size(rho, 30)
+    nsteps = 30 * periods
+    mp = ModuleParams(nx=nx, ny=nx, nz2=nz2, dx=dx, dy=dx, dz2=dz2, rho=rho, eta=eta, kappa=kappa,
+                      gamma_r=gamma_r, aw=aw, fx=fx, fy=fy, periodic=0, szfs=0.0, szfe=nsteps * h,
+                      filt_cutoff=filter_cutoff(rho, nz2, dz2, False, 0.3))
+    flat = 2900 * dz2 * (nz2 / 4096.0)
+    edge = 12 * dz2
+    z0 = 40 * dz2
+    z2 = z0 + flat * rng.random(n) + _truncated_normal(rng, n, edge, 3.0 * edge)
+    sx = 2.0
+    p = dict(x=_truncated_normal(rng, n, sx, 4.0 * sx), y=_truncated_normal(rng, n, sx, 4.0 * sx),
+             z2=z2, px=np.zeros(n), py=np.zeros(n),
+             gam=1.0 + 1.0e-3 * rng.standard_normal(n))
+    p["chi"] = np.full(n, 2 * math.pi * sx * sx * flat / n)
+    k = np.arange(periods, dtype=np.float64)
+    table = 1.0 - 0.01 * (k / max(periods - 1, 1)) ** 2
+    match_in(mp, float(table[0]), p)
+    ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0)
+    run = nsteps if run_periods is None else 30 * run_periods
+    c = Case("c4_planar_large_257x257x4096", mp, p, (ar, ai), h, run, steps_for_diffraction(rho, h),
+             n2col0=float(table[0]), info=dict(gamma_r=gamma_r, periods=periods, seed_amp=1.0e-3))
+    c.taper_table = table
+    c.taper_period = 4.0 * PI_REF * rho
+    return c
+
+
+CASES = {"c0": case_c0, "c1": case_c1, "c3": case_c3, "c4": case_c4}
