@@ -198,10 +198,11 @@ int puff_particle_count(puff_ctx* ctx, int64_t* n);
  * Kinds: 0..3 RK4 stage kernels 1..4, 4 field combination, 5 diffraction
  * (all passes), 6 step start, 7 adjoint stage kernels (all four), 8 adjoint
  * field combination, 9 adjoint diffraction, 10 forward stages replayed by
- * the adjoint step.  ms[k] is the summed duration, counts[k] the number of
- * launches since the last reset; nkinds <= PUFF_PROF_KINDS.
+ * the adjoint step, 11 macroparticle re-sort.  ms[k] is the summed duration,
+ * counts[k] the number of launches since the last reset;
+ * nkinds <= PUFF_PROF_KINDS.
  */
-#define PUFF_PROF_KINDS 11
+#define PUFF_PROF_KINDS 12
 int puff_profile_enable(puff_ctx* ctx, int32_t on);
 int puff_profile_read(puff_ctx* ctx, double* ms, int64_t* counts, int32_t nkinds,
                       int32_t reset);

@@ -55,11 +55,16 @@ struct Box {
 constexpr int BOX_COPIES = 64;
 
 // Workgroup of the RK4 stage kernels (one macroparticle per thread) and the
-// node capacity of its LDS deposit tile (2 FP64 per node: 32 KiB, so that
-// three 8-wave workgroups share a CU's 160 KiB).
-constexpr int STAGE_THREADS = 512;
+// node capacity of its LDS deposit tile (2 FP64 per node).  Four-wave
+// workgroups: the FP64 stage kernels hold 83-99 VGPRs (4-5 waves per SIMD),
+// so one wave per SIMD per workgroup lets 4-5 workgroups share a CU, and a
+// 1536-node tile (24 KiB) keeps LDS from limiting that.
+#ifndef PUFF_STAGE_THREADS
+#define PUFF_STAGE_THREADS 256
+#endif
+constexpr int STAGE_THREADS = PUFF_STAGE_THREADS;
 #ifndef PUFF_TILE_NODES
-#define PUFF_TILE_NODES 2048
+#define PUFF_TILE_NODES 1536
 #endif
 constexpr int TILE_NODES = PUFF_TILE_NODES;
 
@@ -67,6 +72,7 @@ constexpr int TILE_NODES = PUFF_TILE_NODES;
 // particle than the forward, so smaller workgroups with a larger VGPR budget.
 constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_TILE_NODES = 1024;
+constexpr int MAX_BOX_WAVES = (STAGE_THREADS > ADJ_THREADS ? STAGE_THREADS : ADJ_THREADS) / 64;
 
 // Flag words of the step kernels: [0] flag bits of the current step;
 // [FLAG_WORD_FALLBACK] the stage workgroups of the step whose node box did not

@@ -635,6 +635,7 @@ static size_t capture_view(const Ctx* c, char* base, CaptureView<R>* v) {
 
 template <typename R>
 static void sort_particles(Ctx* c) {
+    const int slot = prof_start(c, 11);
     Particles<R>& P = parts<R>(c);
     SortArgs<R> a{};
     a.n = c->n;
@@ -653,6 +654,7 @@ static void sort_particles(Ctx* c) {
     a.perm = c->sort_perm;
     a.tile_sums = c->sort_tiles;
     launch_sort<R>(a, c->stream);
+    prof_stop(c, slot);
     c->chi_spare = spare_chi;
     c->sorted = true;
     c->identity_order = false;
@@ -754,6 +756,8 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
         // particles drift out of their sorted runs (faster once the beam's
         // energy spread has grown): re-sort when the share of workgroups that
         // could not use the LDS tile exceeds twice its value after the sort
+        // (measured on the bunched C1 beam: sorting at that point beats
+        // letting the share grow until the fallback has cost one sort)
         const double groups = 4.0 * (double)((c->n + STAGE_THREADS - 1) / STAGE_THREADS);
         const double share = groups > 0 ? (double)c->h_flags[FLAG_WORD_FALLBACK] / groups : 0.0;
         if (c->steps_since_sort == 1) c->fallback_base = share;

@@ -132,7 +132,7 @@ struct TileBox {
 template <typename R>
 __device__ __forceinline__ TileBox workgroup_box(const MeshGeom& m, bool ok, const Cell<R>& c,
                                                  Box* box) {
-    __shared__ int s_red[6][STAGE_THREADS / 64];
+    __shared__ int s_red[6][MAX_BOX_WAVES];
     __shared__ int s_bb[6];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;

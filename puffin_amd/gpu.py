@@ -73,7 +73,7 @@ SIGNATURES = {
 }
 
 PROFILE_KINDS = ("stage1", "stage2", "stage3", "stage4", "combine", "diffraction", "step_begin",
-                 "adj_stages", "adj_combine", "adj_diffraction", "adj_replay")
+                 "adj_stages", "adj_combine", "adj_diffraction", "adj_replay", "sort")
 
 
 class PuffinError(RuntimeError):
