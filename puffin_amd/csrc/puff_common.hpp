@@ -67,6 +67,13 @@ constexpr int STAGE_THREADS = PUFF_STAGE_THREADS;
 #define PUFF_TILE_NODES 1536
 #endif
 constexpr int TILE_NODES = PUFF_TILE_NODES;
+// Node capacity of the stage kernels' LDS copy of the field over the
+// workgroup's node box (one complex per node; larger boxes gather from
+// global memory).
+#ifndef PUFF_FIELD_TILE_NODES
+#define PUFF_FIELD_TILE_NODES 256
+#endif
+constexpr int FIELD_TILE_NODES = PUFF_FIELD_TILE_NODES;
 
 // Workgroup and LDS tile of the adjoint stage kernels: more live values per
 // particle than the forward, so smaller workgroups with a larger VGPR budget.
@@ -82,6 +89,25 @@ constexpr int MAX_BOX_WAVES = (STAGE_THREADS > ADJ_THREADS ? STAGE_THREADS : ADJ
 constexpr int FLAG_WORDS = 3;
 constexpr int FLAG_WORD_FALLBACK = 1;
 constexpr int FLAG_WORD_STICKY = 2;
+
+// Workgroups of a launch are handed to the XCDs round-robin (workgroup id
+// mod NXCD), and each XCD has its own L2.  xcd_block() gives each XCD a
+// contiguous range of logical block ids instead, so that neighbouring blocks
+// (x-columns sharing cache lines in the z2 FFT pass) meet in one L2.  The
+// launch has xcd_grid(nblocks) workgroups; logical ids >= nblocks are idle.
+// (The stage kernels keep the round-robin order: measured 3 % faster.)
+#ifndef PUFF_XCD_REMAP
+#define PUFF_XCD_REMAP 1
+#endif
+constexpr int NXCD = 8;
+inline int64_t xcd_grid(int64_t nblocks) {
+    return PUFF_XCD_REMAP ? (nblocks + NXCD - 1) / NXCD * NXCD : nblocks;
+}
+__device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
+    if (!PUFF_XCD_REMAP) return blockIdx.x;
+    const int64_t per = (nblocks + NXCD - 1) / NXCD;
+    return (int64_t)(blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
+}
 
 // Wavefront (64-lane) reductions.
 __device__ __forceinline__ int wave_min(int v) {
@@ -105,12 +131,23 @@ struct StageScalars {
     int und_type;
 };
 
-// 1-D FFT plan of one axis: radices (product N), twiddle table on device.
+// 1-D FFT plan of one axis.  Mixed radix: radices (product n) and the
+// twiddle table exp(-2 pi i e / n), e < n.  Rader (n a prime with n - 1 a
+// power of two, e.g. 17 or 257): the n-point DFT as a cyclic convolution of
+// length m = n - 1; radix/npass then describe the m-point transform, tw
+// holds exp(-2 pi i e / m) (n entries, the last one unused), and
+//   rperm[j]  = g^j mod n, riperm[q] = g^-q mod n (g a primitive root),
+//   rb        = cplx<R>[2 m]: DFT_m(b_s) / m for s = -1 then +1, with
+//               b_s[j] = exp(s 2 pi i riperm[j] / n).
 struct FftAxis {
     int n;
     int npass;
     int radix[16];
-    const void* tw;   // cplx<R>[n]: exp(-2 pi i e / n)
+    const void* tw;
+    int rader;
+    const int* rperm;
+    const int* riperm;
+    const void* rb;
 };
 
 }  // namespace puff

@@ -166,16 +166,42 @@ static int dalloc(Ctx* c, void** p, size_t bytes) {
 
 // ---- FFT plans ----------------------------------------------------------------
 
+static bool is_prime(int n) {
+    if (n < 2) return false;
+    for (int f = 2; (int64_t)f * f <= n; ++f)
+        if (n % f == 0) return false;
+    return true;
+}
+
+static int powmod(int b, int e, int n) {
+    int64_t r = 1, x = b % n;
+    for (; e > 0; e >>= 1) {
+        if (e & 1) r = r * x % n;
+        x = x * x % n;
+    }
+    return (int)r;
+}
+
+// Radix passes of an n-point line: factors of 5 and more (direct DFT passes)
+// first, largest first, so that the costliest pass runs with p = 1 and
+// needs no input twiddles; then 4s, 2s and 3s.
 static void factorize(int n, FftAxis& ax) {
     ax.n = n;
     ax.npass = 0;
     int m = n;
-    while (m % 4 == 0) { ax.radix[ax.npass++] = 4; m /= 4; }
-    while (m % 2 == 0) { ax.radix[ax.npass++] = 2; m /= 2; }
-    while (m % 3 == 0) { ax.radix[ax.npass++] = 3; m /= 3; }
+    int n4 = 0, n2 = 0, n3 = 0;
+    while (m % 4 == 0) { ++n4; m /= 4; }
+    while (m % 2 == 0) { ++n2; m /= 2; }
+    while (m % 3 == 0) { ++n3; m /= 3; }
+    int big[32];
+    int nbig = 0;
     for (int f = 5; m > 1 && (int64_t)f * f <= m; f += 2)
-        while (m % f == 0) { ax.radix[ax.npass++] = f; m /= f; }
-    if (m > 1) ax.radix[ax.npass++] = m;
+        while (m % f == 0) { big[nbig++] = f; m /= f; }
+    if (m > 1) big[nbig++] = m;
+    for (int k = nbig - 1; k >= 0; --k) ax.radix[ax.npass++] = big[k];
+    for (int k = 0; k < n4; ++k) ax.radix[ax.npass++] = 4;
+    for (int k = 0; k < n2; ++k) ax.radix[ax.npass++] = 2;
+    for (int k = 0; k < n3; ++k) ax.radix[ax.npass++] = 3;
 }
 
 template <typename R>
@@ -186,17 +212,65 @@ static int make_axis(Ctx* c, int n, FftAxis& ax) {
         set_error("FFT line of " + std::to_string(n) + " points does not fit one workgroup's LDS");
         return PUFF_ERR_ARG;
     }
-    factorize(n, ax);
-    std::vector<cplx<R>> tw(n);
-    for (int e = 0; e < n; ++e) {
-        // exp(-2 pi i e / n) in long double, rounded once
-        const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)e / (long double)n;
+    const long double two_pi = 2.0L * 3.141592653589793238462643383279502884L;
+    const int m = n - 1;
+    ax.rader = (n >= 17 && is_prime(n) && (m & (m - 1)) == 0) ? 1 : 0;
+    ax.rperm = nullptr;
+    ax.riperm = nullptr;
+    ax.rb = nullptr;
+    const int tn = ax.rader ? m : n;   // length of the radix-pass transform
+    factorize(tn, ax);
+    ax.n = n;
+    std::vector<cplx<R>> tw(n, cplx<R>{0, 0});
+    for (int e = 0; e < tn; ++e) {
+        // exp(-2 pi i e / tn) in long double, rounded once
+        const long double a = -two_pi * (long double)e / (long double)tn;
         tw[e] = {(R)std::cos(a), (R)std::sin(a)};
     }
     void* d = nullptr;
     PUFF_TRY(dalloc(c, &d, sizeof(cplx<R>) * n));
     PUFF_HIP(hipMemcpy(d, tw.data(), sizeof(cplx<R>) * n, hipMemcpyHostToDevice));
     ax.tw = d;
+    if (ax.rader) {
+        // primitive root: g^((n-1)/2) = -1 (mod n), n - 1 being a power of two
+        int g = 2;
+        while (powmod(g, m / 2, n) != n - 1) ++g;
+        const int ginv = powmod(g, n - 2, n);
+        std::vector<int> perm(m), iperm(m);
+        for (int j = 0, gj = 1, gi = 1; j < m; ++j) {
+            perm[j] = gj;
+            iperm[j] = gi;
+            gj = (int)((int64_t)gj * g % n);
+            gi = (int)((int64_t)gi * ginv % n);
+        }
+        // rb_s = DFT_m(b_s) / m, b_s[j] = exp(s 2 pi i iperm[j] / n)
+        std::vector<cplx<R>> rb(2 * (size_t)m);
+        for (int si = 0; si < 2; ++si) {
+            const long double sg = si ? 1.0L : -1.0L;
+            for (int k = 0; k < m; ++k) {
+                long double re = 0.0L, im = 0.0L;
+                for (int j = 0; j < m; ++j) {
+                    const long double ab = sg * two_pi * (long double)iperm[j] / (long double)n;
+                    const long double ak = -two_pi * (long double)(((int64_t)j * k) % m) / (long double)m;
+                    re += std::cos(ab + ak);
+                    im += std::sin(ab + ak);
+                }
+                rb[(size_t)si * m + k] = {(R)(re / m), (R)(im / m)};
+            }
+        }
+        void* dp = nullptr;
+        void* dq = nullptr;
+        void* db = nullptr;
+        PUFF_TRY(dalloc(c, &dp, sizeof(int) * m));
+        PUFF_TRY(dalloc(c, &dq, sizeof(int) * m));
+        PUFF_TRY(dalloc(c, &db, sizeof(cplx<R>) * 2 * m));
+        PUFF_HIP(hipMemcpy(dp, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+        PUFF_HIP(hipMemcpy(dq, iperm.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+        PUFF_HIP(hipMemcpy(db, rb.data(), sizeof(cplx<R>) * 2 * m, hipMemcpyHostToDevice));
+        ax.rperm = static_cast<const int*>(dp);
+        ax.riperm = static_cast<const int*>(dq);
+        ax.rb = db;
+    }
     return PUFF_OK;
 }
 
@@ -435,7 +509,12 @@ static int alloc_particles(Ctx* c) {
     PUFF_TRY(dalloc(c, (void**)&c->orig_spare, sizeof(int) * c->n));
     PUFF_TRY(dalloc(c, (void**)&c->sort_key, sizeof(int) * c->n));
     PUFF_TRY(dalloc(c, (void**)&c->sort_perm, sizeof(int) * c->n));
-    c->ncell = c->p.nx * c->p.ny * c->nzp();
+    const int64_t bins = sort_bins(c->p.nx, c->p.ny, c->nzp(), c->p.one_d != 0);
+    if (bins >= (int64_t)INT32_MAX) {
+        set_error("mesh too large for 32-bit sort bins");
+        return PUFF_ERR_ARG;
+    }
+    c->ncell = (int)bins;
     PUFF_TRY(dalloc(c, (void**)&c->sort_counts, sizeof(int) * ((size_t)c->ncell + 1)));
     PUFF_TRY(dalloc(c, (void**)&c->sort_tiles, sizeof(int) * ((size_t)c->ncell / 4096 + 2)));
     launch_iota(c->orig, c->n, nullptr);

@@ -33,15 +33,15 @@ __device__ __forceinline__ cplx<R> tw_get(const cplx<R>* tw, int e, int sign) {
     return w;
 }
 
-// In-LDS transform of B lines of length N.  src holds the input; returns the
-// buffer holding the output (src or dst).
+// In-LDS mixed-radix transform of length N over elements 0..N-1 of B lines
+// laid out with line length LD (N <= LD; elements N..LD-1 are left alone).
+// src holds the input; returns the buffer holding the output (src or dst).
 template <typename R, bool IL>
-__device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
-                            const cplx<R>* tw, int sign) {
-    const int N = ax.n;
+__device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N, int npass,
+                               const int* radix, const cplx<R>* tw, int sign) {
     int p = 1;
-    for (int ps = 0; ps < ax.npass; ++ps) {
-        const int r = ax.radix[ps];
+    for (int ps = 0; ps < npass; ++ps) {
+        const int r = radix[ps];
         const int T = N / r;
         const int twst = N / (p * r);
         if (r == 2 || r == 3 || r == 4) {
@@ -52,30 +52,30 @@ __device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax
                 const int ob = (i / p) * p * r + k;
                 cplx<R> v[4];
                 for (int j = 0; j < r; ++j) {
-                    v[j] = src[lidx<IL>(b, i + j * T, B, N)];
+                    v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
                     if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, (j * k * twst) % N, sign));
                 }
                 if (r == 2) {
-                    dst[lidx<IL>(b, ob, B, N)] = cadd(v[0], v[1]);
-                    dst[lidx<IL>(b, ob + p, B, N)] = csub(v[0], v[1]);
+                    dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], v[1]);
+                    dst[lidx<IL>(b, ob + p, B, LD)] = csub(v[0], v[1]);
                 } else if (r == 3) {
                     const R s3 = (R)0.86602540378443864676 * (R)(sign < 0 ? -1 : 1);
                     const cplx<R> t1 = cadd(v[1], v[2]);
                     const cplx<R> t2 = {v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y};
                     const cplx<R> d = csub(v[1], v[2]);
                     const cplx<R> t3 = {-s3 * d.y, s3 * d.x};   // i s s3 (x1 - x2)
-                    dst[lidx<IL>(b, ob, B, N)] = cadd(v[0], t1);
-                    dst[lidx<IL>(b, ob + p, B, N)] = cadd(t2, t3);
-                    dst[lidx<IL>(b, ob + 2 * p, B, N)] = csub(t2, t3);
+                    dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], t1);
+                    dst[lidx<IL>(b, ob + p, B, LD)] = cadd(t2, t3);
+                    dst[lidx<IL>(b, ob + 2 * p, B, LD)] = csub(t2, t3);
                 } else {
                     const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
                     const cplx<R> b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
                     // W4 b1: sign -1 -> -i b1, sign +1 -> +i b1
                     const cplx<R> wb = (sign < 0) ? cplx<R>{b1.y, -b1.x} : cplx<R>{-b1.y, b1.x};
-                    dst[lidx<IL>(b, ob, B, N)] = cadd(a0, b0);
-                    dst[lidx<IL>(b, ob + p, B, N)] = cadd(a1, wb);
-                    dst[lidx<IL>(b, ob + 2 * p, B, N)] = csub(a0, b0);
-                    dst[lidx<IL>(b, ob + 3 * p, B, N)] = csub(a1, wb);
+                    dst[lidx<IL>(b, ob, B, LD)] = cadd(a0, b0);
+                    dst[lidx<IL>(b, ob + p, B, LD)] = cadd(a1, wb);
+                    dst[lidx<IL>(b, ob + 2 * p, B, LD)] = csub(a0, b0);
+                    dst[lidx<IL>(b, ob + 3 * p, B, LD)] = csub(a1, wb);
                 }
             }
         } else {
@@ -95,7 +95,7 @@ __device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax
                 int e = 0, f = 0;
                 R s0r = 0, s0i = 0, s1r = 0, s1i = 0;
                 for (int j = 0; j < r; ++j) {
-                    cplx<R> x = src[lidx<IL>(b, i + j * T, B, N)];
+                    cplx<R> x = src[lidx<IL>(b, i + j * T, B, LD)];
                     if (e) x = cmul(x, tw_get(tw, e, sign));
                     const cplx<R> om = tw_get(tw, f, sign);
                     const R ac = x.x * om.x, bs = x.y * om.y, as = x.x * om.y, bc = x.y * om.x;
@@ -109,8 +109,8 @@ __device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax
                     if (f >= N) f -= N;
                 }
                 const int ob = (i / p) * p * r + k;
-                dst[lidx<IL>(b, ob + q * p, B, N)] = {s0r, s0i};
-                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, N)] = {s1r, s1i};
+                dst[lidx<IL>(b, ob + q * p, B, LD)] = {s0r, s0i};
+                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, LD)] = {s1r, s1i};
             }
         }
         __syncthreads();
@@ -120,6 +120,69 @@ __device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax
         p *= r;
     }
     return src;
+}
+
+// Rader's algorithm for a prime N with N - 1 = M a power of two (FftAxis):
+// a_j = x[g^j], A = DFT_M(a), C = A . rb_s, c = IDFT_M(C) (unnormalised; rb
+// carries the 1/M), X[g^-q] = x_0 + c_q, X_0 = x_0 + A_0.  Element M of each
+// line buffer is outside the M-point transforms: it keeps x_0 in dst and
+// X_0 in src until the final permutation.
+template <typename R, bool IL>
+__device__ cplx<R>* lds_rader(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
+                              const cplx<R>* tw, int sign) {
+    const int N = ax.n, M = N - 1;
+    const cplx<R>* rb = reinterpret_cast<const cplx<R>*>(ax.rb) + (sign > 0 ? M : 0);
+    cplx<R>* P = dst;   // element M: x_0
+    cplx<R>* Q = src;   // element M: X_0
+    for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+        int b, j;
+        if (IL) { b = w % B; j = w / B; } else { j = w % N; b = w / N; }
+        P[lidx<IL>(b, j, B, N)] = src[lidx<IL>(b, j < M ? ax.rperm[j] : 0, B, N)];
+    }
+    __syncthreads();
+    cplx<R>* res = lds_passes<R, IL>(P, Q, B, N, M, ax.npass, ax.radix, tw, -1);
+    for (int w = threadIdx.x; w < B * M; w += blockDim.x) {
+        int b, k;
+        if (IL) { b = w % B; k = w / B; } else { k = w % M; b = w / M; }
+        const cplx<R> v = res[lidx<IL>(b, k, B, N)];
+        if (k == 0) {
+            const cplx<R> x0 = P[lidx<IL>(b, M, B, N)];
+            Q[lidx<IL>(b, M, B, N)] = cadd(x0, v);
+        }
+        res[lidx<IL>(b, k, B, N)] = cmul(v, rb[k]);
+    }
+    __syncthreads();
+    cplx<R>* other = (res == P) ? Q : P;
+    res = lds_passes<R, IL>(res, other, B, N, M, ax.npass, ax.radix, tw, +1);
+    cplx<R>* out = (res == P) ? Q : P;
+    // every output element except element M (which holds x_0 or X_0 in out)
+    const int qm = M / 2;   // g^-(M/2) = -1 = M (mod N)
+    for (int w = threadIdx.x; w < B * M; w += blockDim.x) {
+        int b, q;
+        if (IL) { b = w % B; q = w / B; } else { q = w % M; b = w / M; }
+        const cplx<R> x0 = P[lidx<IL>(b, M, B, N)];
+        if (q == qm) {
+            out[lidx<IL>(b, 0, B, N)] = Q[lidx<IL>(b, M, B, N)];
+        } else {
+            out[lidx<IL>(b, ax.riperm[q], B, N)] = cadd(x0, res[lidx<IL>(b, q, B, N)]);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        const cplx<R> x0 = P[lidx<IL>(b, M, B, N)];
+        out[lidx<IL>(b, M, B, N)] = cadd(x0, res[lidx<IL>(b, qm, B, N)]);
+    }
+    __syncthreads();
+    return out;
+}
+
+// In-LDS transform of B lines of length ax.n.  src holds the input; returns
+// the buffer holding the output (src or dst).
+template <typename R, bool IL>
+__device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
+                            const cplx<R>* tw, int sign) {
+    if (ax.rader) return lds_rader<R, IL>(src, dst, B, ax, tw, sign);
+    return lds_passes<R, IL>(src, dst, B, ax.n, ax.n, ax.npass, ax.radix, tw, sign);
 }
 
 // ---- x lines (contiguous) --------------------------------------------------
@@ -255,8 +318,13 @@ __global__ __launch_bounds__(256) void k_fft_z(DiffArgs<R> a, int B, int op, int
     const int N = a.nzp;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* s1 = s0 + B * N;
-    const int x0 = blockIdx.x * B;
-    const int iy = blockIdx.y;
+    // blocks of B x-columns, x fastest; neighbouring blocks share the cache
+    // lines of every z2 plane, so they go to one XCD
+    const int nxb = (a.nx + B - 1) / B;
+    const int64_t blk = xcd_block((int64_t)nxb * a.ny);
+    if (blk >= (int64_t)nxb * a.ny) return;
+    const int x0 = (int)(blk % nxb) * B;
+    const int iy = (int)(blk / nxb);
     const int nb = min(B, a.nx - x0);
     const int64_t ntr = (int64_t)a.nx * a.ny;
     // few lines per workgroup here: the twiddles stay in global memory (L2)
@@ -362,8 +430,8 @@ static void run_z(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
     B = B < 1 ? 1 : (B > 16 ? 16 : B);
     const size_t sh = 2 * (size_t)B * a.nzp * sizeof(cplx<R>);
     lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R>), sh);
-    hipLaunchKernelGGL((k_fft_z<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.ny), dim3(256), sh, s,
-                       a, B, op, sign);
+    const int64_t blocks = xcd_grid((int64_t)((a.nx + B - 1) / B) * a.ny);
+    hipLaunchKernelGGL((k_fft_z<R>), dim3((unsigned)blocks), dim3(256), sh, s, a, B, op, sign);
 }
 
 template <typename R>

@@ -141,10 +141,44 @@ void launch_fold_alias(cplx<R>* LA, int64_t ntr, int nz2, hipStream_t s);
 void launch_fill_zero_words(uint32_t* p, int64_t n, hipStream_t s);
 
 // ---- particle sort by cell --------------------------------------------------
+// Sort order of the macroparticle slots: cells are grouped in bricks of
+// BRICK_X x BRICK_Y x BRICK_Z cells, bricks in planes along z2 with snaking
+// rows; the cells of a brick run in snaking rows along x (BRICK_XFAST) or in
+// snaking columns along z2.  A run of slots (one stage workgroup) then
+// covers a compact 3-D block of cells whose node box grows slowly as the
+// beam slips along z2 between sorts, and lanes of a wavefront share x-rows of
+// nodes.  One bin per cell of the padded brick grid, plus one for particles
+// off the mesh.  8 x 2 x 4 rows along x measured best on the C1 beam: 2-5 %
+// faster steps than plane/row order, with 2-6x fewer re-sorts once bunched.
+#ifndef PUFF_BRICK_X
+#define PUFF_BRICK_X 8
+#endif
+#ifndef PUFF_BRICK_Y
+#define PUFF_BRICK_Y 2
+#endif
+#ifndef PUFF_BRICK_Z
+#define PUFF_BRICK_Z 4
+#endif
+#ifndef PUFF_BRICK_XFAST
+#define PUFF_BRICK_XFAST 1
+#endif
+constexpr int BRICK_X = PUFF_BRICK_X, BRICK_Y = PUFF_BRICK_Y, BRICK_Z = PUFF_BRICK_Z;
+constexpr bool BRICK_XFAST = PUFF_BRICK_XFAST != 0;
+constexpr int BRICK_CELLS = BRICK_X * BRICK_Y * BRICK_Z;
+
+// Number of sort bins for particles on the mesh (the last bin, for particles
+// off it, is this value).
+__host__ __device__ inline int64_t sort_bins(int nx, int ny, int nzp, bool one_d) {
+    if (one_d) return nzp;
+    const int64_t bx = (nx - 1 + BRICK_X - 1) / BRICK_X, by = (ny - 1 + BRICK_Y - 1) / BRICK_Y;
+    const int64_t bz = (nzp + BRICK_Z - 1) / BRICK_Z;
+    return bx * by * bz * BRICK_CELLS;
+}
+
 template <typename R>
 struct SortArgs {
     int64_t n;
-    int ncell;             // valid cells; bin ncell collects particles off the mesh
+    int ncell;             // bins on the mesh (sort_bins); bin ncell collects particles off it
     MeshGeom mesh;
     PartPtrs<R>* y;        // sorted in place (pointers swapped with spare)
     PartPtrs<R>* spare;

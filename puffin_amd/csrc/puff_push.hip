@@ -86,18 +86,47 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     int t0, key;
     cell_weights<R>(m, ok, c, tb, lane, w, nd, t0, key);
 
-    // ---- gather F = sum_k w_k (A0 + c dA_{s-1})[node_k]  (getFFelecs)
+    // ---- gather F = sum_k w_k (A0 + c dA_{s-1})[node_k]  (getFFelecs).
+    // When the workgroup's node box is small (the dense part of the beam),
+    // A0 + c dA_{s-1} is formed once per node of the box in LDS and the
+    // corners are read from there; else from global memory.  Same arithmetic
+    // either way.
+    const R cA = (R)a.cin;
+    const bool lds_field = use_tile && tb.tx * tb.ty * tb.tz <= FIELD_TILE_NODES;
+    __shared__ cplx<R> s_fld[FIELD_TILE_NODES];
+    if (lds_field) {
+        const int nt = tb.tx * tb.ty * tb.tz;
+        const int ntr = m.nx * m.ny;
+        for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+            const int lx = q % tb.tx, r = q / tb.tx;
+            const int ly = r % tb.ty, lz = r / tb.ty;
+            const int node = (tb.bz0 + lz) * ntr + (tb.by0 + ly) * m.nx + (tb.bx0 + lx);
+            cplx<R> v = a.A0[node];
+            if (STAGE > 1) {
+                const cplx<R> d = a.dAin[node];
+                v.x = fma(cA, d.x, v.x);
+                v.y = fma(cA, d.y, v.y);
+            }
+            s_fld[q] = v;
+        }
+        __syncthreads();
+    }
     R fr = 0, fi = 0;
     if (ok) {
-        const R cA = (R)a.cin;
+        const int txy = tb.tx * tb.ty;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k < nnb) {
-                cplx<R> v = a.A0[nd[k]];
-                if (STAGE > 1) {
-                    const cplx<R> d = a.dAin[nd[k]];
-                    v.x = fma(cA, d.x, v.x);
-                    v.y = fma(cA, d.y, v.y);
+                cplx<R> v;
+                if (lds_field) {
+                    v = s_fld[m.one_d ? t0 + k : t0 + (k & 1) + ((k & 2) ? tb.tx : 0) + ((k & 4) ? txy : 0)];
+                } else {
+                    v = a.A0[nd[k]];
+                    if (STAGE > 1) {
+                        const cplx<R> d = a.dAin[nd[k]];
+                        v.x = fma(cA, d.x, v.x);
+                        v.y = fma(cA, d.y, v.y);
+                    }
                 }
                 fr = fma(w[k], v.x, fr);
                 fi = fma(w[k], v.y, fi);

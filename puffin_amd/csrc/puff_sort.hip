@@ -7,8 +7,8 @@
 
 namespace puff {
 
-// Bin of each particle's cell in the slot order (planes, then rows, then
-// cells along x); particles outside the mesh go to the last bin.
+// Bin of each particle's cell in the slot order (bricks of cells, see
+// sort_bins); particles outside the mesh go to the last bin.
 template <typename R>
 __global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const double* z2,
                                                    int64_t n, MeshGeom m, int ncell, int* key) {
@@ -33,11 +33,32 @@ __global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const
             const R xq = floor((x[i] + (R)m.halfx) * (R)m.inv_dx);
             const R yq = floor((y[i] + (R)m.halfy) * (R)m.inv_dy);
             if (xq >= (R)0 && xq <= (R)(m.nx - 2) && yq >= (R)0 && yq <= (R)(m.ny - 2)) {
-                // rows run forward on even planes and backward on odd ones, so
-                // that a run of slots crossing into the next plane stays within
-                // a few rows (a compact deposit tile)
-                const int iy = (iz & 1) ? (m.ny - 2 - (int)yq) : (int)yq;
-                k = iz * m.nx * m.ny + iy * m.nx + (int)xq;
+                const int ix = (int)xq, iy = (int)yq;
+                const int nbx = (m.nx - 1 + BRICK_X - 1) / BRICK_X;
+                const int nby = (m.ny - 1 + BRICK_Y - 1) / BRICK_Y;
+                // bricks: planes along z2, rows of bricks snaking, so that
+                // consecutive bricks are neighbours
+                const int bz = iz / BRICK_Z;
+                int by = iy / BRICK_Y, bx = ix / BRICK_X;
+                if (bz & 1) by = nby - 1 - by;
+                const int brow = bz * nby + by;
+                if (brow & 1) bx = nbx - 1 - bx;
+                int lx = ix % BRICK_X, ly = iy % BRICK_Y, lz = iz % BRICK_Z;
+                int cell;
+                if (BRICK_XFAST) {
+                    // cells of a brick: rows along x in planes, snaking
+                    if (lz & 1) ly = BRICK_Y - 1 - ly;
+                    const int row = lz * BRICK_Y + ly;
+                    if (row & 1) lx = BRICK_X - 1 - lx;
+                    cell = row * BRICK_X + lx;
+                } else {
+                    // cells of a brick: columns along z2, snaking
+                    if (ly & 1) lx = BRICK_X - 1 - lx;
+                    const int col = ly * BRICK_X + lx;
+                    if (col & 1) lz = BRICK_Z - 1 - lz;
+                    cell = col * BRICK_Z + lz;
+                }
+                k = (int)(((int64_t)brow * nbx + bx) * BRICK_CELLS + cell);
             }
         }
     }
@@ -182,10 +203,22 @@ __global__ __launch_bounds__(256) void k_scatter_perm(const int* key, int64_t n,
     if (valid) perm[base + (lane - r.head_lane)] = (int)i;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_gather(const T* src, const int* perm, int64_t n, T* dst) {
+// Every per-particle array into slot order: dst[j] = src[perm[j]].
+template <typename R>
+__global__ __launch_bounds__(256) void k_gather_all(PartPtrs<R> y, const R* chi, const int* orig,
+                                                    const int* perm, int64_t n, PartPtrs<R> t,
+                                                    R* chi_out, int* orig_out) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) dst[j] = src[perm[j]];
+    if (j >= n) return;
+    const int p = perm[j];
+    t.x[j] = y.x[p];
+    t.y[j] = y.y[p];
+    t.z2[j] = y.z2[p];
+    t.pr[j] = y.pr[p];
+    t.pi[j] = y.pi[p];
+    t.g[j] = y.g[p];
+    chi_out[j] = chi[p];
+    orig_out[j] = orig[p];
 }
 
 // dst[orig[j]] = (double) src[j]: slot order back to the reference order.
@@ -221,14 +254,8 @@ void launch_sort(SortArgs<R>& a, hipStream_t s) {
     // permute every coordinate into the spare arrays, then swap the pointers
     PartPtrs<R>& y = *a.y;
     PartPtrs<R>& t = *a.spare;
-    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.x, a.perm, n, t.x);
-    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.y, a.perm, n, t.y);
-    hipLaunchKernelGGL((k_gather<double>), g1(n), dim3(256), 0, s, y.z2, a.perm, n, t.z2);
-    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.pr, a.perm, n, t.pr);
-    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, y.pi, a.perm, n, t.pi);
-    hipLaunchKernelGGL((k_gather<double>), g1(n), dim3(256), 0, s, y.g, a.perm, n, t.g);
-    hipLaunchKernelGGL((k_gather<R>), g1(n), dim3(256), 0, s, *a.chi, a.perm, n, *a.chi_spare);
-    hipLaunchKernelGGL((k_gather<int>), g1(n), dim3(256), 0, s, *a.orig, a.perm, n, *a.orig_spare);
+    hipLaunchKernelGGL((k_gather_all<R>), g1(n), dim3(256), 0, s, y, *a.chi, *a.orig, a.perm, n, t,
+                       *a.chi_spare, *a.orig_spare);
     std::swap(y.x, t.x);
     std::swap(y.y, t.y);
     std::swap(y.z2, t.z2);

@@ -73,7 +73,8 @@ def test_f1main_golden_vectors_on_gpu():
 # FFT (Transform_MultiD) against numpy
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("shape", [(9, 7, 8), (33, 33, 128), (129, 5, 16), (5, 6, 257), (11, 4, 43)])
+@pytest.mark.parametrize("shape", [(9, 7, 8), (33, 33, 128), (129, 5, 16), (5, 6, 257), (11, 4, 43),
+                                   (17, 17, 17), (257, 3, 4), (3, 257, 4), (85, 7, 6)])
 @pytest.mark.parametrize("precision", [FP64, MIXED])
 def test_transform_matches_numpy(shape, precision):
     nx, ny, nz = shape
