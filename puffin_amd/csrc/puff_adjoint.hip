@@ -28,6 +28,32 @@
 
 namespace puff {
 
+// Trial field of the stage at one node: A0 + c dA_{s-1} (as the forward
+// stage forms it).
+template <typename R, int STAGE>
+__device__ __forceinline__ cplx<R> adj_trial_field(const AdjArgs<R>& a, int node, R cA) {
+    cplx<R> v = a.A0[node];
+    if (STAGE > 1) {
+        const cplx<R> d = a.dAin[node];
+        v.x = fma(cA, d.x, v.x);
+        v.y = fma(cA, d.y, v.y);
+    }
+    return v;
+}
+
+// Cotangent of the stage's field source at one node: w_s LA + c_{s+1} mA_{s+1}.
+template <typename R, int STAGE>
+__device__ __forceinline__ double2 adj_source_cotangent(const AdjArgs<R>& a, int node) {
+    const cplx<R> l = a.LA[node];
+    double gr = a.ws * (double)l.x, gi = a.ws * (double)l.y;
+    if (STAGE < 4) {
+        const cplx<R> t = a.mAnext[node];
+        gr += a.cnext * (double)t.x;
+        gi += a.cnext * (double)t.y;
+    }
+    return double2{gr, gi};
+}
+
 template <typename R, int STAGE>
 __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     __shared__ int s_skip;
@@ -107,26 +133,12 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     int nd[8];
     int t0, key;
     cell_weights<R>(m, ok, c, tb, lane, w, nd, t0, key);
-    double dwx[8], dwy[8], dwz[8];
-    if (m.one_d) {
-        dwx[0] = dwx[1] = dwy[0] = dwy[1] = 0.0;
-        dwz[0] = -m.inv_dz2;
-        dwz[1] = m.inv_dz2;
-    } else {
-        const double X[2] = {(double)((R)1 - c.fx), (double)c.fx};
-        const double Y[2] = {(double)((R)1 - c.fy), (double)c.fy};
-        const double Z[2] = {(double)((R)1 - c.fz), (double)c.fz};
-        const double dX[2] = {-m.inv_dx, m.inv_dx};
-        const double dY[2] = {-m.inv_dy, m.inv_dy};
-        const double dZ[2] = {-m.inv_dz2, m.inv_dz2};
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int ia = k & 1, ib = (k >> 1) & 1, ic = k >> 2;
-            dwx[k] = dX[ia] * Y[ib] * Z[ic];
-            dwy[k] = X[ia] * dY[ib] * Z[ic];
-            dwz[k] = X[ia] * Y[ib] * dZ[ic];
-        }
-    }
+    // derivatives of the weights inside the cell, formed per corner in the
+    // gather below: dw_k/dx = dX[ia] Y[ib] Z[ic] and likewise for y, z2
+    // (1-D: dw/dz2 = -+1/dz2)
+    const double X[2] = {(double)((R)1 - c.fx), (double)c.fx};
+    const double Y[2] = {(double)((R)1 - c.fy), (double)c.fy};
+    const double Z[2] = {(double)((R)1 - c.fz), (double)c.fz};
 
     // ---- undulator fields and their transverse derivatives
     R bxr = 0, byr = 0, bzr = 0;
@@ -144,25 +156,45 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     const double dep = ph.field_evolve ? (double)chi * ph.inv_dV3 : 0.0;
     const double depfac = dep * fac;
 
-    // ---- gather of the trial field and of the source cotangent
+    // ---- gather of the trial field and of the source cotangent.  When the
+    // workgroup's node box is small (the dense part of the beam), both are
+    // formed once per node of the box in LDS and the corners read from there
+    // (same arithmetic as the global reads).
+    const R cA = (R)a.cin;
+    const bool lds_field = tile_fits<ADJ_TILE_NODES>(m, tb) && tb.tx * tb.ty * tb.tz <= ADJ_FIELD_TILE_NODES;
+    __shared__ cplx<R> s_v[ADJ_FIELD_TILE_NODES];
+    __shared__ double2 s_g[ADJ_FIELD_TILE_NODES];
+    if (lds_field) {
+        static_assert(ADJ_FIELD_TILE_NODES <= ADJ_THREADS, "one field tile node per thread");
+        const int q = threadIdx.x;
+        if (q < tb.tx * tb.ty * tb.tz) {
+            const int lx = q % tb.tx, r = q / tb.tx;
+            const int ly = r % tb.ty, lz = r / tb.ty;
+            const int node = (tb.bz0 + lz) * m.nx * m.ny + (tb.by0 + ly) * m.nx + (tb.bx0 + lx);
+            s_v[q] = adj_trial_field<R, STAGE>(a, node, cA);
+            s_g[q] = adj_source_cotangent<R, STAGE>(a, node);
+        }
+        __syncthreads();
+    }
     double fr = 0, fi = 0, Gr = 0, Gi = 0, sx = 0, sy = 0, sz = 0;
     if (ok) {
-        const R cA = (R)a.cin;
+        const int txy = tb.tx * tb.ty;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k < nnb) {
-                cplx<R> v = a.A0[nd[k]];
-                if (STAGE > 1) {
-                    const cplx<R> d = a.dAin[nd[k]];
-                    v.x = fma(cA, d.x, v.x);
-                    v.y = fma(cA, d.y, v.y);
-                }
-                const cplx<R> l = a.LA[nd[k]];
-                double gr = a.ws * (double)l.x, gi = a.ws * (double)l.y;
-                if (STAGE < 4) {
-                    const cplx<R> t = a.mAnext[nd[k]];
-                    gr += a.cnext * (double)t.x;
-                    gi += a.cnext * (double)t.y;
+                cplx<R> v;
+                double gr, gi;
+                if (lds_field) {
+                    const int tk = m.one_d ? t0 + k : t0 + (k & 1) + ((k & 2) ? tb.tx : 0) + ((k & 4) ? txy : 0);
+                    v = s_v[tk];
+                    const double2 gg = s_g[tk];
+                    gr = gg.x;
+                    gi = gg.y;
+                } else {
+                    v = adj_trial_field<R, STAGE>(a, nd[k], cA);
+                    const double2 gg = adj_source_cotangent<R, STAGE>(a, nd[k]);
+                    gr = gg.x;
+                    gi = gg.y;
                 }
                 const double wk = (double)w[k];
                 fr += wk * (double)v.x;
@@ -170,9 +202,17 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
                 Gr += wk * gr;
                 Gi += wk * gi;
                 const double V = Lfr * (double)v.x + Lfi * (double)v.y + depfac * (prd * gr + pid * gi);
-                sx += dwx[k] * V;
-                sy += dwy[k] * V;
-                sz += dwz[k] * V;
+                const int ia = k & 1, ib = (k >> 1) & 1, ic = k >> 2;
+                if (m.one_d) {
+                    sz += (ia ? m.inv_dz2 : -m.inv_dz2) * V;
+                } else {
+                    const double dX = ia ? m.inv_dx : -m.inv_dx;
+                    const double dY = ib ? m.inv_dy : -m.inv_dy;
+                    const double dZ = ic ? m.inv_dz2 : -m.inv_dz2;
+                    sx += dX * Y[ib] * Z[ic] * V;
+                    sy += X[ia] * dY * Z[ic] * V;
+                    sz += X[ia] * Y[ib] * dZ * V;
+                }
             }
         }
     }

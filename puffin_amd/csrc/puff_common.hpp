@@ -79,6 +79,7 @@ constexpr int FIELD_TILE_NODES = PUFF_FIELD_TILE_NODES;
 // particle than the forward, so smaller workgroups with a larger VGPR budget.
 constexpr int ADJ_THREADS = 256;
 constexpr int ADJ_TILE_NODES = 1024;
+constexpr int ADJ_FIELD_TILE_NODES = 256;   // LDS copy of the trial field and source cotangent
 constexpr int MAX_BOX_WAVES = (STAGE_THREADS > ADJ_THREADS ? STAGE_THREADS : ADJ_THREADS) / 64;
 
 // Flag words of the step kernels: [0] flag bits of the current step;
