@@ -129,10 +129,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     const bool scat = ph.coupling && ph.electrons_evolve;
     const bool use_tile = scat && tile_fits<ADJ_TILE_NODES>(m, tb);
     const int nnb = m.one_d ? 2 : 8;
-    R w[8];
-    int nd[8];
-    int t0, key;
-    cell_weights<R>(m, ok, c, tb, lane, w, nd, t0, key);
+    const CellRef cr = cell_ref<R>(m, ok, c, tb, lane);
     // derivatives of the weights inside the cell, formed per corner in the
     // gather below: dw_k/dx = dX[ia] Y[ib] Z[ic] and likewise for y, z2
     // (1-D: dw/dz2 = -+1/dz2)
@@ -142,11 +139,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
 
     // ---- undulator fields and their transverse derivatives
     R bxr = 0, byr = 0, bzr = 0;
-    double bg[6] = {0, 0, 0, 0, 0, 0};
-    if (ok) {
-        bfield<R>(a.s, x, y, bxr, byr, bzr);
-        bfield_grad(a.s, (double)x, (double)y, bg);
-    }
+    if (ok) bfield<R>(a.s, x, y, bxr, byr, bzr);
     const double bx = bxr, by = byr, bz = bzr;
     const double n2col = a.s.n2col;
     const double ep2k = ph.eta * p2 / (ph.kappa * ph.kappa);
@@ -178,25 +171,25 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     }
     double fr = 0, fi = 0, Gr = 0, Gi = 0, sx = 0, sy = 0, sz = 0;
     if (ok) {
-        const int txy = tb.tx * tb.ty;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k < nnb) {
                 cplx<R> v;
                 double gr, gi;
                 if (lds_field) {
-                    const int tk = m.one_d ? t0 + k : t0 + (k & 1) + ((k & 2) ? tb.tx : 0) + ((k & 4) ? txy : 0);
+                    const int tk = corner_tile(m, tb, cr, k);
                     v = s_v[tk];
                     const double2 gg = s_g[tk];
                     gr = gg.x;
                     gi = gg.y;
                 } else {
-                    v = adj_trial_field<R, STAGE>(a, nd[k], cA);
-                    const double2 gg = adj_source_cotangent<R, STAGE>(a, nd[k]);
+                    const int node = corner_node(m, cr, k);
+                    v = adj_trial_field<R, STAGE>(a, node, cA);
+                    const double2 gg = adj_source_cotangent<R, STAGE>(a, node);
                     gr = gg.x;
                     gi = gg.y;
                 }
-                const double wk = (double)w[k];
+                const double wk = (double)corner_weight<R>(m, c, k);
                 fr += wk * (double)v.x;
                 fi += wk * (double)v.y;
                 Gr += wk * gr;
@@ -220,7 +213,7 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
 
     // ---- field cotangent of this stage: w_k (dL/dFr, dL/dFi) at the nodes
     if (scat) {
-        scatter_cells<R, ADJ_TILE_NODES>(m, tb, use_tile, ok, c, nnb, w, nd, t0, key, ok ? (R)Lfr : (R)0,
+        scatter_cells<R, ADJ_TILE_NODES>(m, tb, use_tile, ok, c, cr, ok ? (R)Lfr : (R)0,
                          ok ? (R)Lfi : (R)0, a.mAout);
     }
 
@@ -257,6 +250,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
         mpr = Epr + Lu * du_dpr + Lfac * dfac_dpr;
         mpi = Epi + Lu * du_dpi + Lfac * dfac_dpi;
         mg = Lu * du_dg + Lfac * dfac_dg;
+        double bg[6];   // transverse derivatives of the undulator fields
+        bfield_grad(a.s, (double)x, (double)y, bg);
         const double Lbx = api * ph.inv2rho * n2col, Lby = apr * ph.inv2rho * n2col;
         const double Lbz = ph.kappa * fac * n2col * (apr * pid - api * prd);
         mx = sx + Lbx * bg[0] + Lby * bg[2] + Lbz * bg[4];

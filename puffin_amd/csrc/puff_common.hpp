@@ -56,17 +56,25 @@ constexpr int BOX_COPIES = 64;
 
 // Workgroup of the RK4 stage kernels (one macroparticle per thread) and the
 // node capacity of its LDS deposit tile (2 FP64 per node).  Four-wave
-// workgroups: the FP64 stage kernels hold 83-99 VGPRs (4-5 waves per SIMD),
-// so one wave per SIMD per workgroup lets 4-5 workgroups share a CU, and a
-// 1536-node tile (24 KiB) keeps LDS from limiting that.
+// workgroups (one wave per SIMD each); the stage kernels hold 51-71 VGPRs,
+// so the LDS tiles set how many workgroups share a CU: 1280 nodes (20 KiB,
+// 6 per CU) in the FP64 form, 1536 (24 KiB) in the mixed form, whose
+// boxes overflowing a smaller tile cost more re-sorts than the occupancy
+// gains (measured on the C1 beam).
 #ifndef PUFF_STAGE_THREADS
 #define PUFF_STAGE_THREADS 256
 #endif
 constexpr int STAGE_THREADS = PUFF_STAGE_THREADS;
-#ifndef PUFF_TILE_NODES
-#define PUFF_TILE_NODES 1536
+#ifndef PUFF_TILE_NODES_FP64
+#define PUFF_TILE_NODES_FP64 1280
 #endif
-constexpr int TILE_NODES = PUFF_TILE_NODES;
+#ifndef PUFF_TILE_NODES_MIXED
+#define PUFF_TILE_NODES_MIXED 1536
+#endif
+template <typename R>
+constexpr int stage_tile_nodes() {
+    return sizeof(R) == 8 ? PUFF_TILE_NODES_FP64 : PUFF_TILE_NODES_MIXED;
+}
 // Node capacity of the stage kernels' LDS copy of the field over the
 // workgroup's node box (one complex per node; larger boxes gather from
 // global memory).

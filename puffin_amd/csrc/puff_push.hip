@@ -76,15 +76,12 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // ---- node box of this workgroup's cells and the step's active box
     const int lane = threadIdx.x & 63;
     const TileBox tb = workgroup_box<R>(m, ok, c, a.box);
-    const bool use_tile = ph.field_evolve && tile_fits(m, tb);
+    const bool use_tile = ph.field_evolve && tile_fits<stage_tile_nodes<R>()>(m, tb);
     if (threadIdx.x == 0 && ph.field_evolve && tb.any && !use_tile) atomicAdd(a.flags + FLAG_WORD_FALLBACK, 1);
 
-    // ---- nodes and tri-linear weights (zero weights on lanes without a cell)
+    // ---- nodes of the cell (weights are formed per corner where used)
     const int nnb = m.one_d ? 2 : 8;
-    R w[8];
-    int nd[8];
-    int t0, key;
-    cell_weights<R>(m, ok, c, tb, lane, w, nd, t0, key);
+    const CellRef cr = cell_ref<R>(m, ok, c, tb, lane);
 
     // ---- gather F = sum_k w_k (A0 + c dA_{s-1})[node_k]  (getFFelecs).
     // When the workgroup's node box is small (the dense part of the beam),
@@ -113,23 +110,24 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     }
     R fr = 0, fi = 0;
     if (ok) {
-        const int txy = tb.tx * tb.ty;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k < nnb) {
                 cplx<R> v;
                 if (lds_field) {
-                    v = s_fld[m.one_d ? t0 + k : t0 + (k & 1) + ((k & 2) ? tb.tx : 0) + ((k & 4) ? txy : 0)];
+                    v = s_fld[corner_tile(m, tb, cr, k)];
                 } else {
-                    v = a.A0[nd[k]];
+                    const int node = corner_node(m, cr, k);
+                    v = a.A0[node];
                     if (STAGE > 1) {
-                        const cplx<R> d = a.dAin[nd[k]];
+                        const cplx<R> d = a.dAin[node];
                         v.x = fma(cA, d.x, v.x);
                         v.y = fma(cA, d.y, v.y);
                     }
                 }
-                fr = fma(w[k], v.x, fr);
-                fi = fma(w[k], v.y, fi);
+                const R wk = corner_weight<R>(m, c, k);
+                fr = fma(wk, v.x, fr);
+                fi = fma(wk, v.y, fi);
             }
         }
     }
@@ -138,7 +136,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     if (ph.field_evolve) {
         const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
         const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
-        scatter_cells<R>(m, tb, use_tile, ok, c, nnb, w, nd, t0, key, sr, si, a.dAout);
+        scatter_cells<R, stage_tile_nodes<R>()>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout);
     }
     if (!in) return;
     if (!ph.coupling) { fr = 0; fi = 0; }   // rhs.f90:221-224

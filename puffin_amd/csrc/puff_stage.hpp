@@ -188,83 +188,89 @@ __device__ __forceinline__ Box merged_box(const Box* box) {
 
 // Whether the workgroup's node box fits the LDS tile and does not wrap
 // around a periodic mesh (block-uniform).
-template <int TILE = TILE_NODES>
+template <int TILE>
 __device__ __forceinline__ bool tile_fits(const MeshGeom& m, const TileBox& t) {
     return t.any && (int64_t)t.tx * t.ty * t.tz <= TILE && !(m.periodic && t.hz + 1 >= m.nzp);
 }
 
-// Tri-linear (or 1-D linear) weights, node indices and the key of a lane's
-// cell (unique negative key on lanes without a cell).
+// Nodes of a lane's cell: first node of its lower (b0) and upper (b1)
+// z2 plane, its first node in the workgroup's tile (t0), and its key
+// (unique negative key on lanes without a cell).
+struct CellRef {
+    int b0, b1, t0, key;
+};
+
 template <typename R>
-__device__ __forceinline__ void cell_weights(const MeshGeom& m, bool ok, const Cell<R>& c,
-                                             const TileBox& t, int lane, R* w, int* nd,
-                                             int& t0, int& key) {
+__device__ __forceinline__ CellRef cell_ref(const MeshGeom& m, bool ok, const Cell<R>& c,
+                                            const TileBox& t, int lane) {
+    CellRef r;
     if (m.one_d) {
-        // lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1)
-        w[0] = ok ? (R)1 - c.fz : (R)0;
-        w[1] = ok ? (R)1 - ((R)1 - c.fz) : (R)0;
-        nd[0] = ok ? c.iz0 : 0;
-        nd[1] = ok ? c.iz1 : 0;
-        t0 = ok ? c.iz0 - t.bz0 : 0;
-        key = ok ? c.iz0 : -1 - lane;
-        return;
+        r.b0 = ok ? c.iz0 : 0;
+        r.b1 = ok ? c.iz1 : 0;
+        r.t0 = ok ? c.iz0 - t.bz0 : 0;
+        r.key = ok ? c.iz0 : -1 - lane;
+        return r;
     }
-    const R x1 = (R)1 - c.fx, x2 = c.fx;
-    const R y1 = (R)1 - c.fy, y2 = c.fy;
-    const R z1 = (R)1 - c.fz, z2w = c.fz;
-    w[0] = x1 * y1 * z1;  w[1] = x2 * y1 * z1;  w[2] = x1 * y2 * z1;  w[3] = x2 * y2 * z1;
-    w[4] = x1 * y1 * z2w; w[5] = x2 * y1 * z2w; w[6] = x1 * y2 * z2w; w[7] = x2 * y2 * z2w;
     const int ntr = m.nx * m.ny;
-    const int b0 = ok ? c.iz0 * ntr + c.iy * m.nx + c.ix : 0;
-    const int b1 = ok ? c.iz1 * ntr + c.iy * m.nx + c.ix : 0;
-    nd[0] = b0; nd[1] = b0 + 1; nd[2] = b0 + m.nx; nd[3] = b0 + m.nx + 1;
-    nd[4] = b1; nd[5] = b1 + 1; nd[6] = b1 + m.nx; nd[7] = b1 + m.nx + 1;
-    t0 = ok ? ((c.iz0 - t.bz0) * t.ty + (c.iy - t.by0)) * t.tx + (c.ix - t.bx0) : 0;
-    key = ok ? b0 : -1 - lane;
-    if (!ok) {
-#pragma unroll
-        for (int k = 0; k < 8; ++k) w[k] = (R)0;
-    }
+    r.b0 = ok ? c.iz0 * ntr + c.iy * m.nx + c.ix : 0;
+    r.b1 = ok ? c.iz1 * ntr + c.iy * m.nx + c.ix : 0;
+    r.t0 = ok ? ((c.iz0 - t.bz0) * t.ty + (c.iy - t.by0)) * t.tx + (c.ix - t.bx0) : 0;
+    r.key = ok ? r.b0 : -1 - lane;
+    return r;
 }
 
-// out[node_k] += w_k (sr, si) for every lane with a cell.
+// Weight and node of corner k of a lane's cell, in getInterps_3D's order
+// (x fastest, then y, then z2): w = (1-fx | fx)(1-fy | fy)(1-fz | fz).
+// 1-D: lis_GR(1) = 1 - locz2/dz2, lis_GR(2) = 1 - lis_GR(1) at nodes iz0, iz1.
+template <typename R>
+__device__ __forceinline__ R corner_weight(const MeshGeom& m, const Cell<R>& c, int k) {
+    if (m.one_d) {
+        const R w0 = (R)1 - c.fz;
+        return k ? (R)1 - w0 : w0;
+    }
+    const R wx = (k & 1) ? c.fx : (R)1 - c.fx;
+    const R wy = (k & 2) ? c.fy : (R)1 - c.fy;
+    const R wz = (k & 4) ? c.fz : (R)1 - c.fz;
+    return wx * wy * wz;
+}
+
+__device__ __forceinline__ int corner_node(const MeshGeom& m, const CellRef& r, int k) {
+    if (m.one_d) return k ? r.b1 : r.b0;
+    return ((k & 4) ? r.b1 : r.b0) + (k & 1) + ((k & 2) ? m.nx : 0);
+}
+
+__device__ __forceinline__ int corner_tile(const MeshGeom& m, const TileBox& t, const CellRef& r, int k) {
+    if (m.one_d) return r.t0 + k;
+    return r.t0 + (k & 1) + ((k & 2) ? t.tx : 0) + ((k & 4) ? t.tx * t.ty : 0);
+}
+
+// out[node_k] += w_k (sr, si) for every lane with a cell (sr = si = 0 on
+// lanes without one).
 //   use_tile: FP64 accumulation in an LDS tile over the workgroup's node box
 //             (real parts, then imaginary parts), then one global atomic per
-//             touched node and component.  Each
-//             lane visits the corners in a rotated order so that lanes of one
-//             cell do not hit the same LDS address in the same instruction.
+//             touched node and component.  Each lane visits the corners in a
+//             rotated order so that lanes of one cell do not hit the same LDS
+//             address in the same instruction.
 //   else:     lanes of a wavefront that share a cell are summed first (runs
 //             of equal keys), then one global atomic per run and corner.
 // use_tile must be block-uniform; every thread of the workgroup must call.
-template <typename R, int TILE = TILE_NODES>
+template <typename R, int TILE>
 __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& t, bool use_tile,
-                                              bool ok, const Cell<R>& c, int nnb, const R* w,
-                                              const int* nd, int t0, int key, R sr, R si,
-                                              cplx<R>* out) {
+                                              bool ok, const Cell<R>& c, const CellRef& cr, R sr,
+                                              R si, cplx<R>* out) {
     const int lane = threadIdx.x & 63;
+    const int nnb = m.one_d ? 2 : 8;
     if (use_tile) {
         __shared__ double s_tile[2 * TILE];
         const int nt = t.tx * t.ty * t.tz;
         for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) s_tile[q] = 0.0;
         __syncthreads();
         if (ok) {
-            const int txy = t.tx * t.ty;
             const double dsr = (double)sr, dsi = (double)si;
-            const double wx[2] = {(double)((R)1 - c.fx), (double)c.fx};
-            const double wy[2] = {(double)((R)1 - c.fy), (double)c.fy};
-            const double wz[2] = {(double)((R)1 - c.fz), (double)c.fz};
             for (int q = 0; q < nnb; ++q) {
                 const int k = (q + lane) & (nnb - 1);
-                double wk;
-                int tk;
-                if (m.one_d) {
-                    wk = (double)w[0];
-                    if (k) wk = (double)w[1];
-                    tk = t0 + k;
-                } else {
-                    wk = (double)((R)wx[k & 1] * (R)wy[(k >> 1) & 1] * (R)wz[k >> 2]);
-                    tk = t0 + (k & 1) + ((k & 2) ? t.tx : 0) + ((k & 4) ? txy : 0);
-                }
+                const double wk = (double)corner_weight<R>(m, c, k);
+                const int tk = corner_tile(m, t, cr, k);
                 atomicAdd(&s_tile[tk], wk * dsr);
                 atomicAdd(&s_tile[nt + tk], wk * dsi);
             }
@@ -282,8 +288,8 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
             }
         }
     } else {
-        const int prev = __shfl_up(key, 1, 64);
-        const bool head = (lane == 0) || (prev != key);
+        const int prev = __shfl_up(cr.key, 1, 64);
+        const bool head = (lane == 0) || (prev != cr.key);
         const unsigned long long heads = __ballot(head);
         const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
         const int seg_end = after ? lane + __builtin_ctzll(after) : 63;
@@ -291,7 +297,8 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k < nnb) {
-                R vr = w[k] * sr, vi = w[k] * si;
+                const R wk = ok ? corner_weight<R>(m, c, k) : (R)0;
+                R vr = wk * sr, vi = wk * si;
                 for (int o = 1; o < maxlen; o <<= 1) {
                     const R tr = __shfl_down(vr, o, 64);
                     const R ti = __shfl_down(vi, o, 64);
@@ -301,8 +308,9 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
                     }
                 }
                 if (ok && head) {
-                    atomic_add(&out[nd[k]].x, vr);
-                    atomic_add(&out[nd[k]].y, vi);
+                    const int node = corner_node(m, cr, k);
+                    atomic_add(&out[node].x, vr);
+                    atomic_add(&out[node].y, vi);
                 }
             }
         }
