@@ -21,12 +21,13 @@ namespace puff {
 // One RK4 stage.  STAGE = 1..4 (rk4par's four derivs calls).
 template <typename R, int STAGE>
 __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
-    // an earlier kernel of this step raised a flag: skip (block-uniform, the
-    // box reduction below uses a barrier)
-    __shared__ int s_skip;
-    if (threadIdx.x == 0) s_skip = *a.flags;
-    __syncthreads();
-    if (s_skip) return;
+    // Flags raised by an earlier kernel of this step do not stop stages 1-3:
+    // they write only the step's scratch (k_s, dA_s), and a particle whose
+    // trial position is off the mesh takes no part in the gather and deposit
+    // (locate raises the flag again).  Stage 4 writes the state, and only
+    // when no flag is set: read here, with the particle loads (no kernel of
+    // stage 4 raises a flag: stage 3 checked its trial positions).
+    const int rejected = (STAGE == 4) ? *a.flags : 0;
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = i < a.n;
     const MeshGeom& m = a.mesh;
@@ -174,6 +175,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
         if (nf) atomicOr(a.flags, nf);
     } else {
         // y += h/6 (k1 + k4 + 2 (k2 + k3))  (puffin_mpi_RK4.f90:274-279, 307-312)
+        if (rejected) return;   // the step is rejected
         const R h6 = (R)a.h6;
         a.y0.x[i] = x0 + h6 * (a.k1.x[i] + kx_ + (R)2 * (a.k2.x[i] + a.kin.x[i]));
         a.y0.y[i] = y0 + h6 * (a.k1.y[i] + ky_ + (R)2 * (a.k2.y[i] + a.kin.y[i]));
