@@ -106,14 +106,14 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
 
     // ---- p2 family in FP64 (gamma_to_p2.f90:84-85) and its derivatives
     const double prd = (double)pr, pid = (double)pi;
-    const double gr2 = ph.gamma_r * ph.gamma_r, aw2 = ph.aw * ph.aw;
+    const double aw2 = ph.aw * ph.aw;
     const double s2 = prd * prd + pid * pid;
-    const double u = in ? 1.0 / sqrt(1.0 - 1.0 / (gr2 * g * g) * (1.0 + aw2 * s2)) : 1.0;
-    const double p2 = (u - 1.0) / ph.eta;
     const double invg = 1.0 / g;
+    const double u = in ? 1.0 / sqrt(1.0 - ph.inv_gr2 * invg * invg * (1.0 + aw2 * s2)) : 1.0;
+    const double p2 = (u - 1.0) * ph.inv_eta;
     const double fac = (1.0 + ph.eta * p2) * invg;
     const double u3 = u * u * u;
-    const double q = u3 / (gr2 * g * g);
+    const double q = u3 * ph.inv_gr2 * invg * invg;
     const double du_dpr = q * aw2 * prd;
     const double du_dpi = q * aw2 * pid;
     const double du_dg = -q * (1.0 + aw2 * s2) * invg;
@@ -161,8 +161,8 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
         static_assert(ADJ_FIELD_TILE_NODES <= ADJ_THREADS, "one field tile node per thread");
         const int q = threadIdx.x;
         if (q < tb.tx * tb.ty * tb.tz) {
-            const int lx = q % tb.tx, r = q / tb.tx;
-            const int ly = r % tb.ty, lz = r / tb.ty;
+            int lx, ly, lz;
+            box_coords(box_div(tb), q, lx, ly, lz);
             const int node = (tb.bz0 + lz) * m.nx * m.ny + (tb.by0 + ly) * m.nx + (tb.bx0 + lx);
             s_v[q] = adj_trial_field<R, STAGE>(a, node, cA);
             s_g[q] = adj_source_cotangent<R, STAGE>(a, node);
@@ -213,8 +213,9 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
 
     // ---- field cotangent of this stage: w_k (dL/dFr, dL/dFi) at the nodes
     if (scat) {
-        scatter_cells<R, ADJ_TILE_NODES>(m, tb, use_tile, ok, c, cr, ok ? (R)Lfr : (R)0,
-                         ok ? (R)Lfi : (R)0, a.mAout);
+        __shared__ double s_tile[2 * ADJ_TILE_NODES];
+        scatter_cells<R>(m, tb, use_tile, ok, c, cr, ok ? (R)Lfr : (R)0,
+                         ok ? (R)Lfi : (R)0, a.mAout, s_tile);
     }
 
     // ---- dJ/dn2col of this stage: workgroup sum into a spread accumulator

@@ -118,15 +118,13 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nblocks) {
     return (int64_t)(blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
 }
 
-// Wavefront (64-lane) reductions.
-__device__ __forceinline__ int wave_min(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
+// Wavefront (64-lane) reductions.  The integer ones use the ROCm device
+// library's wavefront reductions (DPP lane moves, no LDS round trips); all
+// 64 lanes must be active.
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_min_i32(int);
+extern "C" __device__ __attribute__((const)) int __ockl_wfred_max_i32(int);
+__device__ __forceinline__ int wave_min(int v) { return __ockl_wfred_min_i32(v); }
+__device__ __forceinline__ int wave_max(int v) { return __ockl_wfred_max_i32(v); }
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;

@@ -583,6 +583,8 @@ static int create(Ctx* c) {
     ph.inv2rho = 1.0 / (2.0 * p.rho);
     ph.inv_dV3 = 1.0 / (p.dx * p.dy * p.dz2);
     ph.c_dx = 2 * p.rho * p.kappa / std::sqrt(p.eta);
+    ph.inv_eta = 1.0 / p.eta;
+    ph.inv_gr2 = 1.0 / (p.gamma_r * p.gamma_r);
     ph.coupling = p.coupling;
     ph.electrons_evolve = p.electrons_evolve;
     ph.field_evolve = p.field_evolve;

@@ -21,6 +21,8 @@ struct Physics {
     double inv2rho;     // 1 / (2 rho)                    (rhs.f90:366)
     double inv_dV3;     // 1 / (dx dy dz2)                (rhs.f90:361)
     double c_dx;        // 2 rho kappa / sqrt(eta)        (puffin_equations.f90:115)
+    double inv_eta;     // 1 / eta
+    double inv_gr2;     // 1 / gamma_r^2
     int coupling, electrons_evolve, field_evolve;
 };
 

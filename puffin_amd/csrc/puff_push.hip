@@ -59,13 +59,14 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
         }
     }
 
-    // ---- p2 in FP64 (gamma_to_p2.f90:84-85)
+    // ---- p2 in FP64 (gamma_to_p2.f90:84-85), with 1/eta and 1/gamma_r^2
+    // from the host and 1/Gamma formed once
     const double prd = (double)pr, pid = (double)pi;
-    const double p2 = in ? ((1.0 / sqrt(1.0 - 1.0 / (ph.gamma_r * ph.gamma_r * g * g) *
-                                               (1.0 + ph.aw * ph.aw * (prd * prd + pid * pid)))) -
-                            1.0) / ph.eta
-                         : 0.0;
     const double invg = 1.0 / g;
+    const double p2 = in ? ((1.0 / sqrt(1.0 - ph.inv_gr2 * invg * invg *
+                                               (1.0 + ph.aw * ph.aw * (prd * prd + pid * pid)))) -
+                            1.0) * ph.inv_eta
+                         : 0.0;
     const double fac = (1.0 + ph.eta * p2) * invg;   // (1 + eta p2) / Gamma
 
     // ---- node and weights
@@ -89,15 +90,23 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // A0 + c dA_{s-1} is formed once per node of the box in LDS and the
     // corners are read from there; else from global memory.  Same arithmetic
     // either way.
+    // The field tile shares the deposit tile's LDS: it is used only for boxes
+    // of at most FIELD_TILE_NODES nodes, whose deposit needs the first
+    // 2 FIELD_TILE_NODES doubles, so it sits right after them.
+    constexpr int TILE = stage_tile_nodes<R>();
+    static_assert(2 * FIELD_TILE_NODES * sizeof(double) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
+                      2 * TILE * sizeof(double), "field tile inside the deposit tile");
+    __shared__ double s_dep[2 * TILE];
+    cplx<R>* s_fld = reinterpret_cast<cplx<R>*>(s_dep + 2 * FIELD_TILE_NODES);
     const R cA = (R)a.cin;
     const bool lds_field = use_tile && tb.tx * tb.ty * tb.tz <= FIELD_TILE_NODES;
-    __shared__ cplx<R> s_fld[FIELD_TILE_NODES];
     if (lds_field) {
         const int nt = tb.tx * tb.ty * tb.tz;
         const int ntr = m.nx * m.ny;
+        const BoxDiv bd = box_div(tb);
         for (int q = threadIdx.x; q < nt; q += blockDim.x) {
-            const int lx = q % tb.tx, r = q / tb.tx;
-            const int ly = r % tb.ty, lz = r / tb.ty;
+            int lx, ly, lz;
+            box_coords(bd, q, lx, ly, lz);
             const int node = (tb.bz0 + lz) * ntr + (tb.by0 + ly) * m.nx + (tb.bx0 + lx);
             cplx<R> v = a.A0[node];
             if (STAGE > 1) {
@@ -137,7 +146,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     if (ph.field_evolve) {
         const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
         const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
-        scatter_cells<R, stage_tile_nodes<R>()>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout);
+        scatter_cells<R>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout, s_dep);
     }
     if (!in) return;
     if (!ph.coupling) { fr = 0; fi = 0; }   // rhs.f90:221-224

@@ -171,6 +171,27 @@ __device__ __forceinline__ TileBox workgroup_box(const MeshGeom& m, bool ok, con
     return t;
 }
 
+// Node q of a box -> (lx, ly, lz), x fastest, without integer division: the
+// quotients come from float reciprocals of the box extents, exact while
+// q < 2^22 (the relative error of (q + 1/2) * RN(1/t) stays below the
+// distance 1/(2t) of (q + 1/2)/t from the next integer); boxes here hold at
+// most a few thousand nodes.
+struct BoxDiv {
+    int tx, ty;
+    float inv_tx, inv_ty;
+};
+
+__device__ __forceinline__ BoxDiv box_div(const TileBox& t) {
+    return BoxDiv{t.tx, t.ty, 1.0f / (float)t.tx, 1.0f / (float)t.ty};
+}
+
+__device__ __forceinline__ void box_coords(const BoxDiv& d, int q, int& lx, int& ly, int& lz) {
+    const int r = (int)(((float)q + 0.5f) * d.inv_tx);
+    lx = q - r * d.tx;
+    lz = (int)(((float)r + 0.5f) * d.inv_ty);
+    ly = r - lz * d.ty;
+}
+
 // Union of the spread box copies.
 __device__ __forceinline__ Box merged_box(const Box* box) {
     Box b;
@@ -253,15 +274,15 @@ __device__ __forceinline__ int corner_tile(const MeshGeom& m, const TileBox& t, 
 //             address in the same instruction.
 //   else:     lanes of a wavefront that share a cell are summed first (runs
 //             of equal keys), then one global atomic per run and corner.
-// use_tile must be block-uniform; every thread of the workgroup must call.
-template <typename R, int TILE>
+// s_tile: the LDS tile, 2 doubles per node of the box.  use_tile must be
+// block-uniform; every thread of the workgroup must call.
+template <typename R>
 __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& t, bool use_tile,
                                               bool ok, const Cell<R>& c, const CellRef& cr, R sr,
-                                              R si, cplx<R>* out) {
+                                              R si, cplx<R>* out, double* s_tile) {
     const int lane = threadIdx.x & 63;
     const int nnb = m.one_d ? 2 : 8;
     if (use_tile) {
-        __shared__ double s_tile[2 * TILE];
         const int nt = t.tx * t.ty * t.tz;
         for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) s_tile[q] = 0.0;
         __syncthreads();
@@ -277,11 +298,12 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
         }
         __syncthreads();
         const int ntr = m.nx * m.ny;
+        const BoxDiv bd = box_div(t);
         for (int q = threadIdx.x; q < nt; q += blockDim.x) {
             const double tr = s_tile[q], ti = s_tile[nt + q];
             if (tr != 0.0 || ti != 0.0) {
-                const int lx = q % t.tx, r = q / t.tx;
-                const int ly = r % t.ty, lz = r / t.ty;
+                int lx, ly, lz;
+                box_coords(bd, q, lx, ly, lz);
                 const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
                 atomic_add(&out[node].x, (R)tr);
                 atomic_add(&out[node].y, (R)ti);
