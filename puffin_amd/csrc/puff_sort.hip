@@ -7,15 +7,12 @@
 
 namespace puff {
 
-// Bin of each particle's cell in the slot order (bricks of cells, see
+// Bin of a particle's cell in the slot order (bricks of cells, see
 // sort_bins); particles outside the mesh go to the last bin.
 template <typename R>
-__global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const double* z2,
-                                                   int64_t n, MeshGeom m, int ncell, int* key) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ int cell_key(const MeshGeom& m, int ncell, R x, R y, double z2) {
     int k = ncell;
-    const double zq = floor(z2[i] * m.inv_dz2);
+    const double zq = floor(z2 * m.inv_dz2);
     int iz = -1;
     if (m.periodic) {
         if (fabs(zq) < 1.0e9) {
@@ -30,8 +27,8 @@ __global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const
         if (m.one_d) {
             k = iz;
         } else {
-            const R xq = floor((x[i] + (R)m.halfx) * (R)m.inv_dx);
-            const R yq = floor((y[i] + (R)m.halfy) * (R)m.inv_dy);
+            const R xq = floor((x + (R)m.halfx) * (R)m.inv_dx);
+            const R yq = floor((y + (R)m.halfy) * (R)m.inv_dy);
             if (xq >= (R)0 && xq <= (R)(m.nx - 2) && yq >= (R)0 && yq <= (R)(m.ny - 2)) {
                 const int ix = (int)xq, iy = (int)yq;
                 const int nbx = (m.nx - 1 + BRICK_X - 1) / BRICK_X;
@@ -62,7 +59,7 @@ __global__ __launch_bounds__(256) void k_cell_keys(const R* x, const R* y, const
             }
         }
     }
-    key[i] = k;
+    return k;
 }
 
 __global__ void k_fill_int(int* p, int64_t n, int v) {
@@ -100,10 +97,16 @@ __device__ __forceinline__ Run run_of(int k, bool valid) {
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_histogram(const int* key, int64_t n, int* counts) {
+// Key of every particle, and the histogram of the keys (counts zeroed
+// before): one atomic per run of equal keys in a wavefront.
+template <typename R>
+__global__ __launch_bounds__(256) void k_keys_histogram(const R* x, const R* y, const double* z2,
+                                                        int64_t n, MeshGeom m, int ncell, int* key,
+                                                        int* counts) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i < n;
-    const int k = valid ? key[i] : 0;
+    const int k = valid ? cell_key<R>(m, ncell, x[i], y[i], z2[i]) : 0;
+    if (valid) key[i] = k;
     const Run r = run_of(k, valid);
     if (valid && r.head) atomicAdd(&counts[k], r.len);
 }
@@ -241,10 +244,9 @@ template <typename R>
 void launch_sort(SortArgs<R>& a, hipStream_t s) {
     const int64_t n = a.n;
     const int64_t nb = (int64_t)a.ncell + 1;
-    hipLaunchKernelGGL((k_cell_keys<R>), g1(n), dim3(256), 0, s, a.y->x, a.y->y, a.y->z2, n, a.mesh, a.ncell,
-                       a.key);
     hipLaunchKernelGGL(k_fill_int, g1(nb), dim3(256), 0, s, a.counts, nb, 0);
-    hipLaunchKernelGGL(k_histogram, g1(n), dim3(256), 0, s, a.key, n, a.counts);
+    hipLaunchKernelGGL((k_keys_histogram<R>), g1(n), dim3(256), 0, s, a.y->x, a.y->y, a.y->z2, n, a.mesh,
+                       a.ncell, a.key, a.counts);
     const int64_t ntiles = (nb + SCAN_TILE - 1) / SCAN_TILE;
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(SCAN_THREADS), 0, s, a.counts, nb, a.tile_sums);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, a.tile_sums, ntiles);
