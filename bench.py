@@ -315,12 +315,13 @@ def _kinds(prof: dict) -> dict:
     return {k: dict(ms=round(v[0], 3), calls=v[1]) for k, v in prof.items() if v[1]}
 
 
-def c4_bench(steps: int = 30, adjoint_steps: int = 30, every: int = 10):
+def c4_bench(steps: int = 30, adjoint_steps: int = 30, every: int = 4, captures: bool = True):
     """The large FP32-mixed workload (257 x 257 x 4096 temporal mesh, 2^27
     macroparticles, per-period taper table): throughput over `steps` steps
     (one diffraction), device time per kernel kind over as many more, device
     memory, and the gradient of a module of `adjoint_steps` steps (snapshots
-    every `every` steps) with its time and memory."""
+    every `every` steps; with step captures, about 27 GB each, the adjoint
+    steps replay no forward stage) with its time and memory."""
     from puffin_amd import gpu
     from puffin_amd.adjoint import ModuleGradient, module_program
     from puffin_amd.configs import case_c4
@@ -365,7 +366,8 @@ def c4_bench(steps: int = 30, adjoint_steps: int = 30, every: int = 10):
         drv.upload_particles(c.particles)
         ops = module_program(adjoint_steps, c.h, c.steps4diff, diffraction=True, periodic=False)
         shape = (c.field[0] / c.info["seed_amp"], c.field[1] / c.info["seed_amp"])
-        eng = ModuleGradient(drv, ops, shape, every=every, taper_period=c.taper_period)
+        eng = ModuleGradient(drv, ops, shape, every=every, taper_period=c.taper_period,
+                             capture=captures)
         table, seed = c.taper_table, c.info["seed_amp"]
         eng.evaluate(table, seed)   # first use of the per-module inputs
         drv.synchronize()
