@@ -134,13 +134,14 @@ def test_diffraction_step_matches_oracle(periodic):
 # 3-D RK4 steps against the oracle
 # ---------------------------------------------------------------------------
 
-def _small_3d(periodic=0, n=3000, seed_amp=1e-2, und_ends=0, planar=False):
+def _small_3d(periodic=0, n=3000, seed_amp=1e-2, und_ends=0, planar=False, nx=17, ny=17, nz2=None):
     rho = 0.01
     ux, uy = (1.0, 0.0) if planar else (1 / math.sqrt(2), 1 / math.sqrt(2))
     eta, kappa, fx, fy = scaling(rho, 1.0, 100.0, ux, uy)
     h = step_size(rho, 30)
-    nz2 = 65 if periodic else 80
-    mp = ModuleParams(nx=17, ny=17, nz2=nz2, dx=0.4, dy=0.4, dz2=4 * 3.14159274 * rho / 16,
+    if nz2 is None:
+        nz2 = 65 if periodic else 80
+    mp = ModuleParams(nx=nx, ny=ny, nz2=nz2, dx=0.4, dy=0.4, dz2=4 * 3.14159274 * rho / 16,
                       rho=rho, eta=eta, kappa=kappa, gamma_r=100.0, aw=1.0, fx=fx, fy=fy,
                       periodic=periodic, und_ends=und_ends, szfs=0.0 if not und_ends else 8 * 3.14159274 * rho,
                       szfe=60 * h if not und_ends else 60 * h - 8 * 3.14159274 * rho,
@@ -153,6 +154,31 @@ def _small_3d(periodic=0, n=3000, seed_amp=1e-2, und_ends=0, planar=False):
     match_in(mp, 1.0, p)
     ar, ai = gaussian_seed(mp, seed_amp, seed_amp, w0=1.5)
     return mp, p, (ar, ai), h
+
+
+@pytest.mark.parametrize("periodic,nz2", [(0, 79), (1, 70)])
+def test_rk4_steps_odd_mesh_match_oracle(periodic, nz2):
+    """A mesh whose cell counts (13 x 13 x 78 or 69) leave partial sort bricks
+    on every axis, with re-sorts between the steps."""
+    print(f"[gpu] rk4 odd mesh periodic={periodic} nz2={nz2}", flush=True)
+    mp, p, (ar, ai), h = _small_3d(periodic, nx=14, ny=14, nz2=nz2)
+    o = orc.Oracle(mp, p, (ar, ai), taper=(1.0, 1.0, -0.01))
+    with gpu.Integrator(mp, p["x"].size, FP64) as g:
+        g.upload_particles(p)
+        g.upload_field(ar, ai)
+        g.set_taper(1.0, 1.0, -0.01)
+        g.set_sort_interval(3)
+        szl = 0.0
+        for _ in range(12):
+            assert o.rk4par(szl, h) == 0
+            assert g.rk4par(szl, h) == 0
+            szl += h
+        out = g.download_particles()
+        gr, gi = g.download_field()
+    for k in KEYS:
+        assert relerr(out[k], o.parts[k]) < 1e-10, k
+    assert relerr(gr, o.ar) < 1e-10
+    assert relerr(gi, o.ai) < 1e-10
 
 
 @pytest.mark.parametrize("periodic,und_ends,planar", [(0, 0, False), (1, 0, False), (0, 1, True)])
