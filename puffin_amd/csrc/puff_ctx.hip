@@ -182,14 +182,15 @@ static int powmod(int b, int e, int n) {
     return (int)r;
 }
 
-// Radix passes of an n-point line: factors of 5 and more (direct DFT passes)
-// first, largest first, so that the costliest pass runs with p = 1 and
-// needs no input twiddles; then 4s, 2s and 3s.
+// Radix passes of an n-point line: prime factors of 5 and more (direct DFT
+// passes) first, largest first, so that the costliest pass runs with p = 1
+// and needs no input twiddles; then 16s, 4s, 2s and 3s.
 static void factorize(int n, FftAxis& ax) {
     ax.n = n;
     ax.npass = 0;
     int m = n;
-    int n4 = 0, n2 = 0, n3 = 0;
+    int n16 = 0, n4 = 0, n2 = 0, n3 = 0;
+    while (m % 16 == 0) { ++n16; m /= 16; }
     while (m % 4 == 0) { ++n4; m /= 4; }
     while (m % 2 == 0) { ++n2; m /= 2; }
     while (m % 3 == 0) { ++n3; m /= 3; }
@@ -199,6 +200,7 @@ static void factorize(int n, FftAxis& ax) {
         while (m % f == 0) { big[nbig++] = f; m /= f; }
     if (m > 1) big[nbig++] = m;
     for (int k = nbig - 1; k >= 0; --k) ax.radix[ax.npass++] = big[k];
+    for (int k = 0; k < n16; ++k) ax.radix[ax.npass++] = 16;
     for (int k = 0; k < n4; ++k) ax.radix[ax.npass++] = 4;
     for (int k = 0; k < n2; ++k) ax.radix[ax.npass++] = 2;
     for (int k = 0; k < n3; ++k) ax.radix[ax.npass++] = 3;

@@ -7,8 +7,10 @@
 //
 // Each workgroup holds B whole lines in LDS and runs a Stockham autosort
 // transform (natural order in and out), one radix pass per LDS round trip.
-// Radices 2, 3 and 4 have explicit butterflies; any other factor (11, 43,
-// 127, 257, ...) is a direct DFT pass computing one output per work item.
+// Radices 2, 3, 4 and 16 have explicit butterflies (16 as 4 x 4 in
+// registers, halving the passes and barriers of power-of-two lines); any
+// other factor (11, 43, 127, ...) is a direct DFT pass; a prime line length
+// with a power-of-two n - 1 (17, 257) uses Rader's algorithm.
 //
 // DiffractionStep (diffraction.f90:209-348) becomes nine line passes:
 //   x fwd | y fwd | z fwd . multiplyexp . z inv | y inv |
@@ -33,6 +35,45 @@ __device__ __forceinline__ cplx<R> tw_get(const cplx<R>* tw, int e, int sign) {
     return w;
 }
 
+// 4-point DFT in place (x_k = sum_n x_n W4^(nk), W4 = exp(sign 2 pi i / 4)).
+template <typename R>
+__device__ __forceinline__ void dft4(cplx<R>& x0, cplx<R>& x1, cplx<R>& x2, cplx<R>& x3, int sign) {
+    const cplx<R> a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    const cplx<R> b0 = cadd(x1, x3), b1 = csub(x1, x3);
+    // W4 b1: sign -1 -> -i b1, sign +1 -> +i b1
+    const cplx<R> wb = (sign < 0) ? cplx<R>{b1.y, -b1.x} : cplx<R>{-b1.y, b1.x};
+    x0 = cadd(a0, b0);
+    x1 = cadd(a1, wb);
+    x2 = csub(a0, b0);
+    x3 = csub(a1, wb);
+}
+
+// 16-point DFT in registers as 4 x 4 (Cooley-Tukey): with j = 4 j1 + j0 and
+// q = q0 + 4 q1, X_q = sum_j0 W4^(j0 q1) [w^(j0 q0) sum_j1 v_(4 j1 + j0) W4^(j1 q0)],
+// w = exp(sign 2 pi i / 16).  On return v[4 q0 + q1] = X_(q0 + 4 q1).
+template <typename R>
+__device__ __forceinline__ void dft16(cplx<R>* v, int sign) {
+    const R c1 = (R)0.92387953251128675613, s1 = (R)0.38268343236508977173;   // cos, sin of pi/8
+    const R c2 = (R)0.70710678118654752440;                                    // cos pi/4
+    const R sg = (R)(sign < 0 ? -1 : 1);
+#pragma unroll
+    for (int j0 = 0; j0 < 4; ++j0) dft4(v[j0], v[4 + j0], v[8 + j0], v[12 + j0], sign);
+    // v[4 q0 + j0] = Y[j0][q0]; twiddle by w^(j0 q0): m = 1, 2, 3 | 2, 4, 6 | 3, 6, 9
+    const cplx<R> w1 = {c1, sg * s1}, w2 = {c2, sg * c2}, w3 = {s1, sg * c1};
+    const cplx<R> w6 = {-c2, sg * c2}, w9 = {-c1, -sg * s1};
+    v[5] = cmul(v[5], w1);
+    v[9] = cmul(v[9], w2);
+    v[13] = cmul(v[13], w3);
+    v[6] = cmul(v[6], w2);
+    v[10] = (sign < 0) ? cplx<R>{v[10].y, -v[10].x} : cplx<R>{-v[10].y, v[10].x};   // w^4 = W4
+    v[14] = cmul(v[14], w6);
+    v[7] = cmul(v[7], w3);
+    v[11] = cmul(v[11], w6);
+    v[15] = cmul(v[15], w9);
+#pragma unroll
+    for (int q0 = 0; q0 < 4; ++q0) dft4(v[4 * q0], v[4 * q0 + 1], v[4 * q0 + 2], v[4 * q0 + 3], sign);
+}
+
 // In-LDS mixed-radix transform of length N over elements 0..N-1 of B lines
 // laid out with line length LD (N <= LD; elements N..LD-1 are left alone).
 // src holds the input; returns the buffer holding the output (src or dst).
@@ -44,7 +85,26 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
         const int r = radix[ps];
         const int T = N / r;
         const int twst = N / (p * r);
-        if (r == 2 || r == 3 || r == 4) {
+        if (r == 16) {
+            for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
+                int b, i;
+                if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
+                const int k = i % p;
+                const int ob = (i / p) * p * 16 + k;
+                cplx<R> v[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
+                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, (j * k * twst) % N, sign));
+                }
+                dft16(v, sign);
+#pragma unroll
+                for (int q0 = 0; q0 < 4; ++q0)
+#pragma unroll
+                    for (int q1 = 0; q1 < 4; ++q1)
+                        dst[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, LD)] = v[4 * q0 + q1];
+            }
+        } else if (r == 2 || r == 3 || r == 4) {
             for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
                 int b, i;
                 if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
