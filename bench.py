@@ -311,6 +311,39 @@ def device_memory_used() -> int:
     return int(total - free)
 
 
+def diffraction_bench(events: int = 5):
+    """Device time of one diffraction event (DiffractionStep: propagator and
+    absorption, nine line passes) on the C1 mesh (FP64 and FP32-mixed) and
+    the C4 mesh (FP32-mixed), against its algorithmic bytes (14 complex per
+    node)."""
+    from puffin_amd import gpu
+    from puffin_amd.configs import case_c1, case_c4
+    out = {}
+    for name, make, precs in (("c1", case_c1, (0, 1)), ("c4", case_c4, (1,))):
+        c = make(n=1024)
+        mp = c.mp
+        for prec in precs:
+            g = gpu.Integrator(mp, 1024, prec)
+            try:
+                g.upload_field(*c.field)
+                g.diffraction_step(c.steps4diff * c.h)
+                g.synchronize()
+                g.profile(True)
+                g.profile_read(reset=True)
+                for _ in range(events):
+                    g.diffraction_step(c.steps4diff * c.h)
+                ms, cnt = g.profile_read(reset=True)["diffraction"]
+            finally:
+                g.close()
+            nbytes = mp.nx * mp.ny * mp.nzp * 14 * (16 if prec == 0 else 8)
+            gbps = nbytes / (ms / cnt * 1e-3) / 1e9
+            key = f"{name}_{'fp64' if prec == 0 else 'mixed'}"
+            out[key] = dict(mesh=[mp.nx, mp.ny, mp.nz2], ms=ms / cnt, bytes=nbytes, GBps=gbps,
+                            frac=gbps / HBM_PEAK_GBS)
+            log(f"diffraction {key}: {ms / cnt:.2f} ms ({gbps:.0f} GB/s)")
+    return out
+
+
 def _kinds(prof: dict) -> dict:
     return {k: dict(ms=round(v[0], 3), calls=v[1]) for k, v in prof.items() if v[1]}
 
@@ -430,6 +463,8 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the gradient measurement (its JSON object on stdout)")
     ap.add_argument("--no-c4", action="store_true", help="skip the large FP32-mixed workload")
+    ap.add_argument("--diffraction-only", action="store_true",
+                    help="only the diffraction event on the C1 and C4 meshes (its JSON object on stdout)")
     ap.add_argument("--c4-only", action="store_true",
                     help="only the large FP32-mixed workload (its JSON object on stdout)")
     args = ap.parse_args()
@@ -441,6 +476,9 @@ def main():
         return
     if args.c4_only:
         print(json.dumps(c4_bench()), flush=True)
+        return
+    if args.diffraction_only:
+        print(json.dumps(diffraction_bench()), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
