@@ -179,6 +179,43 @@ int puff_transform(puff_ctx* ctx, int32_t sign);
    (para_field.f90:212-215).  No-op on a temporal mesh. */
 int puff_wrap_z2(puff_ctx* ctx);
 
+/*
+ * Lattice elements between undulator modules, on the device macroparticles
+ * (acc_lattice.f90; the element chain is puffin_module.f90:58-90).  Each
+ * returns after the device work is done.  The diffraction that the reference
+ * applies over a drift or a chicane's zbar length (driftSection :512,
+ * disperse :447) is the caller's puff_diffraction_step.
+ *
+ * puff_drift       driftSection (acc_lattice.f90:478-518): z2 += dz p2 and,
+ *                  on a 3-D mesh, x += v px dz, y -= v py dz with
+ *                  v = 2 rho kappa / sqrt(eta) (1 + eta p2) / Gamma.
+ * puff_chicane     disperse (:420-463): z2 -= 2 disp (Gamma - 1) - slip,
+ *                  wrapped on a periodic mesh.  disp and slip as the
+ *                  reference holds them after readLatt (slip already scaled
+ *                  by 4 pi rho, :356-363).
+ * puff_quad        Quad (:531-561): px += sqrt(eta)/(2 rho kappa) x / fx,
+ *                  py -= sqrt(eta)/(2 rho kappa) y / fy (3-D meshes only).
+ * puff_modulation  bModulation (:575-585): Gamma += mag cos(wavenum z2).
+ * puff_correct_trans  correctTrans (:601-623): subtract the chi-weighted
+ *                  means of x, y, px, py.
+ * puff_match       matchIn (sign +1, :743-828) or matchOut (sign -1,
+ *                  :638-725) at zbar sz with strength n2col, for the
+ *                  context's undulator type (curved and plane-pole
+ *                  corrections included; init_conds.f90:65-223).
+ * puff_set_undulator  the per-module undulator of initUndulator
+ *                  (:848-895): type, polarisation fx/fy, in-undulator
+ *                  focusing wavenumbers, and the module-local zbar of the
+ *                  main section (sZFS, sZFE).
+ */
+int puff_drift(puff_ctx* ctx, double dz);
+int puff_chicane(puff_ctx* ctx, double disp, double slip);
+int puff_quad(puff_ctx* ctx, double fx, double fy);
+int puff_modulation(puff_ctx* ctx, double wavenum, double mag);
+int puff_correct_trans(puff_ctx* ctx);
+int puff_match(puff_ctx* ctx, double sz, double n2col, int32_t sign);
+int puff_set_undulator(puff_ctx* ctx, int32_t und_type, double fx, double fy, double kbeta_x,
+                       double kbeta_y, double szfs, double szfe);
+
 /* Power per z2 node: trapezoidal integral of |A|^2 over x and y
    (gPowerP / fPower_3D, av_write.f90:39-202), or |A|^2 in 1-D. */
 int puff_power(puff_ctx* ctx, double* power_host);

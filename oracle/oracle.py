@@ -73,6 +73,10 @@ def lib():
         l.orc_diffraction_adjoint.restype = ctypes.c_int
         l.orc_diffraction_adjoint.argtypes = [P, _dp, _dp, ctypes.c_double, ctypes.c_int]
         l.orc_correct_trans.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp, _dp]
+        l.orc_drift.argtypes = [P, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double]
+        l.orc_chicane.argtypes = [P, ctypes.c_int64, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        l.orc_quad.argtypes = [P, ctypes.c_int64, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        l.orc_modulation.argtypes = [ctypes.c_int64, _dp, _dp, ctypes.c_double, ctypes.c_double]
         l.orc_und_section.restype = ctypes.c_int
         l.orc_und_section.argtypes = ([P, _dp, ctypes.c_int64] + [_dp] * 9 +
                                       [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -236,6 +240,31 @@ def match(mp: ModuleParams, n2col: float, parts: dict, sz: float, sign: int):
 def set_deposit_atomic(on: bool):
     """Shared-grid atomic deposit (timing runs) instead of private grids."""
     lib().orc_set_deposit_atomic(int(on))
+
+
+def drift(mp: ModuleParams, parts: dict, dz: float):
+    """driftSection's particle part (acc_lattice.f90:478-518), in place."""
+    p = mp.to_ctypes()
+    lib().orc_drift(ctypes.byref(p), parts["x"].size, _ptr(parts["x"]), _ptr(parts["y"]), _ptr(parts["z2"]),
+                    _ptr(parts["px"]), _ptr(parts["py"]), _ptr(parts["gam"]), dz)
+
+
+def chicane(mp: ModuleParams, parts: dict, disp: float, slip: float):
+    """disperse's particle part (acc_lattice.f90:420-463), in place."""
+    p = mp.to_ctypes()
+    lib().orc_chicane(ctypes.byref(p), parts["z2"].size, _ptr(parts["z2"]), _ptr(parts["gam"]), disp, slip)
+
+
+def quad(mp: ModuleParams, parts: dict, qfx: float, qfy: float):
+    """Quad (acc_lattice.f90:531-561), in place."""
+    p = mp.to_ctypes()
+    lib().orc_quad(ctypes.byref(p), parts["x"].size, _ptr(parts["x"]), _ptr(parts["y"]), _ptr(parts["px"]),
+                   _ptr(parts["py"]), qfx, qfy)
+
+
+def modulation(parts: dict, wavenum: float, mag: float):
+    """bModulation (acc_lattice.f90:575-585), in place."""
+    lib().orc_modulation(parts["z2"].size, _ptr(parts["z2"]), _ptr(parts["gam"]), wavenum, mag)
 
 
 def correct_trans(parts: dict):

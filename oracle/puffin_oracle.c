@@ -1028,18 +1028,31 @@ static double orc_py_offset(double z, double rho, double ux, double n2col) {
 }
 
 /* matchIn (sign = +1, acc_lattice.f90:743-828) and matchOut (sign = -1,
-   :638-725) for the elliptical / helical undulator, with xOffSet_Array and
-   yOffSet_Array (init_conds.f90:65-86, 129-152). */
+   :638-725) with xOffSet_Array and yOffSet_Array (init_conds.f90:65-86,
+   129-152).  The p offsets are per particle for the curved-pole and
+   plane-pole undulators (2nd-order terms in x, y, :653-685 / :758-790). */
 void orc_match(const puff_params* p, double n2col, int64_t n, double* x,
                double* y, double* px, double* py, const double* gam, double sz,
                int sign) {
-    double spx0 = orc_px_offset(sz, p->rho, p->fy, n2col);
-    double spy0 = -1.0 * orc_py_offset(sz, p->rho, p->fx, n2col);
-    /* matchIn passes -spy0_offset as py, matchOut passes spy0_offset */
-    double pyarg = (sign > 0) ? -spy0 : spy0;
     double ncx = p->aw * p->aw;
     double ncy = 2.0 * p->aw * p->aw / (p->fx * p->fx + p->fy * p->fy);
+    double kx = p->kx_und, ky = p->ky_und;
     for (int64_t i = 0; i < n; i++) {
+        double spx0, spy0;
+        if (p->und_type == 3) {          /* curved */
+            spx0 = orc_px_offset(sz, p->rho, p->fy, n2col) - 0.5 * kx * kx * x[i] * x[i] -
+                   0.5 * ky * ky * y[i] * y[i];
+            spy0 = -1.0 * (orc_py_offset(sz, p->rho, p->fx, n2col) - kx * kx * x[i] * y[i]);
+        } else if (p->und_type == 1) {   /* planepole */
+            spx0 = orc_px_offset(sz, p->rho, p->fy, n2col) -
+                   0.5 * (p->eta / (4.0 * p->rho * p->rho)) * x[i] * x[i];
+            spy0 = -1.0 * orc_py_offset(sz, p->rho, p->fx, n2col);
+        } else {
+            spx0 = orc_px_offset(sz, p->rho, p->fy, n2col);
+            spy0 = -1.0 * orc_py_offset(sz, p->rho, p->fx, n2col);
+        }
+        /* matchIn passes -spy0_offset as py, matchOut passes spy0_offset */
+        double pyarg = (sign > 0) ? -spy0 : spy0;
         double gj = p->gamma_r * gam[i];
         double xo = -p->fy * 4.0 * p->kappa * pow(p->rho, 2.0) / sqrt(p->eta) *
                     (p->gamma_r / sqrt(gj * gj - (1.0 + ncx * (spx0 * spx0 + pyarg * pyarg)))) *
@@ -1059,6 +1072,45 @@ void orc_match(const puff_params* p, double n2col, int64_t n, double* x,
             py[i] = py[i] - spy0;
         }
     }
+}
+
+/* driftSection, acc_lattice.f90:478-518 (particle part): z2 += dz p2 and,
+   in 3-D, x += 2 rho kappa/sqrt(eta) (1 + eta p2)/Gamma px dz, y -= ... py dz. */
+void orc_drift(const puff_params* p, int64_t n, double* x, double* y, double* z2,
+               const double* px, const double* py, const double* gam, double dz) {
+    for (int64_t i = 0; i < n; i++) {
+        double p2 = orc_p2(gam[i], px[i], py[i], p->eta, p->gamma_r, p->aw);
+        z2[i] = z2[i] + dz * p2;
+        if (!p->one_d) {
+            x[i] = x[i] + (2 * p->rho * p->kappa / sqrt(p->eta) * (1 + p->eta * p2) / gam[i] * px[i]) * dz;
+            y[i] = y[i] - (2 * p->rho * p->kappa / sqrt(p->eta) * (1 + p->eta * p2) / gam[i] * py[i]) * dz;
+        }
+    }
+}
+
+/* disperse, acc_lattice.f90:420-463 (particle part), with the periodic wrap. */
+void orc_chicane(const puff_params* p, int64_t n, double* z2, const double* gam,
+                 double disp, double slip) {
+    double lenz2 = p->dz2 * (double)(p->nz2 - 1);
+    for (int64_t i = 0; i < n; i++) {
+        z2[i] = z2[i] - 2.0 * disp * (gam[i] - 1.0) + slip;
+        if (p->periodic) z2[i] = z2[i] - (floor(z2[i] / lenz2) * lenz2);
+    }
+}
+
+/* Quad, acc_lattice.f90:531-561. */
+void orc_quad(const puff_params* p, int64_t n, const double* x, const double* y,
+              double* px, double* py, double qfx, double qfy) {
+    if (p->one_d) return;
+    for (int64_t i = 0; i < n; i++) {
+        px[i] = px[i] + sqrt(p->eta) / (2 * p->rho * p->kappa) * x[i] / qfx;
+        py[i] = py[i] - sqrt(p->eta) / (2 * p->rho * p->kappa) * y[i] / qfy;
+    }
+}
+
+/* bModulation, acc_lattice.f90:575-585. */
+void orc_modulation(int64_t n, const double* z2, double* gam, double wavenum, double mag) {
+    for (int64_t i = 0; i < n; i++) gam[i] = gam[i] + (mag * cos(wavenum * z2[i]));
 }
 
 /* correctTrans, acc_lattice.f90:601-623 with arr_mean_para_weighted

@@ -1651,6 +1651,23 @@ struct puff_ctx {
         }                                                \
     } while (0)
 
+namespace puff {
+
+// One lattice element on the device macroparticles (puff_lattice.hip).  The
+// corrector's sums use the diagnostics buffer.  Elements that move particles
+// leave the slot order stale: the next step re-sorts.
+template <typename R>
+static int lattice_op(Ctx* c, const LatticeOp& op) {
+    Particles<R>& P = parts<R>(c);
+    launch_lattice<R>(op, P.y, P.chi, c->n, c->phys, c->mesh, c->diag, c->stream);
+    PUFF_HIP(hipGetLastError());
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    if (op.kind != LAT_MODULATION && op.kind != LAT_QUAD) c->sort_due = true;
+    return PUFF_OK;
+}
+
+}  // namespace puff
+
 #define DISPATCH(ctx, fn, ...)                                                      \
     ((ctx)->c.precision == PUFF_FP64 ? puff::fn<double>(&(ctx)->c, __VA_ARGS__)     \
                                      : puff::fn<float>(&(ctx)->c, __VA_ARGS__))
@@ -1808,6 +1825,98 @@ int puff_wrap_z2(puff_ctx* ctx) {
     puff::launch_wrap(z2, c.n, c.p.dz2 * (double)(c.p.nz2 - 1), c.stream);
     if (hipGetLastError() != hipSuccess) return PUFF_ERR_HIP;
     return hipStreamSynchronize(c.stream) == hipSuccess ? PUFF_OK : PUFF_ERR_HIP;
+}
+
+int puff_drift(puff_ctx* ctx, double dz) {
+    CTX_CHECK(ctx);
+    puff::LatticeOp op{};
+    op.kind = puff::LAT_DRIFT;
+    op.a = dz;
+    return DISPATCH(ctx, lattice_op, op);
+}
+
+int puff_chicane(puff_ctx* ctx, double disp, double slip) {
+    CTX_CHECK(ctx);
+    puff::LatticeOp op{};
+    op.kind = puff::LAT_CHICANE;
+    op.a = disp;
+    op.b = slip;
+    op.lenz2 = ctx->c.p.periodic ? ctx->c.p.dz2 * (double)(ctx->c.p.nz2 - 1) : 0.0;
+    return DISPATCH(ctx, lattice_op, op);
+}
+
+int puff_quad(puff_ctx* ctx, double fx, double fy) {
+    CTX_CHECK(ctx);
+    if (fx == 0.0 || fy == 0.0) {
+        puff::set_error("quad focal strengths must be non-zero");
+        return PUFF_ERR_ARG;
+    }
+    puff::LatticeOp op{};
+    op.kind = puff::LAT_QUAD;
+    op.a = fx;
+    op.b = fy;
+    return DISPATCH(ctx, lattice_op, op);
+}
+
+int puff_modulation(puff_ctx* ctx, double wavenum, double mag) {
+    CTX_CHECK(ctx);
+    puff::LatticeOp op{};
+    op.kind = puff::LAT_MODULATION;
+    op.a = wavenum;
+    op.b = mag;
+    return DISPATCH(ctx, lattice_op, op);
+}
+
+int puff_correct_trans(puff_ctx* ctx) {
+    CTX_CHECK(ctx);
+    puff::LatticeOp op{};
+    op.kind = puff::LAT_CORRECT_TRANS;
+    return DISPATCH(ctx, lattice_op, op);
+}
+
+int puff_match(puff_ctx* ctx, double sz, double n2col, int32_t sign) {
+    CTX_CHECK(ctx);
+    if (sign != 1 && sign != -1) {
+        puff::set_error("match sign must be +1 (entry) or -1 (exit)");
+        return PUFF_ERR_ARG;
+    }
+    const puff_params& p = ctx->c.p;
+    puff::LatticeOp op{};
+    op.kind = puff::LAT_MATCH;
+    puff::MatchArgs& m = op.match;
+    m.sz = sz;
+    m.n2col = n2col;
+    m.rho = p.rho;
+    m.eta = p.eta;
+    m.kappa = p.kappa;
+    m.gamma_r = p.gamma_r;
+    m.ux = p.fx;
+    m.uy = p.fy;
+    m.ncx = p.aw * p.aw;
+    m.ncy = 2.0 * p.aw * p.aw / (p.fx * p.fx + p.fy * p.fy);
+    m.kx = p.kx_und;
+    m.ky = p.ky_und;
+    m.und_type = p.und_type;
+    m.sign = sign;
+    return DISPATCH(ctx, lattice_op, op);
+}
+
+int puff_set_undulator(puff_ctx* ctx, int32_t und_type, double fx, double fy, double kbeta_x,
+                       double kbeta_y, double szfs, double szfe) {
+    CTX_CHECK(ctx);
+    if (und_type < 0 || und_type > PUFF_UND_CURVED || !(fx * fx + fy * fy > 0.0)) {
+        puff::set_error("invalid undulator type or polarisation");
+        return PUFF_ERR_ARG;
+    }
+    puff_params& p = ctx->c.p;
+    p.und_type = und_type;
+    p.fx = fx;
+    p.fy = fy;
+    p.kbeta_x = kbeta_x;
+    p.kbeta_y = kbeta_y;
+    p.szfs = szfs;
+    p.szfe = szfe;
+    return PUFF_OK;
 }
 
 int puff_power(puff_ctx* ctx, double* out) {

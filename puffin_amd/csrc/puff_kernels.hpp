@@ -197,6 +197,31 @@ struct SortArgs {
 
 template <typename R>
 void launch_sort(SortArgs<R>& a, hipStream_t s);
+
+// Lattice elements between undulator modules (puff_lattice.hip).
+enum LatticeKind { LAT_DRIFT, LAT_CHICANE, LAT_QUAD, LAT_MODULATION, LAT_CORRECT_TRANS, LAT_MATCH };
+
+struct MatchArgs {
+    double sz, n2col;             // module-local zbar, undulator strength
+    double rho, eta, kappa, gamma_r;
+    double ux, uy;                // fx_G, fy_G
+    double ncx, ncy;              // aw^2 and 2 aw^2 / (ux^2 + uy^2)
+    double kx, ky;                // curved-pole wavenumbers
+    int und_type;
+    int sign;                     // +1 matchIn, -1 matchOut
+};
+
+struct LatticeOp {
+    int kind;
+    double a, b;                  // drift: dz; chicane: disp, slip; quad: fx, fy;
+                                  // modulation: wavenumber, magnitude
+    double lenz2;                 // chicane: periodic wrap length (0: none)
+    MatchArgs match;
+};
+
+template <typename R>
+void launch_lattice(const LatticeOp& op, PartPtrs<R> y, const R* chi, int64_t n, const Physics& ph,
+                    const MeshGeom& m, double* sums, hipStream_t s);
 void launch_iota(int* p, int64_t n, hipStream_t s);
 template <typename T>
 void launch_unpermute(const T* src, const int* orig, int64_t n, double* dst, hipStream_t s);

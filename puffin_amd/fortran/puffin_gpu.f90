@@ -7,12 +7,25 @@
 !   diffractIM(sStep, qDiffrctd, qOK)  diffraction/diffraction.f90:46-52
 !   DiffractionStep(h, sAr, sAi, qOK)  diffraction/diffraction.f90:209-220
 !   allact_rk4_arrs / deallact_rk4_arrs  interaction/puffin_mpi_RK4.f90:374-460
+!   matchIn(sZ) / matchOut(sZ)          acc_lattice.f90:743-828 / 638-725
+!   correctTrans()                     acc_lattice.f90:601-623
+! and the bodies of the other lattice elements with their values passed in
+! (the reference indexes its lattice arrays):
+!   gpu_drift(del_dr_z)                driftSection, acc_lattice.f90:478-518
+!   gpu_disperse(disp, slip)           disperse, acc_lattice.f90:420-463
+!   gpu_quad(qfx, qfy)                 Quad, acc_lattice.f90:531-561
+!   gpu_bmodulation(wavenum, mag)      bModulation, acc_lattice.f90:575-585
 !
 ! The particle arrays sElX_G .. sElGam_G, s_chi_bar_G (deriv_globals.f90:94,
 ! 128-133) and the field ac_rfield_in / ac_ifield_in (puffin_mpi_RK4.f90:24,
 ! whole mesh here, x fastest) stay resident on the device between
 ! allact_rk4_arrs and deallact_rk4_arrs; the flags qPArrOK_G / qInnerXYOK_G
 ! (deriv_globals.f90:407-409) are set from the status code of each step.
+! A lattice element called while no device state is live uploads the module
+! globals first and leaves the device state live: from then on the device
+! copy is the current one until deallact_rk4_arrs or gpu_sync_to_host, and a
+! following allact_rk4_arrs keeps it (updating only the module's undulator
+! and taper).
 
 module puffin_gpu_capi
 
@@ -136,6 +149,57 @@ module puffin_gpu_capi
       integer(c_int) :: r
     end function
 
+    function puff_drift(ctx, dz) bind(C, name="puff_drift") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: dz
+      integer(c_int) :: r
+    end function
+
+    function puff_chicane(ctx, disp, slip) bind(C, name="puff_chicane") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: disp, slip
+      integer(c_int) :: r
+    end function
+
+    function puff_quad(ctx, fx, fy) bind(C, name="puff_quad") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: fx, fy
+      integer(c_int) :: r
+    end function
+
+    function puff_modulation(ctx, wavenum, mag) bind(C, name="puff_modulation") result(r)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: wavenum, mag
+      integer(c_int) :: r
+    end function
+
+    function puff_correct_trans(ctx) bind(C, name="puff_correct_trans") result(r)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: r
+    end function
+
+    function puff_match(ctx, sz, n2col, sign) bind(C, name="puff_match") result(r)
+      import :: c_ptr, c_int, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      real(c_double), value :: sz, n2col
+      integer(c_int32_t), value :: sign
+      integer(c_int) :: r
+    end function
+
+    function puff_set_undulator(ctx, und_type, fx, fy, kbeta_x, kbeta_y, szfs, szfe) &
+        bind(C, name="puff_set_undulator") result(r)
+      import :: c_ptr, c_int, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: und_type
+      real(c_double), value :: fx, fy, kbeta_x, kbeta_y, szfs, szfe
+      integer(c_int) :: r
+    end function
+
   end interface
 
 end module puffin_gpu_capi
@@ -191,8 +255,18 @@ contains
   ! puffin_mpi_RK4.f90:374-418, and its outer2Inner copy).
   subroutine allact_rk4_arrs()
     integer(c_int64_t) :: n
-    if (c_associated(ctx_G)) call deallact_rk4_arrs()
     n = int(size(sElX_G), c_int64_t)
+    if (c_associated(ctx_G)) then
+      ! device state left live by a lattice element: keep it, take the
+      ! module's undulator and taper
+      call gpu_check(puff_set_undulator(ctx_G, tParams_G%und_type, tParams_G%fx, tParams_G%fy, &
+                     tParams_G%kbeta_x, tParams_G%kbeta_y, tParams_G%szfs, tParams_G%szfe), &
+                     'puff_set_undulator')
+      call gpu_check(puff_set_taper(ctx_G, n2col, n2col0, undgrad), 'puff_set_taper')
+      qInnerXYOK_G = .true.
+      qPArrOK_G = .true.
+      return
+    end if
     call gpu_check(puff_create(ctx_G, tParams_G, n, iPrecision_G), 'puff_create')
     call gpu_check(puff_upload_particles(ctx_G, PUFF_MEM_HOST, sElX_G, sElY_G, sElZ2_G, &
                    sElPX_G, sElPY_G, sElGam_G, s_chi_bar_G), 'puff_upload_particles')
@@ -274,5 +348,62 @@ contains
     real(wp), intent(out) :: power(:)
     call gpu_check(puff_power(ctx_G, power), 'puff_power')
   end subroutine gPowerP
+
+  ! ---- lattice elements (acc_lattice.f90) on the device macroparticles
+
+  subroutine ensure_device()
+    if (.not. c_associated(ctx_G)) call allact_rk4_arrs()
+  end subroutine ensure_device
+
+  ! matchIn (acc_lattice.f90:743-828): undulator entry offsets at zbar sZ
+  ! with the current n2col, for tParams_G's undulator type.
+  subroutine matchIn(sZ)
+    real(wp), intent(in) :: sZ
+    call ensure_device()
+    call gpu_check(puff_match(ctx_G, sZ, n2col, 1_c_int32_t), 'puff_match')
+  end subroutine matchIn
+
+  ! matchOut (acc_lattice.f90:638-725): the exit offsets taken off again.
+  subroutine matchOut(sZ)
+    real(wp), intent(in) :: sZ
+    call ensure_device()
+    call gpu_check(puff_match(ctx_G, sZ, n2col, -1_c_int32_t), 'puff_match')
+  end subroutine matchOut
+
+  ! correctTrans (acc_lattice.f90:601-623): centre x, y, px, py on their
+  ! chi-weighted means.
+  subroutine correctTrans()
+    call ensure_device()
+    call gpu_check(puff_correct_trans(ctx_G), 'puff_correct_trans')
+  end subroutine correctTrans
+
+  ! driftSection's particle part (acc_lattice.f90:490-508); the caller
+  ! diffracts over del_dr_z (diffractIM) and advances sZ as :512-515 do.
+  subroutine gpu_drift(del_dr_z)
+    real(wp), intent(in) :: del_dr_z
+    call ensure_device()
+    call gpu_check(puff_drift(ctx_G, del_dr_z), 'puff_drift')
+  end subroutine gpu_drift
+
+  ! disperse's particle part with the periodic wrap (acc_lattice.f90:436-461).
+  subroutine gpu_disperse(disp, slip)
+    real(wp), intent(in) :: disp, slip
+    call ensure_device()
+    call gpu_check(puff_chicane(ctx_G, disp, slip), 'puff_chicane')
+  end subroutine gpu_disperse
+
+  ! Quad (acc_lattice.f90:544-554); no kick on a 1-D mesh.
+  subroutine gpu_quad(qfx, qfy)
+    real(wp), intent(in) :: qfx, qfy
+    call ensure_device()
+    call gpu_check(puff_quad(ctx_G, qfx, qfy), 'puff_quad')
+  end subroutine gpu_quad
+
+  ! bModulation (acc_lattice.f90:579-580).
+  subroutine gpu_bmodulation(wavenum, mag)
+    real(wp), intent(in) :: wavenum, mag
+    call ensure_device()
+    call gpu_check(puff_modulation(ctx_G, wavenum, mag), 'puff_modulation')
+  end subroutine gpu_bmodulation
 
 end module puffin_gpu

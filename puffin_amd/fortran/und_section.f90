@@ -1,13 +1,18 @@
 ! One undulator module driven from Fortran through the drop-in routines of
 ! module puffin_gpu: the split-step schedule of UndSection
 ! (undulator.f90:103-333) with rk4par, diffractIM, the periodic re-layout and
-! the power writes.
+! the power writes.  With ``elements`` set, the module is entered with
+! matchIn before the arrays go to the device and left as UndSection and the
+! element chain leave it (undulator.f90:86, 337-340; puffin_module.f90:58-90):
+! matchOut, correctTrans, then a drift (with its diffraction), a quad, a
+! chicane and an energy modulation, all on the device.
 !
 ! usage: und_section <input file> <output file>
 !
 ! Input (stream, native byte order): the puff_params record, then
-!   int64  n, nsteps, steps4diff, redist_every, diffraction, periodic, write_every
+!   int64  n, nsteps, steps4diff, redist_every, diffraction, periodic, write_every, elements
 !   real64 h, n2col, n2col0, undgrad
+!   real64 drift, quad_fx, quad_fy, chic_disp, chic_slip, enmod_wavenum, enmod_mag
 !   real64 x(n), y(n), z2(n), px(n), py(n), gam(n), chi(n), ar(nmesh), ai(nmesh)
 ! Output: int64 nwrites, then real64 x, y, z2, px, py, gam (n each), ar, ai
 ! (nmesh each), power(nz2, nwrites), and the int64 final status.
@@ -20,7 +25,8 @@ program und_section
   implicit none
 
   character(len=4096) :: fin, fout
-  integer(c_int64_t) :: n, nsteps, steps4diff, redist_every, idiff, iper, write_every
+  integer(c_int64_t) :: n, nsteps, steps4diff, redist_every, idiff, iper, write_every, ielem
+  real(wp) :: el(7)
   integer(c_int64_t) :: nmesh, istep, iCsteps, nw, status
   real(wp) :: h, szl, dzdS, dzdF, dzd
   real(wp), allocatable :: powers(:,:), pw(:)
@@ -36,8 +42,9 @@ program und_section
 
   open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
   read(u) tParams_G
-  read(u) n, nsteps, steps4diff, redist_every, idiff, iper, write_every
+  read(u) n, nsteps, steps4diff, redist_every, idiff, iper, write_every, ielem
   read(u) h, n2col, n2col0, undgrad
+  read(u) el
   nmesh = int(tParams_G%nx, c_int64_t) * tParams_G%ny * tParams_G%nz2
   allocate(sElX_G(n), sElY_G(n), sElZ2_G(n), sElPX_G(n), sElPY_G(n), sElGam_G(n), s_chi_bar_G(n))
   allocate(ac_rfield_in(nmesh), ac_ifield_in(nmesh))
@@ -52,6 +59,7 @@ program und_section
   szl = 0.0_wp
   qD = .true.
 
+  if (ielem /= 0) call matchIn(szl)                   ! undulator.f90:86, before the arrays
   call allact_rk4_arrs()
   if (iper /= 0) call wrap_z2()                       ! undulator.f90:103
   dzdS = 0.0_wp
@@ -73,7 +81,8 @@ program und_section
       exit
     end if
     szl = szl + h                                      ! undulator.f90:222
-    qWrite = (write_every > 0) .and. ((mod(iCsteps, write_every) == 0) .or. (iCsteps == nsteps))
+    qWrite = .false.                                   ! (.and. does not short-circuit)
+    if (write_every > 0) qWrite = (mod(iCsteps, write_every) == 0) .or. (iCsteps == nsteps)
     qDone = .false.
     if (idiff /= 0) then
       if ((mod(istep, steps4diff) == 0) .or. (istep == nsteps)) then   ! :229-289
@@ -108,6 +117,16 @@ program und_section
     end if
   end do
   call deallact_rk4_arrs()
+  if (ielem /= 0 .and. status == 0) then
+    call matchOut(szl)                                 ! undulator.f90:337
+    call correctTrans()                                ! :340
+    call gpu_drift(el(1))                              ! driftSection
+    call diffractIM(el(1), qDiffrctd, qOK)
+    call gpu_quad(el(2), el(3))                        ! Quad
+    call gpu_disperse(el(4), el(5))                    ! disperse (no zbar: no diffraction)
+    call gpu_bmodulation(el(6), el(7))                 ! bModulation
+    call deallact_rk4_arrs()
+  end if
 
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) nw

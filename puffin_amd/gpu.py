@@ -9,6 +9,7 @@ fallback: a missing library or device raises.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import os
 
 import numpy as np
@@ -42,6 +43,13 @@ SIGNATURES = {
     "puff_diffraction_step": (ctypes.c_int, [_vp, ctypes.c_double]),
     "puff_transform": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "puff_wrap_z2": (ctypes.c_int, [_vp]),
+    "puff_drift": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "puff_chicane": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "puff_quad": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "puff_modulation": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "puff_correct_trans": (ctypes.c_int, [_vp]),
+    "puff_match": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
+    "puff_set_undulator": (ctypes.c_int, [_vp, ctypes.c_int32] + [ctypes.c_double] * 6),
     "puff_power": (ctypes.c_int, [_vp, _dp]),
     "puff_bunching": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp]),
     "puff_particle_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
@@ -244,6 +252,35 @@ class Integrator:
 
     def wrap_z2(self):
         _check(lib().puff_wrap_z2(self._h), "puff_wrap_z2")
+
+    # ---- lattice elements (acc_lattice.f90)
+    def drift(self, dz: float):
+        """driftSection's particle part; the diffraction over dz is a separate call."""
+        _check(lib().puff_drift(self._h, dz), "puff_drift")
+
+    def chicane(self, disp: float, slip: float):
+        """disperse's particle part (slip in zbar units, as after readLatt)."""
+        _check(lib().puff_chicane(self._h, disp, slip), "puff_chicane")
+
+    def quad(self, fx: float, fy: float):
+        _check(lib().puff_quad(self._h, fx, fy), "puff_quad")
+
+    def modulation(self, wavenum: float, mag: float):
+        _check(lib().puff_modulation(self._h, wavenum, mag), "puff_modulation")
+
+    def correct_trans(self):
+        _check(lib().puff_correct_trans(self._h), "puff_correct_trans")
+
+    def match(self, sz: float, n2col: float, sign: int):
+        """matchIn (sign +1) or matchOut (sign -1) at zbar sz."""
+        _check(lib().puff_match(self._h, sz, n2col, int(sign)), "puff_match")
+
+    def set_undulator(self, und_type: int, fx: float, fy: float, kbeta_x: float = 0.0, kbeta_y: float = 0.0,
+                      szfs: float = 0.0, szfe: float = 0.0):
+        _check(lib().puff_set_undulator(self._h, int(und_type), fx, fy, kbeta_x, kbeta_y, szfs, szfe),
+               "puff_set_undulator")
+        self.mp = dataclasses.replace(self.mp, und_type=int(und_type), fx=fx, fy=fy, kbeta_x=kbeta_x,
+                                      kbeta_y=kbeta_y, szfs=szfs, szfe=szfe)
 
     # ---- instrumentation
     def profile(self, on: bool = True):

@@ -19,7 +19,7 @@ EXE = os.path.join(ROOT, "puffin_amd", "fortran", "und_section")
 KEYS = ("x", "y", "z2", "px", "py", "gam")
 
 
-def _case(periodic=1, n=2000):
+def _case(periodic=1, n=2000, matched=True):
     rho = 0.01
     eta, kappa, fx, fy = scaling(rho, 1.0, 100.0, 1 / np.sqrt(2), 1 / np.sqrt(2))
     h = step_size(rho, 30)
@@ -32,18 +32,21 @@ def _case(periodic=1, n=2000):
     p = dict(x=np.clip(rng.standard_normal(n), -2.5, 2.5), y=np.clip(rng.standard_normal(n), -2.5, 2.5),
              z2=rng.uniform(0.05, 0.6 * lz, n), px=np.zeros(n), py=np.zeros(n),
              gam=1.0 + 1e-3 * rng.standard_normal(n), chi=np.full(n, 2 * np.pi * lz / n))
-    match_in(mp, 1.0, p)
+    if matched:
+        match_in(mp, 1.0, p)
     ar, ai = gaussian_seed(mp, 1e-2, 1e-2, w0=1.5)
     return mp, p, (ar, ai), h
 
 
-def write_input(path, mp, p, field, h, nsteps, s4d, redist, write_every, taper=(1.0, 1.0, 0.0)):
+def write_input(path, mp, p, field, h, nsteps, s4d, redist, write_every, taper=(1.0, 1.0, 0.0),
+                elements=None):
     prm = mp.to_ctypes()
     with open(path, "wb") as f:
         f.write(ctypes.string_at(ctypes.addressof(prm), ctypes.sizeof(prm)))
-        f.write(np.array([p["x"].size, nsteps, s4d, redist, 1, mp.periodic, write_every],
-                         dtype=np.int64).tobytes())
+        f.write(np.array([p["x"].size, nsteps, s4d, redist, 1, mp.periodic, write_every,
+                          int(elements is not None)], dtype=np.int64).tobytes())
         f.write(np.array([h, taper[0], taper[1], taper[2]], dtype=np.float64).tobytes())
+        f.write(np.array(elements if elements is not None else [0.0] * 7, dtype=np.float64).tobytes())
         for k in KEYS + ("chi",):
             f.write(np.ascontiguousarray(p[k], dtype=np.float64).tobytes())
         f.write(np.ascontiguousarray(field[0], dtype=np.float64).tobytes())
@@ -95,3 +98,38 @@ def test_fortran_und_section_matches_oracle(tmp_path, periodic):
     assert np.abs(ai - o.ai).max() <= 1e-10 * scale
     assert powers.shape == opow.shape
     assert np.abs(powers - opow).max() <= 1e-10 * np.abs(opow).max()
+
+
+@pytest.mark.gpu
+def test_fortran_lattice_elements_match_oracle(tmp_path):
+    """matchIn before the arrays reach the device, the module, then matchOut,
+    correctTrans, drift (with its diffraction), quad, chicane and energy
+    modulation through the Fortran drop-ins."""
+    print("[gpu] fortran lattice elements", flush=True)
+    mp, p, field, h = _case(1, matched=False)
+    nsteps, s4d, redist = 30, 15, 20
+    lenz2 = mp.dz2 * (mp.nz2 - 1)
+    el = [0.2, 2.7, -3.1, 2.5e-3, 0.3 * lenz2, 7.0, 2e-3]
+    fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
+    write_input(fin, mp, p, field, h, nsteps, s4d, redist, 0, elements=el)
+    r = subprocess.run([EXE, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out, (ar, ai), _, status = read_output(fout, p["x"].size, mp)
+    assert status == 0
+    q = {k: v.copy() for k, v in p.items()}
+    orc.match(mp, 1.0, q, 0.0, +1)
+    o = orc.Oracle(mp, q, field)
+    st, _, _, szl = o.und_section(nsteps, h, True, s4d, redist)
+    assert st == 0
+    orc.match(mp, float(o.taper[0]), o.parts, szl, -1)
+    orc.correct_trans(o.parts)
+    orc.drift(mp, o.parts, el[0])
+    o.diffraction_step(el[0])
+    orc.quad(mp, o.parts, el[1], el[2])
+    orc.chicane(mp, o.parts, el[3], el[4])
+    orc.modulation(o.parts, el[5], el[6])
+    for k in KEYS:
+        assert np.abs(out[k] - o.parts[k]).max() <= 1e-10 * np.abs(o.parts[k]).max(), k
+    scale = np.abs(o.ar + 1j * o.ai).max()
+    assert np.abs(ar - o.ar).max() <= 1e-10 * scale
+    assert np.abs(ai - o.ai).max() <= 1e-10 * scale

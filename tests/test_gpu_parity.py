@@ -134,7 +134,8 @@ def test_diffraction_step_matches_oracle(periodic):
 # 3-D RK4 steps against the oracle
 # ---------------------------------------------------------------------------
 
-def _small_3d(periodic=0, n=3000, seed_amp=1e-2, und_ends=0, planar=False, nx=17, ny=17, nz2=None):
+def _small_3d(periodic=0, n=3000, seed_amp=1e-2, und_ends=0, planar=False, nx=17, ny=17, nz2=None,
+              matched=True):
     rho = 0.01
     ux, uy = (1.0, 0.0) if planar else (1 / math.sqrt(2), 1 / math.sqrt(2))
     eta, kappa, fx, fy = scaling(rho, 1.0, 100.0, ux, uy)
@@ -151,7 +152,8 @@ def _small_3d(periodic=0, n=3000, seed_amp=1e-2, und_ends=0, planar=False, nx=17
     p = dict(x=np.clip(rng.standard_normal(n), -2.5, 2.5), y=np.clip(rng.standard_normal(n), -2.5, 2.5),
              z2=rng.uniform(0.05, 0.6 * lz, n), px=np.zeros(n), py=np.zeros(n),
              gam=1.0 + 1e-3 * rng.standard_normal(n), chi=np.full(n, 2 * math.pi * lz / n))
-    match_in(mp, 1.0, p)
+    if matched:
+        match_in(mp, 1.0, p)
     ar, ai = gaussian_seed(mp, seed_amp, seed_amp, w0=1.5)
     return mp, p, (ar, ai), h
 
