@@ -1,5 +1,8 @@
 """Extract the reference's own 1-D end-to-end golden vectors into a small
-numpy fixture (tests/golden/f1main_expected.npz).
+numpy fixture (tests/golden/f1main_expected.npz), and copy the three HDF5
+files themselves beside it (f1main_expected_*.h5, 190 KB: data the reference's
+own test holds, used to check the Puffin-format HDF5 reader on files the
+reference wrote).
 
 Source: test/expected/f1main_expected_{electrons,field,integrated}.h5 of the
 reference, produced by its pFUnit test test/testMPIIntegration.pf:54-127 on
@@ -13,6 +16,7 @@ from __future__ import annotations
 
 import os
 import re
+import shutil
 import subprocess
 import sys
 
@@ -63,6 +67,9 @@ def main(ref_root: str) -> None:
              info_keys=np.array(list(info.keys())),
              info_vals=np.array(list(info.values())))
     print("wrote", out, electrons.shape, aperp.shape, power.shape, bunch.shape)
+    for f in (fe, ff, fi):
+        shutil.copyfile(f, os.path.join(HERE, os.path.basename(f)))
+        print("copied", os.path.basename(f))
 
 
 if __name__ == "__main__":

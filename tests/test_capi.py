@@ -49,3 +49,33 @@ def test_params_struct_layout_matches_header():
         names += [n.strip() for n in rest.split(",")]
     assert names == [f[0] for f in PuffParams._fields_]
     assert ctypes.sizeof(PuffParams) == 16 * 4 + 19 * 8
+
+
+H5_HEADER = os.path.join(ROOT, "include", "puffin_h5.h")
+
+
+def test_h5_library_exports_every_declared_symbol():
+    from puffin_amd import h5io
+    text = re.sub(r"/\*.*?\*/", "", open(H5_HEADER).read(), flags=re.S)
+    syms = sorted(set(re.findall(r"\b(puff_h5_[a-z0-9_]+)\s*\(", text)))
+    assert len(syms) == 9
+    lib = h5io.lib()
+    for s in syms:
+        assert hasattr(lib, s), s
+
+
+def test_h5_info_struct_layout_matches_header():
+    import ctypes
+    from puffin_amd.h5io import H5Info
+    text = open(H5_HEADER).read()
+    body = text.split("typedef struct puff_h5_info {", 1)[1].split("} puff_h5_info;", 1)[0]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        typ, rest = decl.split(None, 1)
+        names += [n.strip() for n in rest.split(",")]
+    assert names == [f[0] for f in H5Info._fields_]
+    assert ctypes.sizeof(H5Info) == 7 * 8 + 9 * 4 + 8 * 4 + 16 * 8 + 4

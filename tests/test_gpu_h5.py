@@ -1,0 +1,71 @@
+"""The reference's own 1-D end-to-end test (test/testMPIIntegration.pf:54-127)
+run wholly on the device: matchIn, 30 RK4 steps, matchOut and correctTrans
+through the C ABI, the outputs written as Puffin-format HDF5 files and
+compared with the files the reference wrote (read back through the same
+library) at the reference's own bar, 1e-10 absolute."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from f1main_case import NSTEPS, initial_particles, load_fixture, module_params
+from puffin_amd import gpu, h5io
+from puffin_amd.params import FP64, PI_REF, step_size
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KEYS7 = ("x", "y", "z2", "px", "py", "gam", "chi")
+
+
+def test_f1main_on_device_through_hdf5(tmp_path):
+    print("[gpu] f1main on device through HDF5", flush=True)
+    d, info = load_fixture()
+    ref, _ = h5io.read_electrons(os.path.join(GOLDEN, "f1main_expected_electrons.h5"))
+    rar, rai, rinfo = h5io.read_field(os.path.join(GOLDEN, "f1main_expected_field.h5"))
+    rpow = h5io.read_dataset(os.path.join(GOLDEN, "f1main_expected_integrated.h5"), "power")
+    rbun = h5io.read_dataset(os.path.join(GOLDEN, "f1main_expected_integrated.h5"), "bunchingFundamental")
+    mp = module_params()
+    parts = initial_particles(ref["chi"])
+    h = step_size(mp.rho, 30)
+    with gpu.Integrator(mp, parts["x"].size, FP64) as g:
+        g.upload_particles(parts)
+        g.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+        g.set_taper(1.0, 1.0, 0.0)
+        g.match(0.0, 1.0, +1)                      # matchIn (undulator.f90:86)
+        szl = 0.0
+        for _ in range(NSTEPS):
+            assert g.rk4par(szl, h) == 0
+            szl += h
+        n2col, _ = g.get_taper()
+        g.match(szl, n2col, -1)                    # matchOut (:347)
+        g.correct_trans()                          # correctTrans (:349)
+        out = g.download_particles()
+        ar, ai = g.download_field()
+        power = g.power()
+        bun = g.bunching(1, 32, 4 * PI_REF * mp.rho)    # one resonant wavelength per slice
+    out["chi"] = parts["chi"]
+    f = h5io.info_from_module(mp, time=szl, istep=NSTEPS + 1, icsteps=NSTEPS, n_steps=NSTEPS, step_size=h,
+                              lambda_w=rinfo.lambda_w, trans_area=rinfo.trans_area,
+                              counters=dict(iund_cr=2, ichic_cr=1, idrift_cr=1, iquad_cr=1, imodulation_cr=1,
+                                            il=1, iwrite_cr=2))
+    pe, pf, pi = (str(tmp_path / n) for n in ("electrons.h5", "field.h5", "integrated.h5"))
+    h5io.write_electrons(pe, out, f)
+    h5io.write_field(pf, ar, ai, f)
+    h5io.write_integrated(pi, f, power, bun[None, :])
+    mine, mi = h5io.read_electrons(pe)
+    order = np.argsort(mine["z2"])                 # the reference writes its slots in z2 order
+    for k in KEYS7:
+        assert np.abs(mine[k][order] - ref[k]).max() <= 1e-10, k
+    assert mi.zbar_total == pytest.approx(info["zbarTotal"], rel=1e-14)
+    far, fai, _ = h5io.read_field(pf)
+    assert np.abs(far - rar).max() <= 1e-10 and np.abs(fai - rai).max() <= 1e-10
+    mp_ = h5io.read_dataset(pi, "power")
+    assert np.abs(mp_ - rpow).max() <= 1e-10 * rpow.max()
+    # |b| = |sum chi exp(i z2 / 2 rho)| / sum chi moves by at most max|dz2| / (2 rho),
+    # so the 1e-10 bar on z2 allows 1e-10 / (2 rho) = 5e-9 on the bunching
+    db = np.abs(h5io.read_dataset(pi, "bunchingFundamental") - rbun).max()
+    zerr = np.abs(mine["z2"][order] - ref["z2"]).max()
+    assert db <= max(zerr, 1e-13) / (2 * mp.rho) and db <= 1e-10 / (2 * mp.rho)
