@@ -430,6 +430,26 @@ def c4_bench(steps: int = 30, adjoint_steps: int = 30, every: int = 4, captures:
     return out
 
 
+def copy_rate(mib: int = 1024, reps: int = 10) -> float:
+    """Achievable streaming rate (TB/s): a device-to-device copy of mib MiB,
+    read plus write bytes over the time, timed with events on torch's
+    stream (where the copy runs)."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    return 2.0 * n * 4 / (ms * 1e-3) / 1e12
+
+
 def calibrate_copy(mib: int):
     """A plain device-to-device copy of known size (reads and writes mib MiB
     each, 5 times), used to scale the FETCH_SIZE / WRITE_SIZE counters."""
@@ -539,6 +559,9 @@ def main():
         step_roofline=prim["step_roofline"],
         kernels=prim["kernels"],
     )
+    cr = copy_rate()
+    line["copy_rate"] = dict(TBps=cr, what="device copy of 1 GiB, (read + write) bytes / time")
+    line["roofline"]["frac_of_copy_rate"] = line["roofline"]["achieved"] / (cr * 1e3)
     if "mixed" in results and prim_name == "fp64":
         mx = summary("mixed", 1)
         perr = abs(mx["integrated_power"] - prim["integrated_power"]) / abs(prim["integrated_power"])

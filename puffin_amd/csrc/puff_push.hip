@@ -50,12 +50,26 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
             x = x0; y = y0; z2 = z20; pr = pr0; pi = pi0; g = g0;
         } else {
             const R c = (R)a.cin;
-            x = fma(c, a.kin.x[i], x0);
-            y = fma(c, a.kin.y[i], y0);
-            z2 = fma(a.cin, a.kin.z2[i], z20);
-            pr = fma(c, a.kin.pr[i], pr0);
-            pi = fma(c, a.kin.pi[i], pi0);
-            g = fma(a.cin, a.kin.g[i], g0);
+            const R k3x = a.kin.x[i], k3y = a.kin.y[i], k3pr = a.kin.pr[i], k3pi = a.kin.pi[i];
+            const double k3z2 = a.kin.z2[i], k3g = a.kin.g[i];
+            x = fma(c, k3x, x0);
+            y = fma(c, k3y, y0);
+            z2 = fma(a.cin, k3z2, z20);
+            pr = fma(c, k3pr, pr0);
+            pi = fma(c, k3pi, pi0);
+            g = fma(a.cin, k3g, g0);
+            if (STAGE == 4) {
+                // y0 + h/6 (k1 + 2 (k2 + k3)) formed now, with k1 and k2 loaded
+                // beside y0 and k3: the end of the kernel adds h/6 k4 only, and
+                // no k array is read after the deposit (puffin_mpi_RK4.f90:274-279)
+                const R h6 = (R)a.h6;
+                x0 = fma(h6, a.k1.x[i] + (R)2 * (a.k2.x[i] + k3x), x0);
+                y0 = fma(h6, a.k1.y[i] + (R)2 * (a.k2.y[i] + k3y), y0);
+                z20 = fma(a.h6, a.k1.z2[i] + 2.0 * (a.k2.z2[i] + k3z2), z20);
+                pr0 = fma(h6, a.k1.pr[i] + (R)2 * (a.k2.pr[i] + k3pr), pr0);
+                pi0 = fma(h6, a.k1.pi[i] + (R)2 * (a.k2.pi[i] + k3pi), pi0);
+                g0 = fma(a.h6, a.k1.g[i] + 2.0 * (a.k2.g[i] + k3g), g0);
+            }
         }
     }
 
@@ -93,10 +107,14 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // The field tile shares the deposit tile's LDS: it is used only for boxes
     // of at most FIELD_TILE_NODES nodes, whose deposit needs the first
     // 2 FIELD_TILE_NODES doubles, so it sits right after them.
+    // The deposit tile accumulates in FP64 in both forms.  An FP32 tile for
+    // the mixed form (ds_add_f32 in place of ds_add_f64) measured 3x slower
+    // per stage on the C1 beam.
     constexpr int TILE = stage_tile_nodes<R>();
-    static_assert(2 * FIELD_TILE_NODES * sizeof(double) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
-                      2 * TILE * sizeof(double), "field tile inside the deposit tile");
-    __shared__ double s_dep[2 * TILE];
+    using D = double;
+    static_assert(2 * FIELD_TILE_NODES * sizeof(D) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
+                      2 * TILE * sizeof(D), "field tile inside the deposit tile");
+    __shared__ D s_dep[2 * TILE];
     cplx<R>* s_fld = reinterpret_cast<cplx<R>*>(s_dep + 2 * FIELD_TILE_NODES);
     const R cA = (R)a.cin;
     const bool lds_field = use_tile && tb.tx * tb.ty * tb.tz <= FIELD_TILE_NODES;
@@ -183,15 +201,15 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
         const int nf = locate<R>(m, fma(cn, kx_, x0), fma(cn, ky_, y0), fma(a.cnext, kz2, z20), cc);
         if (nf) atomicOr(a.flags, nf);
     } else {
-        // y += h/6 (k1 + k4 + 2 (k2 + k3))  (puffin_mpi_RK4.f90:274-279, 307-312)
+        // y = (y0 + h/6 (k1 + 2 (k2 + k3))) + h/6 k4  (puffin_mpi_RK4.f90:274-279, 307-312)
         if (rejected) return;   // the step is rejected
         const R h6 = (R)a.h6;
-        a.y0.x[i] = x0 + h6 * (a.k1.x[i] + kx_ + (R)2 * (a.k2.x[i] + a.kin.x[i]));
-        a.y0.y[i] = y0 + h6 * (a.k1.y[i] + ky_ + (R)2 * (a.k2.y[i] + a.kin.y[i]));
-        a.y0.z2[i] = z20 + a.h6 * (a.k1.z2[i] + kz2 + 2.0 * (a.k2.z2[i] + a.kin.z2[i]));
-        a.y0.pr[i] = pr0 + h6 * (a.k1.pr[i] + kpr + (R)2 * (a.k2.pr[i] + a.kin.pr[i]));
-        a.y0.pi[i] = pi0 + h6 * (a.k1.pi[i] + kpi + (R)2 * (a.k2.pi[i] + a.kin.pi[i]));
-        a.y0.g[i] = g0 + a.h6 * (a.k1.g[i] + kg + 2.0 * (a.k2.g[i] + a.kin.g[i]));
+        a.y0.x[i] = fma(h6, kx_, x0);
+        a.y0.y[i] = fma(h6, ky_, y0);
+        a.y0.z2[i] = fma(a.h6, kz2, z20);
+        a.y0.pr[i] = fma(h6, kpr, pr0);
+        a.y0.pi[i] = fma(h6, kpi, pi0);
+        a.y0.g[i] = fma(a.h6, kg, g0);
     }
 }
 

@@ -267,30 +267,30 @@ __device__ __forceinline__ int corner_tile(const MeshGeom& m, const TileBox& t, 
 
 // out[node_k] += w_k (sr, si) for every lane with a cell (sr = si = 0 on
 // lanes without one).
-//   use_tile: FP64 accumulation in an LDS tile over the workgroup's node box
-//             (real parts, then imaginary parts), then one global atomic per
-//             touched node and component.  Each lane visits the corners in a
+//   use_tile: accumulation in an LDS tile over the workgroup's node box
+//             (real parts, then imaginary parts; element type D), then one
+//             global atomic per touched node and component.  Each lane visits the corners in a
 //             rotated order so that lanes of one cell do not hit the same LDS
 //             address in the same instruction.
 //   else:     lanes of a wavefront that share a cell are summed first (runs
 //             of equal keys), then one global atomic per run and corner.
-// s_tile: the LDS tile, 2 doubles per node of the box.  use_tile must be
-// block-uniform; every thread of the workgroup must call.
-template <typename R>
+// s_tile: the LDS tile, 2 elements of type D per node of the box.  use_tile
+// must be block-uniform; every thread of the workgroup must call.
+template <typename R, typename D>
 __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& t, bool use_tile,
                                               bool ok, const Cell<R>& c, const CellRef& cr, R sr,
-                                              R si, cplx<R>* out, double* s_tile) {
+                                              R si, cplx<R>* out, D* s_tile) {
     const int lane = threadIdx.x & 63;
     const int nnb = m.one_d ? 2 : 8;
     if (use_tile) {
         const int nt = t.tx * t.ty * t.tz;
-        for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) s_tile[q] = 0.0;
+        for (int q = threadIdx.x; q < 2 * nt; q += blockDim.x) s_tile[q] = (D)0;
         __syncthreads();
         if (ok) {
-            const double dsr = (double)sr, dsi = (double)si;
+            const D dsr = (D)sr, dsi = (D)si;
             for (int q = 0; q < nnb; ++q) {
                 const int k = (q + lane) & (nnb - 1);
-                const double wk = (double)corner_weight<R>(m, c, k);
+                const D wk = (D)corner_weight<R>(m, c, k);
                 const int tk = corner_tile(m, t, cr, k);
                 atomicAdd(&s_tile[tk], wk * dsr);
                 atomicAdd(&s_tile[nt + tk], wk * dsi);
@@ -300,8 +300,8 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
         const int ntr = m.nx * m.ny;
         const BoxDiv bd = box_div(t);
         for (int q = threadIdx.x; q < nt; q += blockDim.x) {
-            const double tr = s_tile[q], ti = s_tile[nt + q];
-            if (tr != 0.0 || ti != 0.0) {
+            const D tr = s_tile[q], ti = s_tile[nt + q];
+            if (tr != (D)0 || ti != (D)0) {
                 int lx, ly, lz;
                 box_coords(bd, q, lx, ly, lz);
                 const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
