@@ -225,6 +225,24 @@ int puff_power(puff_ctx* ctx, double* power_host);
 int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices,
                   double slice_len, double* b_host);
 
+/* Slice statistics of getSliceTwiss (av_write.f90:405-731) over nslices
+   slices of width slice_len, slice = ceil(z2 / slice_len) with the
+   reference's periodic wrap.  out_host has 13 * nslices doubles,
+   out[k * nslices + s] for k = 0 slice charge (sum of chi-bar), 1 mean x,
+   2 mean y, 3 mean px, 4 mean py (of the stored -py, as the reference),
+   5 r.m.s. x, 6 r.m.s. y, 7 r.m.s. px, 8 r.m.s. py, 9 x emittance,
+   10 y emittance (scaled, before the 2 kappa / sqrt(eta) / Gamma factor of
+   hdf5_puff.f90:330-345), 11 mean Gamma, 12 r.m.s. Gamma.  Empty slices are
+   zero. */
+int puff_slice_stats(puff_ctx* ctx, int32_t nslices, double slice_len, double* out_host);
+
+/* Beam current nodes of getCurr (av_write.f90:297-368): chi-bar binned
+   linearly onto npts nodes spaced sam_len apart (node j at (j - 1) sam_len),
+   before the reference's unit factors npk_bar (x ata in 1-D) x q_e /
+   sam_len x c / Lc.  A macroparticle outside the nodes (the reference
+   STOPs) returns PUFF_ERR_STATE. */
+int puff_current(puff_ctx* ctx, int32_t npts, double sam_len, double* out_host);
+
 /* Number of macroparticles of the context. */
 int puff_particle_count(puff_ctx* ctx, int64_t* n);
 

@@ -106,6 +106,13 @@ int puff_h5_read_field(const char* path, int64_t nmesh, double* ar, double* ai);
 int puff_h5_write_integrated(const char* path, const puff_h5_info* info, int32_t nz2, const double* power,
                              int32_t nharm, int32_t nslices, const double* bunching);
 
+/* Add a 1-D FP64 dataset `name` (with the position attributes of info) to
+   an existing file, such as the slice statistics and beam current of the
+   integrated file (meanGamma, sigmaXbar, beamCurrent, ...,
+   hdf5_puff.f90:229-416).  An existing dataset of that name is an error. */
+int puff_h5_append_1d(const char* path, const puff_h5_info* info, const char* name, int64_t n,
+                      const double* v);
+
 /* Any 1-D FP64 dataset by name: *len receives its length; out (capacity
    cap) is filled when not null and cap >= *len. */
 int puff_h5_read_dataset_1d(const char* path, const char* name, int64_t cap, double* out, int64_t* len);

@@ -59,6 +59,10 @@ def lib():
         l.orc_bunching.restype = ctypes.c_int
         l.orc_bunching.argtypes = [P, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, _dp]
+        l.orc_slice_stats.restype = ctypes.c_int
+        l.orc_slice_stats.argtypes = [P, ctypes.c_int64] + [_dp] * 7 + [ctypes.c_int, ctypes.c_double, _dp]
+        l.orc_current.restype = ctypes.c_int
+        l.orc_current.argtypes = [P, ctypes.c_int64, _dp, _dp, ctypes.c_int, ctypes.c_double, _dp]
         l.orc_match.argtypes = [P, ctypes.c_double, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp,
                                 ctypes.c_double, ctypes.c_int]
         l.orc_set_deposit_atomic.argtypes = [ctypes.c_int]
@@ -138,6 +142,14 @@ class Oracle:
         lib().orc_bunching(self.pp, self.n, _ptr(self.parts["z2"]), _ptr(self.parts["chi"]),
                            harmonic, nslices, slice_len, _ptr(out))
         return out
+
+    def slice_stats(self, nslices: int, slice_len: float):
+        """getSliceTwiss's per-slice outputs: (status, dict of arrays)."""
+        return slice_stats(self.mp, self.parts, nslices, slice_len)
+
+    def current(self, npts: int, sam_len: float):
+        """getCurr's binned charge (before unit scalings): (status, array)."""
+        return current(self.mp, self.parts, npts, sam_len)
 
     def und_section(self, nsteps: int, h: float, diffraction: bool, steps4diff: int,
                     redist_every: int, csteps: int = 0, write_mask=None, sz_local: float = 0.0):
@@ -235,6 +247,30 @@ def match(mp: ModuleParams, n2col: float, parts: dict, sz: float, sign: int):
     n = parts["x"].size
     lib().orc_match(ctypes.byref(p), n2col, n, _ptr(parts["x"]), _ptr(parts["y"]),
                     _ptr(parts["px"]), _ptr(parts["py"]), _ptr(parts["gam"]), sz, sign)
+
+
+SLICE_KEYS = ("sdata", "aveX", "aveY", "avePX", "avePY", "sdX", "sdY", "sdpX", "sdpY", "eX", "eY",
+              "aveGamma", "aveDgamma")
+
+
+def slice_stats(mp: ModuleParams, parts: dict, nslices: int, slice_len: float):
+    p = mp.to_ctypes()
+    out = np.zeros(13 * nslices)
+    P = {k: np.ascontiguousarray(parts[k], dtype=np.float64) for k in ("x", "y", "z2", "px", "py", "gam", "chi")}
+    st = lib().orc_slice_stats(ctypes.byref(p), P["x"].size, *[_ptr(P[k]) for k in ("x", "y", "z2", "px", "py",
+                                                                                       "gam", "chi")],
+                               nslices, slice_len, _ptr(out))
+    o = out.reshape(13, nslices)
+    return st, {k: o[i].copy() for i, k in enumerate(SLICE_KEYS)}
+
+
+def current(mp: ModuleParams, parts: dict, npts: int, sam_len: float):
+    p = mp.to_ctypes()
+    out = np.zeros(npts)
+    z2 = np.ascontiguousarray(parts["z2"], dtype=np.float64)
+    chi = np.ascontiguousarray(parts["chi"], dtype=np.float64)
+    st = lib().orc_current(ctypes.byref(p), z2.size, _ptr(z2), _ptr(chi), npts, sam_len, _ptr(out))
+    return st, out
 
 
 def set_deposit_atomic(on: bool):

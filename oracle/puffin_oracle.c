@@ -1015,6 +1015,106 @@ int orc_bunching(const puff_params* p, int64_t n, const double* z2,
     return status;
 }
 
+/* Slice statistics of getSliceTwiss (io/av_write.f90:405-731): the charge,
+   means, r.m.s. widths, emittances, mean Gamma and energy spread of each
+   slice of length `slice`.  Slice index is = ceil(z2 / slice) with the
+   periodic wrap of :507-513; out-of-range macroparticles are skipped
+   (:514-518).  The p_y sums carry the stored sign of py (-py, :530-536).
+   out[k * nslices + s] for k = 0 sdata, 1 aveX, 2 aveY, 3 avePX, 4 avePY,
+   5 sdX, 6 sdY, 7 sdpX, 8 sdpY, 9 eX, 10 eY, 11 aveGamma, 12 aveDgamma
+   (:664-731, negative variances clipped to zero before the square roots). */
+int orc_slice_stats(const puff_params* p, int64_t n, const double* x, const double* y,
+                    const double* z2, const double* px, const double* py, const double* gam,
+                    const double* chi, int nslices, double slice, double* out) {
+    enum { S, MX, MY, MPX, MPY, MG, MXX, MXPX, MPXPX, MPYPY, MYPY, MYY, MGG, NS };
+    double* acc = (double*)calloc((size_t)NS * nslices, sizeof(double));
+    int status = PUFF_OK;
+    for (int64_t i = 0; i < n; i++) {
+        int is = (int)ceil(z2[i] / slice);
+        if (p->periodic) {
+            if (is >= nslices) is = is - ((int)floor((double)is / (double)nslices) * nslices) + 1;
+            if (is < 1) { status = PUFF_ERR_ARG; continue; }
+        } else {
+            if ((is > nslices) || (is < 1)) { status = PUFF_ERR_ARG; continue; }
+        }
+        double* a = acc + (is - 1);
+        const double c = chi[i];
+        a[S * nslices] += c;
+        a[MX * nslices] += c * x[i];
+        a[MY * nslices] += c * y[i];
+        a[MPX * nslices] += c * px[i];
+        a[MPY * nslices] -= c * py[i];
+        a[MG * nslices] += c * gam[i];
+        a[MXX * nslices] += c * x[i] * x[i];
+        a[MXPX * nslices] += c * x[i] * px[i];
+        a[MYY * nslices] += c * y[i] * y[i];
+        a[MYPY * nslices] -= c * y[i] * py[i];
+        a[MPXPX * nslices] += c * px[i] * px[i];
+        a[MPYPY * nslices] += c * py[i] * py[i];
+        a[MGG * nslices] += c * gam[i] * gam[i];
+    }
+    for (int k = 0; k < 13 * nslices; k++) out[k] = 0.0;
+    for (int is = 0; is < nslices; is++) {
+        double* o = out + is;
+        const double sd = acc[S * nslices + is];
+        o[0] = sd;
+        if (!(sd > 0.0)) continue;
+#define A(k) acc[(k) * nslices + is]
+        const double ax = A(MX) / sd, ay = A(MY) / sd, apx = A(MPX) / sd, apy = A(MPY) / sd;
+        double sdx = (A(MXX) / sd) - pow(A(MX) / sd, 2.0);
+        double sdy = (A(MYY) / sd) - pow(A(MY) / sd, 2.0);
+        double sdpx = (A(MPXPX) / sd) - pow(A(MPX) / sd, 2.0);
+        double sdpy = (A(MPYPY) / sd) - pow(A(MPY) / sd, 2.0);
+        double adg = (A(MGG) / sd) - pow(A(MG) / sd, 2.0);
+        double ex = sdx * sdpx - pow(A(MXPX) / sd - (ax * apx), 2.0);
+        double ey = sdy * sdpy - pow(A(MYPY) / sd - (ay * apy), 2.0);
+#undef A
+        o[1 * nslices] = ax;
+        o[2 * nslices] = ay;
+        o[3 * nslices] = apx;
+        o[4 * nslices] = apy;
+        o[5 * nslices] = sqrt(sdx < 0.0 ? 0.0 : sdx);
+        o[6 * nslices] = sqrt(sdy < 0.0 ? 0.0 : sdy);
+        o[7 * nslices] = sqrt(sdpx < 0.0 ? 0.0 : sdpx);
+        o[8 * nslices] = sqrt(sdpy < 0.0 ? 0.0 : sdpy);
+        o[9 * nslices] = sqrt(ex < 0.0 ? 0.0 : ex);
+        o[10 * nslices] = sqrt(ey < 0.0 ? 0.0 : ey);
+        o[11 * nslices] = acc[MG * nslices + is] / sd;
+        o[12 * nslices] = sqrt(adg < 0.0 ? 0.0 : adg);
+    }
+    free(acc);
+    return status;
+}
+
+/* Charge binned linearly onto npts nodes spaced sam apart (getCurr,
+   io/av_write.f90:297-368, before its unit scalings): node inl =
+   ceil(z2 / sam) takes (1 - f) chi and node inl + 1 takes f chi, with f the
+   fractional position; the periodic wrap of :317-327.  A macroparticle
+   outside the nodes is the reference's STOP: status PUFF_ERR_ARG. */
+int orc_current(const puff_params* p, int64_t n, const double* z2, const double* chi, int npts,
+                double sam, double* out) {
+    int status = PUFF_OK;
+    for (int k = 0; k < npts; k++) out[k] = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        const int inl = (int)ceil(z2[i] / sam);
+        const int inu = inl + 1;
+        int lo = inl, hi = inu;
+        if (p->periodic) {
+            if (inu >= npts) hi = inu - (int)floor((double)inu / (double)npts) * npts + 1;
+            if (inl >= npts) lo = inl - (int)floor((double)inl / (double)npts) * npts + 1;
+            if (inl <= 0) { status = PUFF_ERR_ARG; continue; }
+        } else {
+            if ((inu > npts) || (inl <= 0)) { status = PUFF_ERR_ARG; continue; }
+        }
+        const double locz2 = z2[i] - (double)(inl - 1) * sam;
+        const double li2 = locz2 / sam;
+        const double li1 = 1.0 - li2;
+        out[lo - 1] = li1 * chi[i] + out[lo - 1];
+        out[hi - 1] = li2 * chi[i] + out[hi - 1];
+    }
+    return status;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Undulator entry / exit matching and the exit corrector, used to rebuild    */
 /* the reference's own 1-D test run.                                         */

@@ -1036,6 +1036,75 @@ static int bunching(Ctx* c, int harmonic, int nslices, double slice, double* out
     return PUFF_OK;
 }
 
+// getSliceTwiss's per-slice outputs (av_write.f90:664-731) from the device
+// sums: out[k * nslices + s] for k = 0 charge, 1-2 mean x, y, 3-4 mean px,
+// py, 5-6 r.m.s. x, y, 7-8 r.m.s. px, py, 9-10 emittances, 11 mean Gamma,
+// 12 r.m.s. Gamma; negative variances clipped to zero.
+template <typename R>
+static int slice_stats(Ctx* c, int nslices, double slice, double* out) {
+    if (nslices < 1 || (int64_t)NSLICE_SUMS * nslices > c->diag_len || !(slice > 0.0)) {
+        set_error("nslices must be in [1, " + std::to_string(c->diag_len / NSLICE_SUMS) + "] and slice_len > 0");
+        return PUFF_ERR_ARG;
+    }
+    Particles<R>& P = parts<R>(c);
+    launch_fill_zero(c->diag, (int64_t)NSLICE_SUMS * nslices, c->stream);
+    launch_slice_moments<R>(P.y, P.chi, c->n, nslices, slice, c->p.periodic, c->diag, c->stream);
+    PUFF_HIP(hipGetLastError());
+    std::vector<double> a((size_t)NSLICE_SUMS * nslices);
+    PUFF_HIP(hipMemcpyAsync(a.data(), c->diag, sizeof(double) * a.size(), hipMemcpyDeviceToHost, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 13 * nslices; ++k) out[k] = 0.0;
+    auto root = [](double v) { return std::sqrt(v < 0.0 ? 0.0 : v); };
+    for (int is = 0; is < nslices; ++is) {
+        auto A = [&](int k) { return a[(size_t)k * nslices + is]; };
+        const double sd = A(0);
+        out[is] = sd;
+        if (!(sd > 0.0)) continue;
+        const double ax = A(1) / sd, ay = A(2) / sd, apx = A(3) / sd, apy = A(4) / sd;
+        const double vx = A(6) / sd - std::pow(A(1) / sd, 2.0), vy = A(11) / sd - std::pow(A(2) / sd, 2.0);
+        const double vpx = A(8) / sd - std::pow(A(3) / sd, 2.0), vpy = A(9) / sd - std::pow(A(4) / sd, 2.0);
+        const double vg = A(12) / sd - std::pow(A(5) / sd, 2.0);
+        const double ex = vx * vpx - std::pow(A(7) / sd - ax * apx, 2.0);
+        const double ey = vy * vpy - std::pow(A(10) / sd - ay * apy, 2.0);
+        double* o = out + is;
+        o[1 * nslices] = ax;
+        o[2 * nslices] = ay;
+        o[3 * nslices] = apx;
+        o[4 * nslices] = apy;
+        o[5 * nslices] = root(vx);
+        o[6 * nslices] = root(vy);
+        o[7 * nslices] = root(vpx);
+        o[8 * nslices] = root(vpy);
+        o[9 * nslices] = root(ex);
+        o[10 * nslices] = root(ey);
+        o[11 * nslices] = A(5) / sd;
+        o[12 * nslices] = root(vg);
+    }
+    return PUFF_OK;
+}
+
+// getCurr's binned charge (av_write.f90:297-368) before its unit scalings.
+template <typename R>
+static int current(Ctx* c, int npts, double sam, double* out) {
+    if (npts < 2 || npts + 1 > c->diag_len || !(sam > 0.0)) {
+        set_error("npts must be in [2, " + std::to_string(c->diag_len - 1) + "] and sam_len > 0");
+        return PUFF_ERR_ARG;
+    }
+    Particles<R>& P = parts<R>(c);
+    launch_fill_zero(c->diag, (int64_t)npts + 1, c->stream);
+    launch_current<R>(P.y.z2, P.chi, c->n, npts, sam, c->p.periodic, c->diag, c->stream);
+    PUFF_HIP(hipGetLastError());
+    std::vector<double> a((size_t)npts + 1);
+    PUFF_HIP(hipMemcpyAsync(a.data(), c->diag, sizeof(double) * a.size(), hipMemcpyDeviceToHost, c->stream));
+    PUFF_HIP(hipStreamSynchronize(c->stream));
+    if (a[(size_t)npts] > 0.0) {
+        set_error(std::to_string((int64_t)a[(size_t)npts]) + " macroparticles lie outside the current nodes");
+        return PUFF_ERR_STATE;
+    }
+    for (int k = 0; k < npts; ++k) out[k] = a[(size_t)k];
+    return PUFF_OK;
+}
+
 // ---- discrete adjoint ---------------------------------------------------------
 
 static int need_adj(const Ctx* c) {
@@ -1929,6 +1998,18 @@ int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices, double slice
     CTX_CHECK(ctx);
     if (!out || !(slice_len > 0)) return PUFF_ERR_ARG;
     return DISPATCH(ctx, bunching, harmonic, nslices, slice_len, out);
+}
+
+int puff_slice_stats(puff_ctx* ctx, int32_t nslices, double slice_len, double* out) {
+    CTX_CHECK(ctx);
+    if (!out) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, slice_stats, nslices, slice_len, out);
+}
+
+int puff_current(puff_ctx* ctx, int32_t npts, double sam_len, double* out) {
+    CTX_CHECK(ctx);
+    if (!out) return PUFF_ERR_ARG;
+    return DISPATCH(ctx, current, npts, sam_len, out);
 }
 
 int puff_profile_enable(puff_ctx* ctx, int32_t on) {

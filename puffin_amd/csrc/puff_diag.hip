@@ -3,6 +3,8 @@
 //   power    gPowerP / fPower_3D / fPower_1D, io/av_write.f90:39-202
 //   bunching getSliceTwiss, io/av_write.f90:405-773 (harmonic sums 562-650,
 //            magnitudes 725-729)
+//   slice moments  getSliceTwiss's per-slice sums (av_write.f90:505-536)
+//   current  getCurr's linear binning of the charge (av_write.f90:297-368)
 #include "puff_common.hpp"
 #include "puff_kernels.hpp"
 
@@ -76,6 +78,80 @@ __global__ __launch_bounds__(256) void k_bunching(const double* z2, const R* chi
     __syncthreads();
     for (int t = threadIdx.x; t < 3 * nslices; t += blockDim.x)
         if (sh[t] != 0.0) atomicAdd(&acc[t], sh[t]);
+}
+
+// Per-slice sums of getSliceTwiss for the slices [s0, s0 + nch) of nslices
+// (av_write.f90:505-536), in LDS then one global atomic per slice and sum:
+// acc[k * nslices + slice] for the NSLICE_SUMS sums, in the order
+// chi, chi x, chi y, chi px, -chi py, chi Gamma, chi x^2, chi x px,
+// chi px^2, chi py^2, -chi y py, chi y^2, chi Gamma^2 (py is stored as -py,
+// hence the signs).  Slice index and periodic wrap as k_bunching.
+template <typename R>
+__global__ __launch_bounds__(256) void k_slice_moments(PartPtrs<R> y, const R* chi, int64_t n, int nslices,
+                                                       double slice, int periodic, int s0, int nch,
+                                                       double* acc) {
+    extern __shared__ double sh[];
+    for (int t = threadIdx.x; t < NSLICE_SUMS * nch; t += blockDim.x) sh[t] = 0.0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double z = y.z2[i];
+        int is = (int)ceil(z / slice);
+        if (periodic) {
+            if (is >= nslices) is = is - ((int)floor((double)is / (double)nslices) * nslices) + 1;
+            if (is < 1) continue;
+        } else {
+            if (is > nslices || is < 1) continue;
+        }
+        const int l = is - 1 - s0;
+        if (l < 0 || l >= nch) continue;
+        const double c = (double)chi[i], x = (double)y.x[i], yy = (double)y.y[i];
+        const double px = (double)y.pr[i], py = (double)y.pi[i], g = y.g[i];
+        const double v[NSLICE_SUMS] = {c, c * x, c * yy, c * px, -(c * py), c * g, c * x * x, c * x * px,
+                                       c * px * px, c * py * py, -(c * yy * py), c * yy * yy, c * g * g};
+#pragma unroll
+        for (int k = 0; k < NSLICE_SUMS; ++k) atomicAdd(&sh[k * nch + l], v[k]);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NSLICE_SUMS * nch; t += blockDim.x)
+        if (sh[t] != 0.0) atomicAdd(&acc[(t / nch) * nslices + s0 + (t % nch)], sh[t]);
+}
+
+// Charge binned linearly onto npts nodes spaced sam apart (getCurr,
+// av_write.f90:297-368): nodes [p0, p0 + nch) of this launch, in LDS.  A
+// macroparticle outside the nodes (the reference STOPs) adds 1 to
+// acc[npts].
+template <typename R>
+__global__ __launch_bounds__(256) void k_current(const double* z2, const R* chi, int64_t n, int npts,
+                                                 double sam, int periodic, int p0, int nch, double* acc) {
+    extern __shared__ double sh[];
+    for (int t = threadIdx.x; t < nch; t += blockDim.x) sh[t] = 0.0;
+    __syncthreads();
+    double bad = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double z = z2[i];
+        const int inl = (int)ceil(z / sam);
+        const int inu = inl + 1;
+        int lo = inl, hi = inu;
+        if (periodic) {
+            if (inu >= npts) hi = inu - (int)floor((double)inu / (double)npts) * npts + 1;
+            if (inl >= npts) lo = inl - (int)floor((double)inl / (double)npts) * npts + 1;
+            if (inl <= 0) { bad += 1.0; continue; }
+        } else {
+            if (inu > npts || inl <= 0) { bad += 1.0; continue; }
+        }
+        const double li2 = (z - (double)(inl - 1) * sam) / sam;
+        const double li1 = 1.0 - li2;
+        const double c = (double)chi[i];
+        if (lo - 1 - p0 >= 0 && lo - 1 - p0 < nch) atomicAdd(&sh[lo - 1 - p0], li1 * c);
+        if (hi - 1 - p0 >= 0 && hi - 1 - p0 < nch) atomicAdd(&sh[hi - 1 - p0], li2 * c);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < nch; t += blockDim.x)
+        if (sh[t] != 0.0) atomicAdd(&acc[p0 + t], sh[t]);
+    bad = wave_sum(bad);
+    if (p0 == 0 && (threadIdx.x & 63) == 0 && bad > 0.0) atomicAdd(&acc[npts], bad);
 }
 
 template <typename R>
@@ -167,6 +243,33 @@ void launch_bunching(const double* z2, const R* chi, int64_t n, double rho, int 
 }
 
 template <typename R>
+void launch_slice_moments(PartPtrs<R> y, const R* chi, int64_t n, int nslices, double slice, int periodic,
+                          double* acc, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    for (int s0 = 0; s0 < nslices; s0 += SLICE_CHUNK) {
+        const int nch = nslices - s0 < SLICE_CHUNK ? nslices - s0 : SLICE_CHUNK;
+        hipLaunchKernelGGL((k_slice_moments<R>), dim3((unsigned)blocks), dim3(256),
+                           (size_t)NSLICE_SUMS * nch * sizeof(double), s, y, chi, n, nslices, slice, periodic,
+                           s0, nch, acc);
+    }
+}
+
+template <typename R>
+void launch_current(const double* z2, const R* chi, int64_t n, int npts, double sam, int periodic,
+                    double* acc, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks < 1) blocks = 1;
+    for (int p0 = 0; p0 < npts; p0 += CURRENT_CHUNK) {
+        const int nch = npts - p0 < CURRENT_CHUNK ? npts - p0 : CURRENT_CHUNK;
+        hipLaunchKernelGGL((k_current<R>), dim3((unsigned)blocks), dim3(256), (size_t)nch * sizeof(double), s,
+                           z2, chi, n, npts, sam, periodic, p0, nch, acc);
+    }
+}
+
+template <typename R>
 void launch_pack_field(const double* ar, const double* ai, cplx<R>* a, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL((k_pack<R>), grid_for(n), dim3(256), 0, s, ar, ai, a, n);
 }
@@ -188,6 +291,10 @@ void launch_convert_back(const R* src, double* dst, int64_t n, hipStream_t s) {
                                   hipStream_t);                                                   \
     template void launch_bunching<R>(const double*, const R*, int64_t, double, int, int, double,  \
                                      int, double*, hipStream_t);                                  \
+    template void launch_slice_moments<R>(PartPtrs<R>, const R*, int64_t, int, double, int,       \
+                                          double*, hipStream_t);                                  \
+    template void launch_current<R>(const double*, const R*, int64_t, int, double, int, double*,  \
+                                    hipStream_t);                                                 \
     template void launch_pack_field<R>(const double*, const double*, cplx<R>*, int64_t,           \
                                        hipStream_t);                                              \
     template void launch_unpack_field<R>(const cplx<R>*, double*, double*, int64_t, hipStream_t); \

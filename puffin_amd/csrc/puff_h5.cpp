@@ -376,6 +376,17 @@ int puff_h5_write_integrated(const char* path, const puff_h5_info* info, int32_t
     return H5Fflush(f, H5F_SCOPE_LOCAL) >= 0 ? PUFF_H5_OK : fail(PUFF_H5_ERR_IO, "flush failed");
 }
 
+int puff_h5_append_1d(const char* path, const puff_h5_info* info, const char* name, int64_t n,
+                      const double* v) {
+    Quiet q;
+    if (!path || !info || !name || n < 1 || !v) return fail(PUFF_H5_ERR_ARG, "puff_h5_append_1d: bad arguments");
+    Id f(H5Fopen(path, H5F_ACC_RDWR, H5P_DEFAULT), H5Fclose);
+    if (!f.ok()) return fail(PUFF_H5_ERR_IO, std::string("cannot open ") + path + " for writing");
+    if (H5Lexists(f, name, H5P_DEFAULT) > 0) return fail(PUFF_H5_ERR_FORMAT, std::string(name) + " exists");
+    if (!write_1d(f, name, v, (hsize_t)n, *info)) return fail(PUFF_H5_ERR_IO, std::string("writing ") + name + " failed");
+    return H5Fflush(f, H5F_SCOPE_LOCAL) >= 0 ? PUFF_H5_OK : fail(PUFF_H5_ERR_IO, "flush failed");
+}
+
 int puff_h5_read_dataset_1d(const char* path, const char* name, int64_t cap, double* out, int64_t* len) {
     Quiet q;
     if (!path || !name || !len) return fail(PUFF_H5_ERR_ARG, "puff_h5_read_dataset_1d: bad arguments");

@@ -276,6 +276,19 @@ void launch_power(const cplx<R>* a, int nx, int ny, int nz2, double dx, double d
 template <typename R>
 void launch_bunching(const double* z2, const R* chi, int64_t n, double rho, int harmonic,
                      int nslices, double slice, int periodic, double* acc, hipStream_t s);
+// getSliceTwiss's per-slice sums: acc holds NSLICE_SUMS x nslices doubles,
+// zeroed by the caller; launched in chunks of SLICE_CHUNK slices (LDS).
+constexpr int NSLICE_SUMS = 13;
+constexpr int SLICE_CHUNK = 448;
+template <typename R>
+void launch_slice_moments(PartPtrs<R> y, const R* chi, int64_t n, int nslices, double slice, int periodic,
+                          double* acc, hipStream_t s);
+// getCurr's binned charge: acc holds npts + 1 doubles (the last counts
+// macroparticles outside the nodes), zeroed by the caller.
+constexpr int CURRENT_CHUNK = 4096;
+template <typename R>
+void launch_current(const double* z2, const R* chi, int64_t n, int npts, double sam, int periodic,
+                    double* acc, hipStream_t s);
 
 // ---- data movement ---------------------------------------------------------
 // Up to 64 words written to device memory by a kernel (stream-ordered, no

@@ -52,6 +52,8 @@ SIGNATURES = {
     "puff_set_undulator": (ctypes.c_int, [_vp, ctypes.c_int32] + [ctypes.c_double] * 6),
     "puff_power": (ctypes.c_int, [_vp, _dp]),
     "puff_bunching": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp]),
+    "puff_slice_stats": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, _dp]),
+    "puff_current": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, _dp]),
     "puff_particle_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "puff_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "puff_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
@@ -310,6 +312,23 @@ class Integrator:
         out = np.empty(nslices)
         _check(lib().puff_bunching(self._h, harmonic, nslices, slice_len, out.ctypes.data_as(_dp)),
                "puff_bunching")
+        return out
+
+    SLICE_KEYS = ("sdata", "aveX", "aveY", "avePX", "avePY", "sdX", "sdY", "sdpX", "sdpY", "eX", "eY",
+                  "aveGamma", "aveDgamma")
+
+    def slice_stats(self, nslices: int, slice_len: float) -> dict:
+        """getSliceTwiss's per-slice outputs (charge, means, r.m.s., emittances,
+        mean and r.m.s. Gamma), keyed by the reference's variable names."""
+        out = np.empty(13 * nslices)
+        _check(lib().puff_slice_stats(self._h, nslices, slice_len, out.ctypes.data_as(_dp)), "puff_slice_stats")
+        o = out.reshape(13, nslices)
+        return {k: o[i].copy() for i, k in enumerate(self.SLICE_KEYS)}
+
+    def current(self, npts: int, sam_len: float) -> np.ndarray:
+        """getCurr's binned charge per node, before its unit factors."""
+        out = np.empty(npts)
+        _check(lib().puff_current(self._h, npts, sam_len, out.ctypes.data_as(_dp)), "puff_current")
         return out
 
     # ---- state snapshots (device blobs in the slot order)

@@ -73,9 +73,10 @@ def lib():
         L.puff_h5_read_field.argtypes = [c, ctypes.c_int64, _dp, _dp]
         L.puff_h5_write_integrated.argtypes = [c, ip, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32, _dp]
         L.puff_h5_read_dataset_1d.argtypes = [c, c, ctypes.c_int64, _dp, ctypes.POINTER(ctypes.c_int64)]
+        L.puff_h5_append_1d.argtypes = [c, ip, c, ctypes.c_int64, _dp]
         for name in ("puff_h5_write_electrons", "puff_h5_count_electrons", "puff_h5_read_electrons",
                      "puff_h5_write_field", "puff_h5_read_field_info", "puff_h5_read_field",
-                     "puff_h5_write_integrated", "puff_h5_read_dataset_1d"):
+                     "puff_h5_write_integrated", "puff_h5_read_dataset_1d", "puff_h5_append_1d"):
             getattr(L, name).restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -175,6 +176,38 @@ def write_integrated(path: str, info: H5Info, power: np.ndarray, bunching: np.nd
     b = np.zeros((0, 1)) if bunching is None else np.ascontiguousarray(np.atleast_2d(bunching), dtype=np.float64)
     _check(lib().puff_h5_write_integrated(os.fsencode(path), ctypes.byref(info), p.size, _p(p), b.shape[0],
                                           b.shape[1], _p(b)), "write_integrated")
+
+
+def append_dataset(path: str, info: H5Info, name: str, values):
+    v = _col(values)
+    _check(lib().puff_h5_append_1d(os.fsencode(path), ctypes.byref(info), name.encode(), v.size, _p(v)),
+           "append_dataset")
+
+
+# The reference's physical constants are default-real literals
+# (common/puffin_constants.f90:23-26): single precision, widened.
+Q_E = float(np.float32(1.60217653e-19))
+C_LIGHT = float(np.float32(2.99792458e8))
+
+
+def slice_datasets(stats: dict, info: H5Info, current=None, sam_len: float = 0.0) -> dict:
+    """The integrated file's slice datasets (hdf5_puff.f90:229-416, scaled
+    forms) from a slice_stats() dict and, when given, the binned charge of
+    current() with its unit factors (av_write.f90:365-368):
+    npk_bar (x transArea in 1-D) x q_e / sam_len x c / Lc."""
+    g = np.where(stats["aveGamma"] == 0.0, 1.0, stats["aveGamma"])
+    emf = 2.0 * info.kappa / np.sqrt(info.eta) / g
+    out = {"Slice Charge": stats["sdata"], "meanXbar": stats["aveX"], "meanYbar": stats["aveY"],
+           "meanPXbar": stats["avePX"], "meanPYbar": stats["avePY"], "meanGamma": stats["aveGamma"],
+           "meanSigmaGamma": stats["aveDgamma"], "sigmaXbar": stats["sdX"], "sigmaYbar": stats["sdY"],
+           "sigmaPxbar": stats["sdpX"], "sigmaPybar": stats["sdpY"],
+           "emittanceXbar": emf * stats["eX"], "emittanceYbar": emf * stats["eY"]}
+    if current is not None:
+        cur = np.asarray(current, dtype=np.float64) * info.npk_bar
+        if info.num_spatial_dims == 1:
+            cur = cur * info.trans_area
+        out["beamCurrent"] = cur * Q_E / sam_len * C_LIGHT / info.lc
+    return out
 
 
 def read_dataset(path: str, name: str) -> np.ndarray:

@@ -58,7 +58,7 @@ def test_h5_library_exports_every_declared_symbol():
     from puffin_amd import h5io
     text = re.sub(r"/\*.*?\*/", "", open(H5_HEADER).read(), flags=re.S)
     syms = sorted(set(re.findall(r"\b(puff_h5_[a-z0-9_]+)\s*\(", text)))
-    assert len(syms) == 9
+    assert len(syms) == 10
     lib = h5io.lib()
     for s in syms:
         assert hasattr(lib, s), s

@@ -46,6 +46,8 @@ def test_f1main_on_device_through_hdf5(tmp_path):
         ar, ai = g.download_field()
         power = g.power()
         bun = g.bunching(1, 32, 4 * PI_REF * mp.rho)    # one resonant wavelength per slice
+        stats = g.slice_stats(32, 4 * PI_REF * mp.rho)
+        cur = g.current(33, 4 * PI_REF * mp.rho)
     out["chi"] = parts["chi"]
     f = h5io.info_from_module(mp, time=szl, istep=NSTEPS + 1, icsteps=NSTEPS, n_steps=NSTEPS, step_size=h,
                               lambda_w=rinfo.lambda_w, trans_area=rinfo.trans_area,
@@ -55,6 +57,10 @@ def test_f1main_on_device_through_hdf5(tmp_path):
     h5io.write_electrons(pe, out, f)
     h5io.write_field(pf, ar, ai, f)
     h5io.write_integrated(pi, f, power, bun[None, :])
+    f.npk_bar = rinfo.npk_bar
+    f.lc = rinfo.lc
+    for name, v in h5io.slice_datasets(stats, f, cur, 4 * PI_REF * mp.rho).items():
+        h5io.append_dataset(pi, f, name, v)
     mine, mi = h5io.read_electrons(pe)
     order = np.argsort(mine["z2"])                 # the reference writes its slots in z2 order
     for k in KEYS7:
@@ -69,3 +75,19 @@ def test_f1main_on_device_through_hdf5(tmp_path):
     db = np.abs(h5io.read_dataset(pi, "bunchingFundamental") - rbun).max()
     zerr = np.abs(mine["z2"][order] - ref["z2"]).max()
     assert db <= max(zerr, 1e-13) / (2 * mp.rho) and db <= 1e-10 / (2 * mp.rho)
+    # slice statistics and current: the charge to summation order (chi is not
+    # evolved), the momentum and Gamma statistics and the current at the
+    # particle bar (1e-10)
+    fi = os.path.join(GOLDEN, "f1main_expected_integrated.h5")
+    a, b = h5io.read_dataset(pi, "Slice Charge"), h5io.read_dataset(fi, "Slice Charge")
+    assert np.abs(a - b).max() <= 1e-14 * b.max()
+    # The r.m.s. Gamma comes from sum chi Gamma^2 / S - (sum chi Gamma / S)^2 with
+    # Gamma ~ 1 and spreads ~1e-7: the variance is at the rounding floor of
+    # Gamma^2 (2.2e-16), so the r.m.s. is fixed only to sqrt(eps) ~ 1.5e-8
+    # absolute whatever the summation order; the reference's value is that noise.
+    for name in ("meanGamma", "meanSigmaGamma", "meanPXbar", "meanPYbar", "sigmaPxbar", "sigmaPybar"):
+        a, b = h5io.read_dataset(pi, name), h5io.read_dataset(fi, name)
+        bar = 2.0 * np.sqrt(np.finfo(np.float64).eps) if name == "meanSigmaGamma" else 1e-10
+        assert np.abs(a - b).max() <= bar, name
+    a, b = h5io.read_dataset(pi, "beamCurrent"), h5io.read_dataset(fi, "beamCurrent")
+    assert np.abs(a - b).max() <= 1e-10 * b.max()

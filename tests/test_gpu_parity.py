@@ -277,6 +277,43 @@ def test_power_and_bunching_match_oracle():
     assert relerr(b, o.bunching(1, 8, 4 * 3.14159274 * mp.rho)) < 1e-12
 
 
+@pytest.mark.parametrize("prec", [FP64, MIXED])
+@pytest.mark.parametrize("periodic", [0, 1])
+def test_slice_stats_and_current_match_oracle(periodic, prec):
+    """getSliceTwiss's slice statistics and getCurr's binned charge on the
+    device against the oracle, after steps that spread p and Gamma."""
+    print(f"[gpu] slice stats periodic={periodic} prec={prec}", flush=True)
+    mp, p, (ar, ai), h = _small_3d(periodic, n=4000)
+    L = 4 * 3.1415927410125732 * mp.rho
+    lz = (mp.nz2 - 1) * mp.dz2
+    nslices, npts = int(np.ceil(lz / L)), int(np.ceil(lz / L)) + 1
+    with gpu.Integrator(mp, p["x"].size, prec) as g:
+        g.upload_particles(p)
+        g.upload_field(ar, ai)
+        szl = 0.0
+        for _ in range(6):
+            assert g.rk4par(szl, h) == 0
+            szl += h
+        out = g.download_particles()
+        s = g.slice_stats(nslices, L)
+        cur = g.current(npts, L)
+    # the charge weights as the device holds them (FP32 in the mixed form)
+    out["chi"] = p["chi"] if prec == FP64 else p["chi"].astype(np.float32).astype(np.float64)
+    st, ref = orc.slice_stats(mp, out, nslices, L)
+    assert st == 0
+    # Both sides sum the same stored values, in different orders.  Means and
+    # charge: 1e-12.  Spreads and emittances are differences of near-equal
+    # second moments (Gamma spreads 1e-3 about 1: six digits cancel), so
+    # summation order moves them by ~1e-9 relative: 1e-8.
+    spread = ("sdX", "sdY", "sdpX", "sdpY", "eX", "eY", "aveDgamma")
+    for k, v in ref.items():
+        tol = 1e-8 if k in spread else 1e-12
+        assert np.abs(s[k] - v).max() <= tol * max(np.abs(v).max(), 1e-30), k
+    st, rc = orc.current(mp, out, npts, L)
+    assert st == 0
+    assert relerr(cur, rc) < 1e-12
+
+
 # ---------------------------------------------------------------------------
 # mixed precision on the small case
 # ---------------------------------------------------------------------------
