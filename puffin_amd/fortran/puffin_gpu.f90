@@ -15,6 +15,9 @@
 !   gpu_disperse(disp, slip)           disperse, acc_lattice.f90:420-463
 !   gpu_quad(qfx, qfy)                 Quad, acc_lattice.f90:531-561
 !   gpu_bmodulation(wavenum, mag)      bModulation, acc_lattice.f90:575-585
+! and the output reductions on the device particles:
+!   getSliceTwiss(nslices, slicetrim, aveX, ..., sdata)  io/av_write.f90:405-773
+!   getCurr(sam_len, Iarray)                            io/av_write.f90:297-368
 !
 ! The particle arrays sElX_G .. sElGam_G, s_chi_bar_G (deriv_globals.f90:94,
 ! 128-133) and the field ac_rfield_in / ac_ifield_in (puffin_mpi_RK4.f90:24,
@@ -149,6 +152,33 @@ module puffin_gpu_capi
       integer(c_int) :: r
     end function
 
+    function puff_bunching(ctx, harmonic, nslices, slice_len, b) bind(C, name="puff_bunching") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: harmonic, nslices
+      real(c_double), value :: slice_len
+      real(c_double), intent(out) :: b(*)
+      integer(c_int) :: r
+    end function
+
+    function puff_slice_stats(ctx, nslices, slice_len, out) bind(C, name="puff_slice_stats") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: nslices
+      real(c_double), value :: slice_len
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: r
+    end function
+
+    function puff_current(ctx, npts, sam_len, out) bind(C, name="puff_current") result(r)
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), value :: npts
+      real(c_double), value :: sam_len
+      real(c_double), intent(out) :: out(*)
+      integer(c_int) :: r
+    end function
+
     function puff_drift(ctx, dz) bind(C, name="puff_drift") result(r)
       import :: c_ptr, c_int, c_double
       type(c_ptr), value :: ctx
@@ -231,6 +261,11 @@ module puffin_gpu
   logical, save :: qZ2MeshOK_G = .true.
 
   type(c_ptr), save :: ctx_G = c_null_ptr
+
+  ! scalings of the output routines (deriv_globals.f90:102-106, 199), filled
+  ! by the caller as setup leaves them
+  real(wp), save :: sRho_G = 0.01_wp, npk_bar_G = 1.0_wp, ata_G = 1.0_wp, lc_G = 1.0_wp
+  logical, save :: qOneD_G = .false.
 
 contains
 
@@ -405,5 +440,59 @@ contains
     call ensure_device()
     call gpu_check(puff_modulation(ctx_G, wavenum, mag), 'puff_modulation')
   end subroutine gpu_bmodulation
+
+  ! ---- output reductions (io/av_write.f90) on the device particles
+
+  ! getSliceTwiss (av_write.f90:405-773): slices of 4 pi rho.  slicetrim is
+  ! unused there too; the Twiss alpha and beta outputs are zero, as the
+  ! reference leaves them.
+  subroutine getSliceTwiss(nslices, slicetrim, aveX, aveY, avePX, avePY, sdX, sdY, sdpx, sdpy, &
+                           eX, eY, ax, ay, bx, by, aveGamma, aveDgamma, b1, b2, b3, b4, b5, sdata)
+    integer, intent(in) :: nslices
+    real(wp), intent(in) :: slicetrim
+    real(wp), dimension(nslices), intent(out) :: aveX, aveY, avePX, avePY, sdX, sdY, sdpx, sdpy, &
+                                                 eX, eY, ax, ay, bx, by, aveGamma, aveDgamma, &
+                                                 b1, b2, b3, b4, b5, sdata
+    real(wp), allocatable :: o(:)
+    real(wp) :: sl
+    integer :: n
+    n = nslices
+    sl = 4.0_wp * pi_ref() * sRho_G
+    call ensure_device()
+    allocate(o(13 * n))
+    call gpu_check(puff_slice_stats(ctx_G, int(n, c_int32_t), sl, o), 'puff_slice_stats')
+    sdata = o(1:n);            aveX = o(n+1:2*n);       aveY = o(2*n+1:3*n)
+    avePX = o(3*n+1:4*n);      avePY = o(4*n+1:5*n);    sdX = o(5*n+1:6*n)
+    sdY = o(6*n+1:7*n);        sdpx = o(7*n+1:8*n);     sdpy = o(8*n+1:9*n)
+    eX = o(9*n+1:10*n);        eY = o(10*n+1:11*n);     aveGamma = o(11*n+1:12*n)
+    aveDgamma = o(12*n+1:13*n)
+    ax = 0.0_wp; ay = 0.0_wp; bx = 0.0_wp; by = 0.0_wp
+    call gpu_check(puff_bunching(ctx_G, 1_c_int32_t, int(n, c_int32_t), sl, b1), 'puff_bunching')
+    call gpu_check(puff_bunching(ctx_G, 2_c_int32_t, int(n, c_int32_t), sl, b2), 'puff_bunching')
+    call gpu_check(puff_bunching(ctx_G, 3_c_int32_t, int(n, c_int32_t), sl, b3), 'puff_bunching')
+    call gpu_check(puff_bunching(ctx_G, 4_c_int32_t, int(n, c_int32_t), sl, b4), 'puff_bunching')
+    call gpu_check(puff_bunching(ctx_G, 5_c_int32_t, int(n, c_int32_t), sl, b5), 'puff_bunching')
+    deallocate(o)
+  end subroutine getSliceTwiss
+
+  ! getCurr (av_write.f90:297-368): charge on size(Iarray) nodes sam_len
+  ! apart, in amperes, with the reference's single-precision constants.
+  subroutine getCurr(sam_len, Iarray)
+    real(wp), intent(in) :: sam_len
+    real(wp), intent(inout) :: Iarray(:)
+    real(wp), parameter :: q_e = 1.60217653e-19, c = 2.99792458e8
+    call ensure_device()
+    call gpu_check(puff_current(ctx_G, int(size(Iarray), c_int32_t), sam_len, Iarray), 'puff_current')
+    Iarray = Iarray * npk_bar_G
+    if (qOneD_G) Iarray = Iarray * ata_G
+    Iarray = Iarray * q_e / sam_len
+    Iarray = Iarray * c / lc_G
+  end subroutine getCurr
+
+  ! pi as the reference's default-real literal, widened
+  pure function pi_ref() result(p)
+    real(wp) :: p
+    p = real(3.14159265358979323846, wp)
+  end function pi_ref
 
 end module puffin_gpu

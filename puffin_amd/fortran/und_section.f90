@@ -15,7 +15,10 @@
 !   real64 drift, quad_fx, quad_fy, chic_disp, chic_slip, enmod_wavenum, enmod_mag
 !   real64 x(n), y(n), z2(n), px(n), py(n), gam(n), chi(n), ar(nmesh), ai(nmesh)
 ! Output: int64 nwrites, then real64 x, y, z2, px, py, gam (n each), ar, ai
-! (nmesh each), power(nz2, nwrites), and the int64 final status.
+! (nmesh each), power(nz2, nwrites); with ``elements``, int64 nslices and
+! real64 sdata, aveGamma, aveDgamma, avePX, sdpx, b1 (nslices each) from
+! getSliceTwiss and Iarray (nslices + 1) from getCurr with unit npk_bar and
+! Lc; and the int64 final status.
 
 program und_section
 
@@ -27,6 +30,8 @@ program und_section
   character(len=4096) :: fin, fout
   integer(c_int64_t) :: n, nsteps, steps4diff, redist_every, idiff, iper, write_every, ielem
   real(wp) :: el(7)
+  integer :: nsl
+  real(wp), allocatable :: tw(:,:), cur(:)
   integer(c_int64_t) :: nmesh, istep, iCsteps, nw, status
   real(wp) :: h, szl, dzdS, dzdF, dzd
   real(wp), allocatable :: powers(:,:), pw(:)
@@ -125,6 +130,15 @@ program und_section
     call gpu_quad(el(2), el(3))                        ! Quad
     call gpu_disperse(el(4), el(5))                    ! disperse (no zbar: no diffraction)
     call gpu_bmodulation(el(6), el(7))                 ! bModulation
+    ! output reductions of the final state (slices of 4 pi rho)
+    sRho_G = tParams_G%rho
+    qOneD_G = tParams_G%one_d /= 0
+    nsl = ceiling(tParams_G%dz2 * real(tParams_G%nz2 - 1, wp) / (4.0_wp * pi_ref() * sRho_G))
+    allocate(tw(nsl, 22), cur(nsl + 1))
+    call getSliceTwiss(nsl, 0.0_wp, tw(:,1), tw(:,2), tw(:,3), tw(:,4), tw(:,5), tw(:,6), tw(:,7), &
+                       tw(:,8), tw(:,9), tw(:,10), tw(:,11), tw(:,12), tw(:,13), tw(:,14), tw(:,15), &
+                       tw(:,16), tw(:,17), tw(:,18), tw(:,19), tw(:,20), tw(:,21), tw(:,22))
+    call getCurr(4.0_wp * pi_ref() * sRho_G, cur)
     call deallact_rk4_arrs()
   end if
 
@@ -133,6 +147,11 @@ program und_section
   write(u) sElX_G, sElY_G, sElZ2_G, sElPX_G, sElPY_G, sElGam_G
   write(u) ac_rfield_in, ac_ifield_in
   if (nw > 0) write(u) powers(:, 1:nw)
+  if (allocated(tw)) then
+    write(u) int(nsl, c_int64_t)
+    write(u) tw(:,22), tw(:,15), tw(:,16), tw(:,3), tw(:,7), tw(:,17)
+    write(u) cur
+  end if
   write(u) status
   close(u)
 

@@ -116,6 +116,12 @@ def test_fortran_lattice_elements_match_oracle(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     out, (ar, ai), _, status = read_output(fout, p["x"].size, mp)
     assert status == 0
+    raw = np.frombuffer(open(fout, "rb").read(), dtype=np.float64)
+    nsl = int(np.ceil(mp.dz2 * (mp.nz2 - 1) / (4 * PI_REF * mp.rho)))
+    tail = raw[-1 - (6 * nsl + nsl + 1):-1]          # before the status word
+    assert int(raw[-2 - (6 * nsl + nsl + 1)].view(np.int64)) == nsl
+    f_sd, f_ag, f_adg, f_apx, f_sdpx, f_b1 = tail[:6 * nsl].reshape(6, nsl)
+    f_cur = tail[6 * nsl:]
     q = {k: v.copy() for k, v in p.items()}
     orc.match(mp, 1.0, q, 0.0, +1)
     o = orc.Oracle(mp, q, field)
@@ -133,3 +139,16 @@ def test_fortran_lattice_elements_match_oracle(tmp_path):
     scale = np.abs(o.ar + 1j * o.ai).max()
     assert np.abs(ar - o.ar).max() <= 1e-10 * scale
     assert np.abs(ai - o.ai).max() <= 1e-10 * scale
+    # getSliceTwiss and getCurr drop-ins on the final state
+    L = 4 * PI_REF * mp.rho
+    st, ss = orc.slice_stats(mp, o.parts, nsl, L)
+    assert st == 0
+    for got, k in ((f_sd, "sdata"), (f_ag, "aveGamma"), (f_apx, "avePX")):
+        assert np.abs(got - ss[k]).max() <= 1e-10 * np.abs(ss[k]).max(), k
+    for got, k in ((f_adg, "aveDgamma"), (f_sdpx, "sdpX")):
+        assert np.abs(got - ss[k]).max() <= 1e-7 * np.abs(ss[k]).max(), k
+    assert np.abs(f_b1 - o.bunching(1, nsl, L)).max() <= 1e-10 * np.abs(f_b1).max()
+    st, cur = orc.current(mp, o.parts, nsl + 1, L)
+    assert st == 0
+    q_e, c = float(np.float32(1.60217653e-19)), float(np.float32(2.99792458e8))
+    np.testing.assert_allclose(f_cur, cur * q_e / L * c, rtol=1e-10)
