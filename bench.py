@@ -258,7 +258,8 @@ def case_power0(case):
 def adjoint_bench(steps_check: bool = True):
     """d(integrated power)/d(per-period a_w, seed amplitude) on the adjoint
     workload (helical, periodic 33 x 33 x 256, 2^20 macroparticles, 16
-    periods, checkpoints every 30 steps): forward and gradient wall time and
+    periods; every step's capture kept when they fit the engine's budget,
+    else checkpoints every 30 steps): forward and gradient wall time and
     central differences (h = 1e-6) of two components."""
     from puffin_amd import gpu
     from puffin_amd.adjoint import ModuleGradient, module_program
@@ -284,7 +285,9 @@ def adjoint_bench(steps_check: bool = True):
         t3 = time.perf_counter()
         out = dict(workload=WORKLOADS["c3"] + ", 16 periods of 30 steps", J=J,
                    forward_s=t1 - t0, forward_checkpointed_s=t2 - t1, backward_s=t3 - t2,
-                   gradient_cost_vs_forward=(t3 - t1) / (t1 - t0), checkpoint_every=30,
+                   gradient_cost_vs_forward=(t3 - t1) / (t1 - t0),
+                   memory_plan=("every step captured in the forward, no replay" if eng.keep_all
+                                else "checkpoints every 30 steps, segments replayed"),
                    snapshot_bytes=eng.memory_bytes(), grad_seed=g["seed"],
                    grad_table=[float(v) for v in g["table"]])
         if steps_check:

@@ -106,13 +106,21 @@ def segments(ops: list[Op], every: int) -> list[tuple[int, int]]:
 class ModuleGradient:
     """Forward run of one module with checkpoints, and its adjoint.
 
+    Two memory plans: when every step's capture fits (``keep_all``, chosen
+    automatically against ``keep_budget`` and the free device memory) the
+    forward keeps them all and the backward is adjoint steps only; otherwise
+    the forward keeps a checkpoint every ``every`` steps and the backward
+    replays each segment (with step captures or start-state snapshots)
+    before reversing it.
+
     ``drv`` is an Integrator whose particles hold the module's initial state;
     ``field_shape`` is the seed field of unit amplitude (split Re / Im);
     ``taper_period`` / ``taper_z0`` place the per-period table on zbar."""
 
     def __init__(self, drv: Integrator, ops: list[Op], field_shape: tuple, *, every: int = 30,
                  taper_period: float | None = None, taper_z0: float = 0.0,
-                 capture: bool | None = None, capture_budget: int = 32 << 30):
+                 capture: bool | None = None, capture_budget: int = 32 << 30,
+                 keep_all: bool | None = None, keep_budget: int = 128 << 30):
         if torch is None:
             raise RuntimeError("the gradient engine keeps its snapshots in torch device memory")
         self.drv = drv
@@ -133,18 +141,32 @@ class ModuleGradient:
         self.initial = self._blob()
         self.initial_hdr = hdr()
         drv.state_save(self.initial.data_ptr(), self.initial_hdr)
-        self.ckpts = [self._blob() for _ in self.segs]
-        self.ckpt_hdr = [hdr() for _ in self.segs]
-        # per-step data of the segment being reversed: step captures (start
-        # state and stage data, no forward stage recomputed by the adjoint
-        # step) when they fit the budget, else start-state snapshots
         nc = drv.capture_bytes()
-        self.capture = (self.every * nc <= capture_budget) if capture is None else bool(capture)
-        if self.capture:
-            self.steps = [torch.empty(nc, dtype=torch.uint8, device=dev) for _ in range(self.every)]
+        nrk4 = sum(1 for op in ops if op.kind == "rk4")
+        # Whole-module captures: the forward keeps every step's capture (start
+        # state and stage data) and the backward reverses the steps with no
+        # replay, when they fit keep_budget and 60 % of the free device memory.
+        if keep_all is None:
+            free, _ = torch.cuda.mem_get_info(dev)
+            keep_all = nrk4 * nc <= min(keep_budget, 0.6 * free)
+        self.keep_all = bool(keep_all)
+        if self.keep_all:
+            self.ckpts, self.ckpt_hdr = [], []
+            self.capture = True
+            self.steps = [torch.empty(nc, dtype=torch.uint8, device=dev) for _ in range(nrk4)]
+            self.step_hdr = [hdr() for _ in range(nrk4)]
         else:
-            self.steps = [self._blob() for _ in range(self.every)]
-        self.step_hdr = [hdr() for _ in range(self.every)]
+            self.ckpts = [self._blob() for _ in self.segs]
+            self.ckpt_hdr = [hdr() for _ in self.segs]
+            # per-step data of the segment being reversed: step captures (start
+            # state and stage data, no forward stage recomputed by the adjoint
+            # step) when they fit the budget, else start-state snapshots
+            self.capture = (self.every * nc <= capture_budget) if capture is None else bool(capture)
+            if self.capture:
+                self.steps = [torch.empty(nc, dtype=torch.uint8, device=dev) for _ in range(self.every)]
+            else:
+                self.steps = [self._blob() for _ in range(self.every)]
+            self.step_hdr = [hdr() for _ in range(self.every)]
         self.table = None
         self.seed = None
         self.J = None
@@ -185,7 +207,16 @@ class ModuleGradient:
         self._set_inputs(table, seed)
         self._deferred(True)
         try:
-            for k, (a, b) in enumerate(self.segs):
+            if self.keep_all:
+                j = 0
+                for op in self.ops:
+                    if op.kind == "rk4":
+                        self.drv.rk4par_capture(op.sz, op.h, self.steps[j].data_ptr(), self.step_hdr[j])
+                        self.rk4_calls += 1
+                        j += 1
+                    else:
+                        self._run(op)
+            for k, (a, b) in enumerate(self.segs if not self.keep_all else []):
                 self.drv.state_save(self.ckpts[k].data_ptr(), self.ckpt_hdr[k])
                 for op in self.ops[a:b]:
                     self._run(op)
@@ -217,7 +248,16 @@ class ModuleGradient:
         drv.adjoint_seed_power(self.wz, scale)
         self._deferred(True)
         try:
-            for k in range(len(self.segs) - 1, -1, -1):
+            if self.keep_all:
+                j = len(self.steps)
+                for op in reversed(self.ops):
+                    if op.kind == "rk4":
+                        j -= 1
+                        drv.rk4par_adjoint_captured(op.sz, op.h, self.steps[j].data_ptr(), self.step_hdr[j])
+                    elif op.kind == "diff":
+                        drv.diffraction_adjoint(op.h)
+                self._check_status("adjoint")
+            for k in range(len(self.segs) - 1 if not self.keep_all else -1, -1, -1):
                 a, b = self.segs[k]
                 drv.state_load(self.ckpts[k].data_ptr(), self.ckpt_hdr[k])
                 slots = {}

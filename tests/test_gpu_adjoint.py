@@ -123,28 +123,31 @@ def test_state_snapshot_round_trip():
             assert np.abs(q1[k] - q2[k]).max() <= 1e-12 * np.abs(q1[k]).max() + 1e-300, k
 
 
-def _engine(gpu, c, precision, every, capture=None):
+def _engine(gpu, c, precision, every, capture=None, keep_all=False):
     from puffin_amd.adjoint import ModuleGradient
     mp = c["mp"]
     drv = gpu.Integrator(mp, c["parts"]["x"].size, precision)
     drv.upload_particles(c["parts"])
     drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
     eng = ModuleGradient(drv, c["ops"], c["shape"], every=every, taper_period=c["period"],
-                         capture=capture)
+                         capture=capture, keep_all=keep_all)
+    assert eng.keep_all == bool(keep_all)
     return drv, eng
 
 
-@pytest.mark.parametrize("capture", [False, True])
+@pytest.mark.parametrize("capture,keep_all", [(False, False), (True, False), (None, True)])
 @pytest.mark.parametrize("periodic", [1, 0])
-def test_module_gradient_matches_oracle(periodic, capture):
+def test_module_gradient_matches_oracle(periodic, capture, keep_all):
     """Checkpoints every 10 steps (segments end between diffractions and
     wraps); the default sort interval (30 steps) sorts inside segments; the
-    adjoint steps replay their forward stages or use the step captures."""
-    print(f"[gpu] module gradient periodic={periodic} capture={capture}", flush=True)
+    adjoint steps replay their forward stages or use the step captures; or
+    the forward keeps every step's capture and nothing is replayed (the sorts
+    then fall between captured steps across the whole module)."""
+    print(f"[gpu] module gradient periodic={periodic} capture={capture} keep_all={keep_all}", flush=True)
     gpu = _gpu()
     c = small_module(periodic=periodic)
     J0, gt0, gs0, _ = OracleModule(c).gradient(c["table"], c["seed"])
-    drv, eng = _engine(gpu, c, FP64, every=10, capture=capture)
+    drv, eng = _engine(gpu, c, FP64, every=10, capture=capture, keep_all=keep_all)
     try:
         J = eng.forward(c["table"], c["seed"])
         g = eng.backward()
