@@ -24,7 +24,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .params import (PI_REF, UND_ELLIPTICAL, ModuleParams, filter_cutoff, gamma_r_for_eta,
+from .params import (PI_REF, ModuleParams, filter_cutoff, gamma_r_for_eta,
                      scaling, step_size, steps_for_diffraction)
 
 SEED_BASE = 20261202

@@ -14,7 +14,7 @@ import os
 
 import numpy as np
 
-from .params import FP64, MEM_DEVICE, MEM_HOST, MIXED, STATUS_NAMES, ModuleParams, PuffParams
+from .params import FP64, MEM_DEVICE, MEM_HOST, STATUS_NAMES, ModuleParams, PuffParams
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpuffin_gpu.so")

@@ -14,7 +14,6 @@ pinned without reference data (the reference computes no derivatives):
 """
 from __future__ import annotations
 
-import math
 
 import numpy as np
 import pytest

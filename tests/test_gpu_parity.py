@@ -6,7 +6,6 @@ FP64 parity with the oracle is held at 1e-10 relative to the largest value.
 from __future__ import annotations
 
 import math
-import sys
 
 import numpy as np
 import pytest
@@ -14,7 +13,7 @@ import pytest
 import oracle as orc
 from f1main_case import NSTEPS, initial_particles, load_fixture, module_params
 from puffin_amd import gpu
-from puffin_amd.configs import case_c0, gaussian_seed, match_in
+from puffin_amd.configs import gaussian_seed, match_in
 from puffin_amd.params import FP64, MIXED, ModuleParams, scaling, step_size
 from puffin_amd.undulator import und_section
 

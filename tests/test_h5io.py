@@ -14,7 +14,7 @@ import pytest
 
 from f1main_case import load_fixture, module_params
 from puffin_amd import h5io
-from puffin_amd.params import ModuleParams, PI_REF, scaling, step_size
+from puffin_amd.params import ModuleParams, scaling, step_size
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 FE = os.path.join(GOLDEN, "f1main_expected_electrons.h5")
