@@ -243,6 +243,12 @@ int puff_slice_stats(puff_ctx* ctx, int32_t nslices, double slice_len, double* o
    STOPs) returns PUFF_ERR_STATE. */
 int puff_current(puff_ctx* ctx, int32_t npts, double sam_len, double* out_host);
 
+/* Share of the stage workgroups (four stages) whose node box did not fit
+   the LDS deposit tile and deposited with global atomics: for the last step
+   run without deferred status, and for the first step after the last
+   re-sort (the re-sort trigger compares the two). */
+int puff_tile_fallback(puff_ctx* ctx, double* last_step, double* after_sort);
+
 /* Number of macroparticles of the context. */
 int puff_particle_count(puff_ctx* ctx, int64_t* n);
 

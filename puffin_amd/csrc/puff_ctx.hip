@@ -93,6 +93,7 @@ struct Ctx {
     // adaptive re-sort: share of stage workgroups that fell back to global
     // atomics on the first step after the last sort, and the due flag
     double fallback_base = 0.0;
+    double fallback_last = 0.0;   // share of the last synchronised step
     bool sort_due = false;
     // kernel timing (puff_profile_*)
     bool prof = false;
@@ -843,6 +844,7 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
         // letting the share grow until the fallback has cost one sort)
         const double groups = 4.0 * (double)((c->n + STAGE_THREADS - 1) / STAGE_THREADS);
         const double share = groups > 0 ? (double)c->h_flags[FLAG_WORD_FALLBACK] / groups : 0.0;
+        c->fallback_last = share;
         if (c->steps_since_sort == 1) c->fallback_base = share;
         c->sort_due = share > std::max(0.02, 2.0 * c->fallback_base);
     }
@@ -1998,6 +2000,14 @@ int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices, double slice
     CTX_CHECK(ctx);
     if (!out || !(slice_len > 0)) return PUFF_ERR_ARG;
     return DISPATCH(ctx, bunching, harmonic, nslices, slice_len, out);
+}
+
+int puff_tile_fallback(puff_ctx* ctx, double* last_step, double* after_sort) {
+    CTX_CHECK(ctx);
+    if (!last_step || !after_sort) return PUFF_ERR_ARG;
+    *last_step = ctx->c.fallback_last;
+    *after_sort = ctx->c.fallback_base;
+    return PUFF_OK;
 }
 
 int puff_slice_stats(puff_ctx* ctx, int32_t nslices, double slice_len, double* out) {

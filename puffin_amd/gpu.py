@@ -52,6 +52,7 @@ SIGNATURES = {
     "puff_set_undulator": (ctypes.c_int, [_vp, ctypes.c_int32] + [ctypes.c_double] * 6),
     "puff_power": (ctypes.c_int, [_vp, _dp]),
     "puff_bunching": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _dp]),
+    "puff_tile_fallback": (ctypes.c_int, [_vp, _dp, _dp]),
     "puff_slice_stats": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, _dp]),
     "puff_current": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, _dp]),
     "puff_particle_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
@@ -313,6 +314,13 @@ class Integrator:
         _check(lib().puff_bunching(self._h, harmonic, nslices, slice_len, out.ctypes.data_as(_dp)),
                "puff_bunching")
         return out
+
+    def tile_fallback(self):
+        """(last step, first step after the last re-sort): share of stage
+        workgroups that deposited with global atomics."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().puff_tile_fallback(self._h, ctypes.byref(a), ctypes.byref(b)), "puff_tile_fallback")
+        return a.value, b.value
 
     SLICE_KEYS = ("sdata", "aveX", "aveY", "avePX", "avePY", "sdX", "sdY", "sdpX", "sdpY", "eX", "eY",
                   "aveGamma", "aveDgamma")

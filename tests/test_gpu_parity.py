@@ -276,6 +276,19 @@ def test_power_and_bunching_match_oracle():
     assert relerr(b, o.bunching(1, 8, 4 * 3.14159274 * mp.rho)) < 1e-12
 
 
+def test_tile_fallback_share_is_a_fraction():
+    """The share of stage workgroups that deposited with global atomics is
+    recorded per synchronised step (and right after a re-sort)."""
+    mp, p, (ar, ai), h = _small_3d(0, n=3000)
+    with gpu.Integrator(mp, p["x"].size, FP64) as g:
+        g.upload_particles(p)
+        g.upload_field(ar, ai)
+        assert g.rk4par(0.0, h) == 0
+        assert g.rk4par(h, h) == 0
+        last, base = g.tile_fallback()
+    assert 0.0 <= last <= 1.0 and 0.0 <= base <= 1.0
+
+
 @pytest.mark.parametrize("prec", [FP64, MIXED])
 @pytest.mark.parametrize("periodic", [0, 1])
 def test_slice_stats_and_current_match_oracle(periodic, prec):
