@@ -148,3 +148,28 @@ def test_lattice_run_with_undulator_ends_matches_oracle():
     for k in KEYS:
         assert relerr(out[k], op[k]) < 1e-10, k
     assert relerr(gr, oar) < 1e-10 and relerr(gi, oai) < 1e-10
+
+
+def test_one_d_drift_moves_z2_only_and_quad_is_inert():
+    """On a 1-D mesh (the reference's own f1main module) driftSection moves z2
+    only and Quad leaves the momenta alone (acc_lattice.f90:501, 544)."""
+    print("[gpu] lattice 1-D drift and quad", flush=True)
+    from f1main_case import initial_particles, load_fixture, module_params
+    d, _ = load_fixture()
+    mp = module_params()
+    p = initial_particles(d["electrons"][:, 6])
+    rng = np.random.default_rng(3)
+    p["px"] = 0.2 * rng.standard_normal(p["x"].size)
+    p["py"] = 0.2 * rng.standard_normal(p["x"].size)
+    p["gam"] = 1.0 + 1e-3 * rng.standard_normal(p["x"].size)
+    with gpu.Integrator(mp, p["x"].size, FP64) as g:
+        g.upload_particles(p)
+        g.drift(0.05)
+        g.quad(2.0, -2.0)
+        out = g.download_particles()
+    ref = {k: v.copy() for k, v in p.items()}
+    orc.drift(mp, ref, 0.05)
+    for k in ("x", "y", "px", "py", "gam"):
+        assert np.array_equal(out[k], p[k]), k
+    assert relerr(out["z2"], ref["z2"]) < 1e-13
+    assert np.abs(out["z2"] - p["z2"]).min() > 0.0
