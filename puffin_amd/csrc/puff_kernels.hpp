@@ -150,10 +150,12 @@ void launch_fill_zero_words(uint32_t* p, int64_t n, hipStream_t s);
 // covers a compact 3-D block of cells whose node box grows slowly as the
 // beam slips along z2 between sorts, and lanes of a wavefront share x-rows of
 // nodes.  One bin per cell of the padded brick grid, plus one for particles
-// off the mesh.  8 x 2 x 4 rows along x measured best on the C1 beam: 2-5 %
-// faster steps than plane/row order, with 2-6x fewer re-sorts once bunched.
+// off the mesh.  Bricks of 4 x 2 x 4 with rows along x measured best: 2-5 %
+// faster steps than plane/row order on the C1 beam, with 2-6x fewer re-sorts
+// once bunched; 1 % faster than 8 x 2 x 4 there and 4 % on the sparser C4
+// beam (fewer nodes flushed per macroparticle, fewer tile overflows).
 #ifndef PUFF_BRICK_X
-#define PUFF_BRICK_X 8
+#define PUFF_BRICK_X 4
 #endif
 #ifndef PUFF_BRICK_Y
 #define PUFF_BRICK_Y 2
