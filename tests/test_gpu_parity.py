@@ -276,6 +276,24 @@ def test_power_and_bunching_match_oracle():
     assert relerr(b, o.bunching(1, 8, 4 * 3.14159274 * mp.rho)) < 1e-12
 
 
+def test_current_outside_the_nodes_is_an_error():
+    """getCurr STOPs when a macroparticle lies past the last node; the device
+    returns an error instead, and the particles are left as they were."""
+    mp, p, _, _ = _small_3d(0, n=2000)
+    L = 4 * 3.1415927410125732 * mp.rho
+    with gpu.Integrator(mp, p["x"].size, FP64) as g:
+        g.upload_particles(p)
+        with pytest.raises(gpu.PuffinError):
+            g.current(2, L)                         # nodes cover [0, L] only
+        with pytest.raises(gpu.PuffinError):
+            g.slice_stats(0, L)
+        out = g.download_particles()
+        ok = g.current(int(np.ceil((mp.nz2 - 1) * mp.dz2 / L)) + 1, L)
+    for k in KEYS:
+        assert np.array_equal(out[k], p[k]), k
+    assert abs(ok.sum() - p["chi"].sum()) <= 1e-12 * p["chi"].sum()
+
+
 def test_tile_fallback_share_is_a_fraction():
     """The share of stage workgroups that deposited with global atomics is
     recorded per synchronised step (and right after a re-sort)."""
