@@ -33,7 +33,8 @@ def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-# Algorithmic bytes: per macroparticle per RK4 stage and per
+# Algorithmic bytes: per macroparticle per RK4 stage (stage s reads y0, k_{s-1}
+# and chi and writes k_s; stage 4 also reads k1, k2 and writes y) and per
 # active node per stage; the field combination and the diffraction per node.
 def stage_bytes(precision: int):
     if precision == 0:   # FP64: six-vector 48 B, chi 8 B, complex 16 B
@@ -504,8 +505,8 @@ def main():
         print(json.dumps(diffraction_bench()), flush=True)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(z2 slab decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path "
+              "(the z2 slab decomposition across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     from puffin_amd.configs import CASES

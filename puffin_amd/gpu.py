@@ -97,12 +97,11 @@ class PuffinError(RuntimeError):
 
 
 def lib() -> ctypes.CDLL:
-    """Load libpuffin_gpu.so (built by __graft_entry__.build())."""
+    """Load libpuffin_gpu.so (built by `make -C puffin_amd`)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C puffin_amd` "
-                               "or __graft_entry__.build()")
+            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C puffin_amd`")
         # torch bundles its own HIP runtime under the same soname
         # (libamdhip64.so.7); whichever loads first serves the whole process.
         # Load torch's first when torch is present, so that the library and

@@ -26,7 +26,7 @@ def test_header_declares_entry_points():
 
 def test_library_exports_every_declared_symbol():
     if not os.path.exists(gpu.LIB_PATH):
-        pytest.fail("libpuffin_gpu.so is not built (run __graft_entry__.build())")
+        pytest.fail("libpuffin_gpu.so is not built (run `make -C puffin_amd`)")
     lib = gpu.lib()
     for s in declared_symbols():
         assert hasattr(lib, s), s
