@@ -15,6 +15,25 @@
  * OpenMP: the particle loops run in parallel; the deposit uses per-thread
  * private grids reduced in thread order, so results do not depend on thread
  * timing (they do depend on the thread count, at the rounding level).
+ *
+ * How each part is pinned (tests/test_oracle.py, test_lattice.py,
+ * test_adjoint_oracle.py):
+ *   the reference's own 1-D run (test/expected/f1main_expected_*.h5):
+ *     p2, main-section b-fields, 1-D interpolation and deposit, equations,
+ *     RK4, taper, matchIn/matchOut and correctTrans (elliptical), power,
+ *     bunching, slice statistics and the binned current;
+ *   closed forms, invariants and library routines: the FFT and k-vectors
+ *   (numpy, brute-force DFT), propagator and absorption, masks, b-fields at
+ *   special points, 3-D interpolation/deposit (partition of unity, adjoint
+ *   identity, brute force), the lattice elements (drift additivity and its
+ *   getGamma inverse, chicane slip and delay, quad inverse, modulation,
+ *   corrector means, matching round trips and the plane-pole and curved
+ *   entry offsets), slice bounds and the periodic wrap, the adjoint
+ *   (central differences, dot-product identity).
+ * Parity unpinned against reference outputs (none exist and the reference
+ * cannot be built here): the 3-D integrator, diffraction and FFT numbers,
+ * the undulator-end b-field branches and the curved-pole b-fields; these
+ * rest on the closed forms and invariants above.
  */
 #include <math.h>
 #include <stdint.h>
