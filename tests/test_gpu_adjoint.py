@@ -181,13 +181,15 @@ def test_module_gradient_against_gpu_finite_differences():
         drv.close()
 
 
-def test_mixed_gradient_tracks_fp64():
-    print("[gpu] mixed-precision gradient", flush=True)
+@pytest.mark.parametrize("keep_all", [False, True])
+def test_mixed_gradient_tracks_fp64(keep_all):
+    print(f"[gpu] mixed-precision gradient keep_all={keep_all}", flush=True)
     gpu = _gpu()
     c = small_module(periodic=1, n=4000)
     out = {}
     for prec in (FP64, MIXED):
-        drv, eng = _engine(gpu, c, prec, every=12, capture=(prec == MIXED))
+        drv, eng = _engine(gpu, c, prec, every=12, capture=(prec == MIXED) if not keep_all else None,
+                           keep_all=keep_all)
         try:
             J = eng.forward(c["table"], c["seed"])
             out[prec] = (J, eng.backward())
