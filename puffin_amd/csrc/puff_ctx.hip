@@ -846,7 +846,7 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
         const double share = groups > 0 ? (double)c->h_flags[FLAG_WORD_FALLBACK] / groups : 0.0;
         c->fallback_last = share;
         if (c->steps_since_sort == 1) c->fallback_base = share;
-        c->sort_due = share > std::max(0.02, 2.0 * c->fallback_base);
+        c->sort_due = share > std::max(PUFF_RESORT_MIN_SHARE, PUFF_RESORT_GROWTH * c->fallback_base);
     }
     if (f) {
         c->n2col = saved[0];

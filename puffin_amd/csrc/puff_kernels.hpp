@@ -167,6 +167,15 @@ void launch_fill_zero_words(uint32_t* p, int64_t n, hipStream_t s);
 #define PUFF_BRICK_XFAST 1
 #endif
 constexpr int BRICK_X = PUFF_BRICK_X, BRICK_Y = PUFF_BRICK_Y, BRICK_Z = PUFF_BRICK_Z;
+// Re-sort trigger: the share of stage workgroups that overflowed the LDS
+// tile in a step exceeds both PUFF_RESORT_MIN_SHARE and PUFF_RESORT_GROWTH
+// times its value on the first step after the last sort.
+#ifndef PUFF_RESORT_MIN_SHARE
+#define PUFF_RESORT_MIN_SHARE 0.02
+#endif
+#ifndef PUFF_RESORT_GROWTH
+#define PUFF_RESORT_GROWTH 2.0
+#endif
 constexpr bool BRICK_XFAST = PUFF_BRICK_XFAST != 0;
 constexpr int BRICK_CELLS = BRICK_X * BRICK_Y * BRICK_Z;
 
