@@ -139,9 +139,13 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
                 }
             }
         } else {
-            // direct DFT pass for an odd prime r: the outputs q and r - q share
-            // the pre-twiddled inputs y_j = x_j W^(j k twst), with
-            // X_q = sum y_j w^(jq) and X_(r-q) = sum y_j conj(w^(jq)).
+            // direct DFT pass for an odd prime r, folded: with the pre-twiddled
+            // inputs y_j = x_j W^(j k twst), u_j = y_j + y_(r-j),
+            // v_j = y_j - y_(r-j) (j = 1 .. (r-1)/2) and w^(jq) = c + i s
+            // (w^((r-j)q) = c - i s),
+            //   X_q     = y_0 + sum u_j c + i sum v_j s
+            //   X_(r-q) = y_0 + sum u_j c - i sum v_j s,
+            // four FMAs per input pair for both outputs.
             const int nr = N / r;
             const int hq = (r + 1) / 2;
             for (int w = threadIdx.x; w < B * T * hq; w += blockDim.x) {
@@ -152,25 +156,32 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
                 const int k = i % p;
                 const int dk = k * twst;
                 const int dq = q * nr;
-                int e = 0, f = 0;
-                R s0r = 0, s0i = 0, s1r = 0, s1i = 0;
-                for (int j = 0; j < r; ++j) {
-                    cplx<R> x = src[lidx<IL>(b, i + j * T, B, LD)];
-                    if (e) x = cmul(x, tw_get(tw, e, sign));
+                const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
+                int e1 = dk, e2 = (int)(((int64_t)(r - 1) * dk) % N), f = dq;
+                R sr = 0, si = 0, tr = 0, ti = 0;
+#pragma unroll 4
+                for (int j = 1; j < hq; ++j) {
+                    cplx<R> xa = src[lidx<IL>(b, i + j * T, B, LD)];
+                    cplx<R> xb = src[lidx<IL>(b, i + (r - j) * T, B, LD)];
+                    if (dk) {
+                        xa = cmul(xa, tw_get(tw, e1, sign));
+                        xb = cmul(xb, tw_get(tw, e2, sign));
+                    }
                     const cplx<R> om = tw_get(tw, f, sign);
-                    const R ac = x.x * om.x, bs = x.y * om.y, as = x.x * om.y, bc = x.y * om.x;
-                    s0r += ac - bs;
-                    s0i += as + bc;
-                    s1r += ac + bs;
-                    s1i += bc - as;
-                    e += dk;
-                    if (e >= N) e -= N;
+                    sr = fma(xa.x + xb.x, om.x, sr);
+                    si = fma(xa.y + xb.y, om.x, si);
+                    tr = fma(xa.x - xb.x, om.y, tr);
+                    ti = fma(xa.y - xb.y, om.y, ti);
+                    e1 += dk;
+                    if (e1 >= N) e1 -= N;
+                    e2 -= dk;
+                    if (e2 < 0) e2 += N;
                     f += dq;
                     if (f >= N) f -= N;
                 }
                 const int ob = (i / p) * p * r + k;
-                dst[lidx<IL>(b, ob + q * p, B, LD)] = {s0r, s0i};
-                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, LD)] = {s1r, s1i};
+                dst[lidx<IL>(b, ob + q * p, B, LD)] = {y0.x + sr - ti, y0.y + si + tr};
+                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, LD)] = {y0.x + sr + ti, y0.y + si - tr};
             }
         }
         __syncthreads();
@@ -454,7 +465,7 @@ static void lds_opt_in(const void* kernel, size_t bytes) {
 #define PUFF_Z_LDS_BUDGET (32 * 1024)
 #endif
 #ifndef PUFF_XY_LDS_BUDGET
-#define PUFF_XY_LDS_BUDGET (24 * 1024)
+#define PUFF_XY_LDS_BUDGET (32 * 1024)
 #endif
 static int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_LDS_BUDGET) {
     const size_t tw = (size_t)n * elem;
