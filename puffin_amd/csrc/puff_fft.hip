@@ -129,6 +129,7 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
     for (int ps = 0; ps < npass; ++ps) {
         const int r = radix[ps];
         const int T = N / r;
+        // twiddle exponents j k twst (j < r, k < p) stay below N: no reduction
         const int twst = N / (p * r);
         if (r == 16) {
             for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
@@ -140,7 +141,7 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
-                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, (j * k * twst) % N, sign));
+                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
                 }
                 dft16(v, sign);
 #pragma unroll
@@ -158,7 +159,7 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
                 cplx<R> v[4];
                 for (int j = 0; j < r; ++j) {
                     v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
-                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, (j * k * twst) % N, sign));
+                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
                 }
                 if (r == 2) {
                     dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], v[1]);
@@ -202,7 +203,7 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
                 const int dk = k * twst;
                 const int dq = q * nr;
                 const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
-                int e1 = dk, e2 = (int)(((int64_t)(r - 1) * dk) % N), f = dq;
+                int e1 = dk, e2 = (r - 1) * dk, f = dq;   // e1, e2 < N throughout
                 R sr = 0, si = 0, tr = 0, ti = 0;
 #pragma unroll 4
                 for (int j = 1; j < hq; ++j) {
@@ -218,9 +219,7 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
                     tr = fma(xa.x - xb.x, om.y, tr);
                     ti = fma(xa.y - xb.y, om.y, ti);
                     e1 += dk;
-                    if (e1 >= N) e1 -= N;
                     e2 -= dk;
-                    if (e2 < 0) e2 += N;
                     f += dq;
                     if (f >= N) f -= N;
                 }
@@ -316,6 +315,9 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
     for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
     cplx<R>* base = a.a + L0 * N;
     const int64_t ntr = (int64_t)a.nx * a.ny;
+    // plane and row of the block's first line; line L0 + b is row iy0 + b
+    // counted on from there (b < B)
+    const int iy0 = (int)(L0 % a.ny), iz0 = (int)(L0 / a.ny);
     if (op == XOP_ADJ_FOLD_SAVE_FWD) {
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w / N;
@@ -346,8 +348,8 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
         const cplx<R>* lam = a.anb + L0 * N;
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
             const int b = w / N, j = w % N;
-            const int64_t L = L0 + b;
-            const int iy = (int)(L % a.ny), iz = (int)(L / a.ny);
+            const int lb = iy0 + b;   // (iy, iz) of line L0 + b
+            const int iy = lb % a.ny, iz = iz0 + lb / a.ny;
             const double mxy = a.mxy[(int64_t)iy * a.nx + j];
             const double m = mxy + (a.mz[iz] * (1.0 - mxy));
             const cplx<R> v = res[w];
@@ -369,8 +371,8 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
         cplx<R>* anb = a.anb + L0 * N;
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
             const int b = w / N, j = w % N;
-            const int64_t L = L0 + b;
-            const int iy = (int)(L % a.ny), iz = (int)(L / a.ny);
+            const int lb = iy0 + b;   // (iy, iz) of line L0 + b
+            const int iy = lb % a.ny, iz = iz0 + lb / a.ny;
             const double mxy = a.mxy[(int64_t)iy * a.nx + j];
             const double m = mxy + (a.mz[iz] * (1.0 - mxy));
             cplx<R> v = res[w];
