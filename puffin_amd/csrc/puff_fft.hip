@@ -293,11 +293,104 @@ __device__ cplx<R>* lds_rader(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& 
 
 // In-LDS transform of B lines of length ax.n.  src holds the input; returns
 // the buffer holding the output (src or dst).
+// Radix-2/3/4 butterfly in place on v[0..r-1] (same arithmetic as the
+// ping-pong passes).
+template <typename R>
+__device__ __forceinline__ void small_radix(cplx<R>* v, int r, int sign) {
+    if (r == 2) {
+        const cplx<R> t = v[0];
+        v[0] = cadd(t, v[1]);
+        v[1] = csub(t, v[1]);
+    } else if (r == 3) {
+        const R s3 = (R)0.86602540378443864676 * (R)(sign < 0 ? -1 : 1);
+        const cplx<R> t1 = cadd(v[1], v[2]);
+        const cplx<R> t2 = {v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y};
+        const cplx<R> d = csub(v[1], v[2]);
+        const cplx<R> t3 = {-s3 * d.y, s3 * d.x};
+        v[0] = cadd(v[0], t1);
+        v[1] = cadd(t2, t3);
+        v[2] = csub(t2, t3);
+    } else {
+        dft4(v[0], v[1], v[2], v[3], sign);
+    }
+}
+
+// In-place form of lds_passes for lines whose passes are all radix 16, 4, 3
+// or 2, with at most one butterfly per thread in every pass (B N / r <=
+// blockDim, checked on the host): each thread reads its butterfly's inputs
+// into registers, the workgroup passes a barrier, and the outputs overwrite
+// the same buffer.  Half the LDS of the ping-pong form, hence more
+// workgroups per CU.
+template <typename R, bool IL>
+__device__ void lds_passes_inplace(cplx<R>* buf, int B, int N, int npass, const int* radix,
+                                   const cplx<R>* tw, int sign) {
+    int p = 1;
+    for (int ps = 0; ps < npass; ++ps) {
+        const int r = radix[ps];
+        const int T = N / r;
+        const int twst = N / (p * r);   // j k twst < N, as in lds_passes
+        const int w = threadIdx.x;
+        const bool act = w < B * T;
+        int b = 0, ob = 0;
+        cplx<R> v[16];
+        if (act) {
+            int i;
+            if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
+            const int k = i % p;
+            ob = (i / p) * p * r + k;
+            if (r == 16) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
+                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
+                }
+                dft16(v, sign);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j < r) {
+                        v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
+                        if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
+                    }
+                }
+                small_radix(v, r, sign);
+            }
+        }
+        __syncthreads();
+        if (act) {
+            if (r == 16) {
+#pragma unroll
+                for (int q0 = 0; q0 < 4; ++q0)
+#pragma unroll
+                    for (int q1 = 0; q1 < 4; ++q1)
+                        buf[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, N)] = v[4 * q0 + q1];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < r) buf[lidx<IL>(b, ob + q * p, B, N)] = v[q];
+            }
+        }
+        __syncthreads();
+        p *= r;
+    }
+}
+
 template <typename R, bool IL>
 __device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
                             const cplx<R>* tw, int sign) {
     if (ax.rader) return lds_rader<R, IL>(src, dst, B, ax, tw, sign);
     return lds_passes<R, IL>(src, dst, B, ax.n, ax.n, ax.npass, ax.radix, tw, sign);
+}
+
+// z2-line transform: in place (one LDS buffer) or ping-pong.
+template <typename R, bool INPLACE>
+__device__ __forceinline__ cplx<R>* z_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
+                                          const cplx<R>* tw, int sign) {
+    if (INPLACE) {
+        lds_passes_inplace<R, true>(src, B, ax.n, ax.npass, ax.radix, tw, sign);
+        return src;
+    }
+    return lds_fft<R, true>(src, dst, B, ax, tw, sign);
 }
 
 // ---- x lines (contiguous) --------------------------------------------------
@@ -441,12 +534,12 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int B, int sign) {
 #ifndef PUFF_Z_THREADS
 #define PUFF_Z_THREADS 256
 #endif
-template <typename R>
+template <typename R, bool INPLACE>
 __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, int op, int sign) {
     extern __shared__ unsigned char smem_raw[];
     const int N = a.nzp;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* s1 = s0 + B * N;
+    cplx<R>* s1 = INPLACE ? s0 : s0 + B * N;
     // blocks of B x-columns, x fastest; neighbouring blocks share the cache
     // lines of every z2 plane, so they go to one XCD
     const int nxb = (a.nx + B - 1) / B;
@@ -466,9 +559,9 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, 
     __syncthreads();
     cplx<R>* res;
     if (op < 0) {
-        res = lds_fft<R, true>(s0, s1, B, a.az, tw, sign);
+        res = z_fft<R, INPLACE>(s0, s1, B, a.az, tw, sign);
     } else {
-        res = lds_fft<R, true>(s0, s1, B, a.az, tw, -1);
+        res = z_fft<R, INPLACE>(s0, s1, B, a.az, tw, -1);
         const double ky = a.ky[iy];
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w % B, q = w / B;
@@ -500,7 +593,7 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, 
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = lds_fft<R, true>(res, other, B, a.az, tw, +1);
+        res = z_fft<R, INPLACE>(res, other, B, a.az, tw, +1);
     }
     lds_drain<1>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
@@ -553,14 +646,42 @@ static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
 
 template <typename R>
 static void run_z(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
-    // two line buffers, no twiddle table (wider workgroups with longer runs
-    // along x measured slower: occupancy matters more than coalescing here)
+    // In place (one LDS buffer) when every pass is radix 16, 4, 3 or 2 with
+    // at most one butterfly per thread, and that leaves at least as many
+    // lines per workgroup as two buffers do: C1's 1024-point lines in FP64
+    // (616 -> 454 us per pass) and C4's 4096-point lines (7.50 -> 6.92 ms).
+    // Otherwise two line buffers, no twiddle table (wider workgroups with
+    // longer runs along x measured slower: occupancy matters more than
+    // coalescing here).
+    const FftAxis& ax = a.az;
+    bool inplace = !ax.rader;
+    int minr = 16;
+    for (int k = 0; k < ax.npass; ++k) {
+        const int r = ax.radix[k];
+        if (r != 2 && r != 3 && r != 4 && r != 16) inplace = false;
+        if (r < minr) minr = r;
+    }
     int B = (int)((PUFF_Z_LDS_BUDGET) / (2 * (size_t)a.nzp * sizeof(cplx<R>)));
-    B = B < 1 ? 1 : (B > 16 ? 16 : B);
-    const size_t sh = 2 * (size_t)B * a.nzp * sizeof(cplx<R>);
-    lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R>), sh);
+    B = B < 1 ? 1 : (B > 16 ? 16 : B);   // lines per workgroup with two buffers
+    if (inplace) {
+        int Bi = (int)((PUFF_Z_LDS_BUDGET) / ((size_t)a.nzp * sizeof(cplx<R>)));
+        Bi = Bi > 16 ? 16 : Bi;
+        const int bt = PUFF_Z_THREADS / (a.nzp / minr);   // one butterfly per thread
+        Bi = Bi > bt ? bt : Bi;
+        inplace = Bi >= 1 && Bi >= B;
+        if (inplace) B = Bi;
+    }
+    const size_t sh = (inplace ? 1 : 2) * (size_t)B * a.nzp * sizeof(cplx<R>);
     const int64_t blocks = xcd_grid((int64_t)((a.nx + B - 1) / B) * a.ny);
-    hipLaunchKernelGGL((k_fft_z<R>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op, sign);
+    if (inplace) {
+        lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R, true>), sh);
+        hipLaunchKernelGGL((k_fft_z<R, true>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op,
+                           sign);
+    } else {
+        lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R, false>), sh);
+        hipLaunchKernelGGL((k_fft_z<R, false>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op,
+                           sign);
+    }
 }
 
 template <typename R>
