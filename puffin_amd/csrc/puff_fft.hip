@@ -194,17 +194,24 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
             // four FMAs per input pair for both outputs.
             const int nr = N / r;
             const int hq = (r + 1) / 2;
-            for (int w = threadIdx.x; w < B * T * hq; w += blockDim.x) {
+            // each work item forms the outputs of two values of q (qa = 2 qq
+            // and qb = qa + 1, with their mirrors r - q), sharing the input
+            // loads, the folding and the loop bookkeeping
+            const int nqq = (hq + 1) / 2;
+            for (int w = threadIdx.x; w < B * T * nqq; w += blockDim.x) {
                 int b, t;
-                if (IL) { b = w % B; t = w / B; } else { t = w % (T * hq); b = w / (T * hq); }
-                const int q = t / T;
+                if (IL) { b = w % B; t = w / B; } else { t = w % (T * nqq); b = w / (T * nqq); }
+                const int qa = 2 * (t / T);
+                const bool hasb = qa + 1 < hq;
+                const int qb = hasb ? qa + 1 : 0;   // qb = 0: a valid table index, result unused
                 const int i = t % T;
                 const int k = i % p;
                 const int dk = k * twst;
-                const int dq = q * nr;
+                const int dqa = qa * nr, dqb = qb * nr;
                 const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
-                int e1 = dk, e2 = (r - 1) * dk, f = dq;   // e1, e2 < N throughout
-                R sr = 0, si = 0, tr = 0, ti = 0;
+                int e1 = dk, e2 = (r - 1) * dk, fa = dqa, fb = dqb;   // e1, e2 < N throughout
+                R sra = 0, sia = 0, tra = 0, tia = 0;
+                R srb = 0, sib = 0, trb = 0, tib = 0;
 #pragma unroll 4
                 for (int j = 1; j < hq; ++j) {
                     cplx<R> xa = src[lidx<IL>(b, i + j * T, B, LD)];
@@ -213,19 +220,32 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
                         xa = cmul(xa, tw_get(tw, e1, sign));
                         xb = cmul(xb, tw_get(tw, e2, sign));
                     }
-                    const cplx<R> om = tw_get(tw, f, sign);
-                    sr = fma(xa.x + xb.x, om.x, sr);
-                    si = fma(xa.y + xb.y, om.x, si);
-                    tr = fma(xa.x - xb.x, om.y, tr);
-                    ti = fma(xa.y - xb.y, om.y, ti);
+                    const R ux = xa.x + xb.x, uy = xa.y + xb.y;
+                    const R vx = xa.x - xb.x, vy = xa.y - xb.y;
+                    const cplx<R> oa = tw_get(tw, fa, sign);
+                    const cplx<R> obw = tw_get(tw, fb, sign);
+                    sra = fma(ux, oa.x, sra);
+                    sia = fma(uy, oa.x, sia);
+                    tra = fma(vx, oa.y, tra);
+                    tia = fma(vy, oa.y, tia);
+                    srb = fma(ux, obw.x, srb);
+                    sib = fma(uy, obw.x, sib);
+                    trb = fma(vx, obw.y, trb);
+                    tib = fma(vy, obw.y, tib);
                     e1 += dk;
                     e2 -= dk;
-                    f += dq;
-                    if (f >= N) f -= N;
+                    fa += dqa;
+                    if (fa >= N) fa -= N;
+                    fb += dqb;
+                    if (fb >= N) fb -= N;
                 }
                 const int ob = (i / p) * p * r + k;
-                dst[lidx<IL>(b, ob + q * p, B, LD)] = {y0.x + sr - ti, y0.y + si + tr};
-                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, LD)] = {y0.x + sr + ti, y0.y + si - tr};
+                dst[lidx<IL>(b, ob + qa * p, B, LD)] = {y0.x + sra - tia, y0.y + sia + tra};
+                if (qa > 0) dst[lidx<IL>(b, ob + (r - qa) * p, B, LD)] = {y0.x + sra + tia, y0.y + sia - tra};
+                if (hasb) {
+                    dst[lidx<IL>(b, ob + qb * p, B, LD)] = {y0.x + srb - tib, y0.y + sib + trb};
+                    dst[lidx<IL>(b, ob + (r - qb) * p, B, LD)] = {y0.x + srb + tib, y0.y + sib - trb};
+                }
             }
         }
         __syncthreads();
