@@ -638,6 +638,12 @@ static void lds_opt_in(const void* kernel, size_t bytes) {
 #ifndef PUFF_XY_LDS_BUDGET
 #define PUFF_XY_LDS_BUDGET (32 * 1024)
 #endif
+// Threads per x/y workgroup on a Rader axis.  Its radix-16 sub-passes keep
+// B x 16 of them busy (112 for C4's 257-point lines), but 128 and 192
+// threads measured 15 % and 7 % slower C4 events than 256.
+#ifndef PUFF_RADER_THREADS
+#define PUFF_RADER_THREADS 256
+#endif
 static int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_LDS_BUDGET) {
     const size_t tw = (size_t)n * elem;
     int b = budget > tw ? (int)((budget - tw) / (2 * (size_t)n * elem)) : 0;
@@ -652,7 +658,8 @@ static void run_x(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
     const int64_t nlines = (int64_t)a.ny * a.nzp;
     const size_t sh = (2 * (size_t)B + 1) * a.nx * sizeof(cplx<R>);
     lds_opt_in(reinterpret_cast<const void*>(&k_fft_x<R>), sh);
-    hipLaunchKernelGGL((k_fft_x<R>), dim3((unsigned)((nlines + B - 1) / B)), dim3(256), sh, s, a, B, op, sign);
+    const int thr = a.ax.rader ? PUFF_RADER_THREADS : 256;
+    hipLaunchKernelGGL((k_fft_x<R>), dim3((unsigned)((nlines + B - 1) / B)), dim3(thr), sh, s, a, B, op, sign);
 }
 
 template <typename R>
@@ -660,7 +667,8 @@ static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
     const int B = lines_per_block(a.ny, sizeof(cplx<R>), 16);
     const size_t sh = (2 * (size_t)B + 1) * a.ny * sizeof(cplx<R>);
     lds_opt_in(reinterpret_cast<const void*>(&k_fft_y<R>), sh);
-    hipLaunchKernelGGL((k_fft_y<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(256), sh, s,
+    const int thr = a.ay.rader ? PUFF_RADER_THREADS : 256;
+    hipLaunchKernelGGL((k_fft_y<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(thr), sh, s,
                        a, B, sign);
 }
 
