@@ -49,6 +49,11 @@ def stage_bytes(precision: int):
 
 
 def run_gpu(case, precision: int, steps: int, warmup: int):
+    """Warm up on the module's schedule (diffraction every iSteps4Diff steps),
+    then time `steps` RK4 steps and, separately, E = ceil(steps / iSteps4Diff)
+    diffraction events, each segment bracketed by device synchronisation.
+    A step is one RK4 step plus 1 / iSteps4Diff of a diffraction event:
+    ms_per_step = rk4_ms + event_ms / iSteps4Diff, whatever `steps` is."""
     from puffin_amd import gpu
     mp = case.mp
     name = "fp64" if precision == 0 else "mixed"
@@ -60,36 +65,41 @@ def run_gpu(case, precision: int, steps: int, warmup: int):
         g.set_taper(case.n2col0, case.n2col0, case.undgrad)
         h, s4d = case.h, case.steps4diff
         szl = 0.0
-        istep = 0
-
-        def step():
-            nonlocal szl, istep
+        for istep in range(1, warmup + 1):
             st = g.rk4par(szl, h)
             if st != 0:
-                raise RuntimeError(f"rk4par status {st} at step {istep}")
+                raise RuntimeError(f"rk4par status {st} at warm-up step {istep}")
             szl += h
-            istep += 1
             if istep % s4d == 0:
                 g.diffraction_step(s4d * h)
-
-        for _ in range(warmup):
-            step()
+        g.diffraction_step(s4d * h)
         g.synchronize()
         g.profile(True)
         g.profile_read(reset=True)
-        log(f"{name}: timing {steps} steps")
+        events = max(1, -(-steps // s4d))
+        log(f"{name}: timing {steps} RK4 steps and {events} diffraction events")
         t0 = time.perf_counter()
-        for _ in range(steps):
-            step()
+        for istep in range(steps):
+            st = g.rk4par(szl, h)
+            if st != 0:
+                raise RuntimeError(f"rk4par status {st} at timed step {istep}")
+            szl += h
         g.synchronize()
         t1 = time.perf_counter()
+        for _ in range(events):
+            g.diffraction_step(s4d * h)
+        g.synchronize()
+        t2 = time.perf_counter()
         prof = g.profile_read(reset=True)
         g.profile(False)
         box = g.active_box()
         power = g.power()
-        ms = (t1 - t0) * 1e3 / steps
-        log(f"{name}: {ms:.3f} ms/step")
-        return dict(ms_per_step=ms, prof=prof, box=box, power=power)
+        rk4_ms = (t1 - t0) * 1e3 / steps
+        event_ms = (t2 - t1) * 1e3 / events
+        ms = rk4_ms + event_ms / s4d
+        log(f"{name}: {ms:.3f} ms/step ({rk4_ms:.3f} RK4 + {event_ms:.3f} / {s4d} diffraction)")
+        return dict(ms_per_step=ms, rk4_ms=rk4_ms, event_ms=event_ms, events=events,
+                    timed_s=t2 - t0, prof=prof, box=box, power=power)
     finally:
         g.close()
 
@@ -148,9 +158,21 @@ def integrated_power(power: np.ndarray, dz2: float) -> float:
     return float(np.trapezoid(power, dx=dz2)) if hasattr(np, "trapezoid") else float(np.trapz(power, dx=dz2))
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(case, nsample: int, steps: int):
     """The CPU oracle (FP64 C restatement, OpenMP, atomic deposit as the
-    reference's !$OMP ATOMIC) on a sample of the same workload."""
+    reference's !$OMP ATOMIC) on a sample of the same workload: `steps` RK4
+    steps of a sample of the beam on the full mesh, and one diffraction event
+    of the full mesh, amortised over iSteps4Diff steps as on the GPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -162,7 +184,7 @@ def cpu_baseline(case, nsample: int, steps: int):
     orc.set_deposit_atomic(True)
     o = orc.Oracle(case.mp, parts, case.field, taper=(case.n2col0, case.n2col0, case.undgrad),
                    nthreads=threads)
-    log(f"cpu baseline: {idx.size} macroparticles, {threads} threads")
+    log(f"cpu baseline: {idx.size} macroparticles, {threads} threads, {cpu_model()}")
     szl = 0.0
     st = o.rk4par(szl, case.h)   # warm-up
     szl += case.h
@@ -171,38 +193,44 @@ def cpu_baseline(case, nsample: int, steps: int):
         st = o.rk4par(szl, case.h)
         szl += case.h
     t1 = time.perf_counter()
+    o.diffraction_step(case.steps4diff * case.h)
+    t2 = time.perf_counter()
     orc.set_deposit_atomic(False)
     assert st == 0, st
-    val = idx.size * steps / (t1 - t0)
-    log(f"cpu baseline: {val:.4g} macroparticle-steps/s")
+    step_s = (t1 - t0) / steps + (t2 - t1) / case.steps4diff
+    val = idx.size / step_s
+    log(f"cpu baseline: {val:.4g} macroparticle-steps/s ({(t1 - t0) / steps:.3f} s per RK4 step, "
+        f"{t2 - t1:.2f} s per diffraction event)")
     return dict(value=val, unit="macroparticle-steps/s", cores=threads, kind="port",
+                cpu=cpu_model(), rk4_s_per_step=(t1 - t0) / steps, diffraction_s_per_event=t2 - t1,
                 sample=(f"{idx.size} of the {case.n} macroparticles on the full "
-                        f"{case.mp.nx}x{case.mp.ny}x{case.mp.nz2} mesh, {steps} RK4 steps "
-                        f"(no diffraction), FP64 C oracle with OpenMP atomics"))
+                        f"{case.mp.nx}x{case.mp.ny}x{case.mp.nz2} mesh: {steps} RK4 steps plus one "
+                        f"diffraction event of the full mesh amortised over {case.steps4diff} steps; "
+                        f"FP64 C oracle with OpenMP atomics"))
 
 
 def accuracy_run(case, precision: int, steps: int):
-    """The module's schedule (rk4par, diffraction every iSteps4Diff steps) for
-    ``steps`` steps; returns power(z2), slice bunching |b| and the on-axis
-    spectrum of the final field."""
+    """The module's schedule (UndSection: first half diffraction step, rk4par,
+    merged diffraction every iSteps4Diff steps, last half step) for `steps`
+    steps; returns the final power(z2), slice bunching |b| and the on-axis
+    spectrum."""
     from puffin_amd import gpu
+    from puffin_amd.undulator import und_section
     mp = case.mp
     g = gpu.Integrator(mp, case.n, precision)
     try:
         g.upload_particles(case.particles)
         g.upload_field(*case.field)
         g.set_taper(case.n2col0, case.n2col0, case.undgrad)
-        szl = 0.0
+        if case.taper_table is not None:
+            g.set_taper_table(case.taper_table, 0.0, case.taper_period)
         t0 = time.perf_counter()
-        for istep in range(1, steps + 1):
-            st = g.rk4par(szl, case.h)
-            if st != 0:
-                raise RuntimeError(f"rk4par status {st} at step {istep}")
-            szl += case.h
-            if istep % case.steps4diff == 0:
-                g.diffraction_step(case.steps4diff * case.h)
+        res = und_section(g, steps, case.h, diffraction=True, steps4diff=case.steps4diff,
+                          periodic=bool(mp.periodic))
         g.synchronize()
         wall = time.perf_counter() - t0
+        if res.status != 0:
+            raise RuntimeError(f"rk4par status {res.status} after {res.steps_done} steps")
         power = g.power()
         lam = 4.0 * math.pi * mp.rho
         nsl = int(math.ceil((mp.nz2 - 1) * mp.dz2 / lam))
@@ -216,11 +244,13 @@ def accuracy_run(case, precision: int, steps: int):
     spec = np.abs(np.fft.fft(axis)) ** 2
     spec = spec[: mp.nzp // 2 + 1]                  # one-sided (plotSpecInt.py:79-100)
     return dict(power=power, bunching=bunch, spectrum=spec, wall_s=wall,
-                integrated_power=integrated_power(power, mp.dz2))
+                integrated_power=integrated_power(power, mp.dz2), steps=res.steps_done,
+                diffractions=res.n_diffractions)
 
 
 def precision_report(case, steps: int):
-    """Error of the FP32-mixed form against FP64 per output after ``steps``."""
+    """Error of the FP32-mixed form against FP64 per output after `steps`
+    steps of the module's schedule."""
     log(f"accuracy: {steps} steps FP64 and mixed")
     r64 = accuracy_run(case, 0, steps)
     r32 = accuracy_run(case, 1, steps)
@@ -231,6 +261,8 @@ def precision_report(case, steps: int):
 
     p0 = integrated_power(np.asarray(case_power0(case)), case.mp.dz2)
     out = dict(steps=steps, periods=steps * case.h / (4.0 * math.pi * case.mp.rho),
+               diffraction_events=r64["diffractions"],
+               integrated_power_fp64=r64["integrated_power"],
                integrated_power_rel_err=abs(r32["integrated_power"] - r64["integrated_power"]) /
                abs(r64["integrated_power"]),
                power_z_max_rel_err=err("power"), bunching_max_rel_err=err("bunching"),
@@ -434,6 +466,47 @@ def c4_bench(steps: int = 30, adjoint_steps: int = 30, every: int = 4, captures:
     return out
 
 
+def c4_full_module(precision: int = 1):
+    """The whole 128-period module of the large workload (3840 steps on
+    UndSection's schedule, 257 x 257 x 4096 mesh, 2^27 macroparticles):
+    status, integrated power every 16 periods and the final outputs."""
+    from puffin_amd import gpu
+    from puffin_amd.configs import case_c4
+    from puffin_amd.undulator import und_section
+    c = case_c4()
+    mp = c.mp
+    out = dict(workload="planar FEL, temporal mesh 257x257x4096, 2^27 macroparticles, whole module",
+               steps=c.nsteps, precision="fp64" if precision == 0 else "mixed", powers=[])
+    g = gpu.Integrator(mp, c.n, precision)
+    try:
+        g.upload_particles(c.particles)
+        g.upload_field(*c.field)
+        g.set_taper_table(c.taper_table, 0.0, c.taper_period)
+        t0 = time.perf_counter()
+
+        def write(istep):
+            p = integrated_power(g.power(), mp.dz2)
+            out["powers"].append([istep, p])
+            log(f"c4 module: step {istep}, integrated power {p:.6e}, "
+                f"{time.perf_counter() - t0:.1f} s")
+
+        res = und_section(g, c.nsteps, c.h, diffraction=True, steps4diff=c.steps4diff,
+                          write=write, write_every=480, int_write_every=480)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        out.update(status=res.status, steps_done=res.steps_done, wall_s=wall,
+                   ms_per_step=wall * 1e3 / max(res.steps_done, 1),
+                   diffraction_events=res.n_diffractions)
+        if res.status == 0:
+            q = g.download_particles()
+            out["max_z2_node"] = float(q["z2"].max() / mp.dz2)
+            out["integrated_power"] = integrated_power(g.power(), mp.dz2)
+    finally:
+        g.close()
+    log(f"c4 module: status {out['status']}, {out['steps_done']} steps in {out['wall_s']:.1f} s")
+    return out
+
+
 def copy_rate(mib: int = 1024, reps: int = 10) -> float:
     """Achievable streaming rate (TB/s): a device-to-device copy of mib MiB,
     read plus write bytes over the time, timed with events on torch's
@@ -474,15 +547,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--particles", type=int, default=1 << 24)
     ap.add_argument("--precision", choices=("both", "fp64", "mixed"), default="both")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--case", choices=("c1", "c0", "c3"), default="c1",
                     help="workload (the JSON line is quoted on c1)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
-    ap.add_argument("--accuracy-periods", type=int, default=10,
-                    help="undulator periods of the FP64-vs-mixed accuracy run (0: skip)")
+    ap.add_argument("--accuracy-periods", type=int, default=-1,
+                    help="undulator periods of the FP64-vs-mixed accuracy run "
+                         "(-1: the whole module, 0: skip)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the gradient measurement")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the gradient measurement (its JSON object on stdout)")
@@ -491,6 +565,8 @@ def main():
                     help="only the diffraction event on the C1 and C4 meshes (its JSON object on stdout)")
     ap.add_argument("--c4-only", action="store_true",
                     help="only the large FP32-mixed workload (its JSON object on stdout)")
+    ap.add_argument("--c4-module", action="store_true",
+                    help="only the whole 3840-step module of the large workload (JSON on stdout)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -500,6 +576,9 @@ def main():
         return
     if args.c4_only:
         print(json.dumps(c4_bench()), flush=True)
+        return
+    if args.c4_module:
+        print(json.dumps(c4_full_module()), flush=True)
         return
     if args.diffraction_only:
         print(json.dumps(diffraction_bench()), flush=True)
@@ -524,12 +603,18 @@ def main():
         r = results[name]
         kernels, dom, n_act, sbytes = roofline(case, prec, r, args.steps)
         k = kernels[dom]
+        traffic = (measured_traffic(prec, dom) if case.n == 1 << 24 and args.case == "c1" else None)
         return dict(
             value=case.n / (r["ms_per_step"] * 1e-3),
             ms_per_step=r["ms_per_step"],
+            ms_per_step_rk4_only=r["rk4_ms"],
+            diffraction_event_ms=r["event_ms"],
             roofline=dict(bound="hbm", kernel=dom, achieved=k["GBps"], peak=HBM_PEAK_GBS, unit="GB/s",
-                          frac=k["frac"], traffic=measured_traffic(prec, dom) if case.n == 1 << 24 and
-                          args.case == "c1" else None),
+                          frac=k["frac"], traffic=traffic,
+                          traffic_source=(f"profiles/hbm_traffic_c1_{name}.md (rocprofv3 --pmc "
+                                          "FETCH_SIZE and WRITE_SIZE passes of this workload, "
+                                          "scaled by a copy-kernel calibration), not measured in "
+                                          "this run") if traffic is not None else None),
             step_roofline=dict(bytes_per_step=sbytes,
                                achieved=sbytes / (r["ms_per_step"] * 1e-3) / 1e9,
                                frac=sbytes / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS),
@@ -549,6 +634,11 @@ def main():
         steps=args.steps,
         warmup=args.warmup,
         ms_per_step=prim["ms_per_step"],
+        ms_per_step_rk4_only=prim["ms_per_step_rk4_only"],
+        diffraction_event_ms=prim["diffraction_event_ms"],
+        timing=("ms_per_step = (wall time of `steps` RK4 steps) / steps + (wall time of "
+                "ceil(steps / steps_per_diffraction) diffraction events) / events / "
+                "steps_per_diffraction; each segment synchronised on the device"),
         higher_is_better=True,
         scaling="weak",
         vs_baseline=None,
@@ -570,12 +660,15 @@ def main():
         mx = summary("mixed", 1)
         perr = abs(mx["integrated_power"] - prim["integrated_power"]) / abs(prim["integrated_power"])
         line["mixed"] = dict(value=mx["value"], ms_per_step=mx["ms_per_step"],
+                             ms_per_step_rk4_only=mx["ms_per_step_rk4_only"],
+                             diffraction_event_ms=mx["diffraction_event_ms"],
                              speedup_vs_fp64=mx["value"] / prim["value"], roofline=mx["roofline"],
                              step_roofline=mx["step_roofline"], kernels=mx["kernels"],
                              integrated_power_rel_err=perr,
                              power_err_after_steps=args.warmup + args.steps)
-    if "mixed" in line and args.case == "c1" and args.accuracy_periods > 0:
-        line["mixed"]["accuracy"] = precision_report(case, 30 * args.accuracy_periods)
+    if "mixed" in line and args.case == "c1" and args.accuracy_periods != 0:
+        acc_steps = case.nsteps if args.accuracy_periods < 0 else 30 * args.accuracy_periods
+        line["mixed"]["accuracy"] = precision_report(case, acc_steps)
     if not args.no_adjoint and args.case == "c1":
         line["adjoint"] = adjoint_bench()
     if not args.no_c4 and args.case == "c1":

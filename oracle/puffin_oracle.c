@@ -638,9 +638,25 @@ static int orc_smallest_factor(int n) {
     return n;
 }
 
-/* y[k] = sum_j x[j*xs] exp(sign 2 pi i j k / n); y contiguous. */
-static void orc_fft_rec(int n, const double* x, int64_t xs, double* y, int sign,
-                        double* tmp) {
+/* Twiddle table of a top-level length N: tw[e] = exp(sign 2 pi i e / N) for
+   e = 0..N-1.  A sub-transform of length n = N / twst reads its factor
+   exp(sign 2 pi i e / n) at index e * twst: e twst / N and e / n are the same
+   rational, so the table holds the very doubles a direct cos / sin of
+   sign 2 pi (e / n) gives. */
+static double* orc_twiddles(int n, int sign) {
+    double* tw = (double*)malloc(sizeof(double) * 2 * n);
+    for (int e = 0; e < n; e++) {
+        double a = sign * 2.0 * M_PI * (double)e / (double)n;
+        tw[2 * e] = cos(a);
+        tw[2 * e + 1] = sin(a);
+    }
+    return tw;
+}
+
+/* y[k] = sum_j x[j*xs] exp(sign 2 pi i j k / n); y contiguous.  tw is the
+   table of length n * twst (orc_twiddles). */
+static void orc_fft_rec(int n, const double* x, int64_t xs, double* y, double* tmp,
+                        const double* tw, int64_t twst) {
     if (n == 1) {
         y[0] = x[0];
         y[1] = x[1];
@@ -653,8 +669,7 @@ static void orc_fft_rec(int n, const double* x, int64_t xs, double* y, int sign,
             double sr = 0.0, si = 0.0;
             for (int j = 0; j < n; j++) {
                 int64_t e = ((int64_t)j * k) % n;
-                double a = sign * 2.0 * M_PI * (double)e / (double)n;
-                double c = cos(a), s = sin(a);
+                double c = tw[2 * e * twst], s = tw[2 * e * twst + 1];
                 double xr = x[2 * j * xs], xi = x[2 * j * xs + 1];
                 sr += xr * c - xi * s;
                 si += xr * s + xi * c;
@@ -665,15 +680,15 @@ static void orc_fft_rec(int n, const double* x, int64_t xs, double* y, int sign,
         return;
     }
     /* decimation in time: r sub-transforms of length m */
-    for (int q = 0; q < r; q++) orc_fft_rec(m, x + 2 * q * xs, xs * r, y + 2 * q * m, sign, tmp);
+    for (int q = 0; q < r; q++)
+        orc_fft_rec(m, x + 2 * q * xs, xs * r, y + 2 * q * m, tmp, tw, twst * r);
     for (int k = 0; k < m; k++) {
         for (int s = 0; s < r; s++) {
             int64_t kk = k + (int64_t)s * m;
             double sr = 0.0, si = 0.0;
             for (int q = 0; q < r; q++) {
                 int64_t e = ((int64_t)q * kk) % n;
-                double a = sign * 2.0 * M_PI * (double)e / (double)n;
-                double c = cos(a), sn = sin(a);
+                double c = tw[2 * e * twst], sn = tw[2 * e * twst + 1];
                 double yr = y[2 * (q * m + k)], yi = y[2 * (q * m + k) + 1];
                 sr += yr * c - yi * sn;
                 si += yr * sn + yi * c;
@@ -690,12 +705,14 @@ void orc_fft1d(double* data, int n, int sign) {
     double* y = (double*)malloc(sizeof(double) * 2 * n);
     double* in = (double*)malloc(sizeof(double) * 2 * n);
     double* tmp = (double*)malloc(sizeof(double) * 2 * n);
+    double* tw = orc_twiddles(n, sign);
     memcpy(in, data, sizeof(double) * 2 * n);
-    orc_fft_rec(n, in, 1, y, sign, tmp);
+    orc_fft_rec(n, in, 1, y, tmp, tw, 1);
     memcpy(data, y, sizeof(double) * 2 * n);
     free(y);
     free(in);
     free(tmp);
+    free(tw);
 }
 
 /* 3-D transform of a (n3, n2, n1) row-major complex array (n1 fastest). */
@@ -703,6 +720,9 @@ void orc_fft3d(double* a, int n1, int n2, int n3, int sign, int nthreads) {
     if (nthreads < 1) nthreads = 1;
     int nmax = n1 > n2 ? n1 : n2;
     if (n3 > nmax) nmax = n3;
+    double* tw1 = orc_twiddles(n1, sign);
+    double* tw2 = orc_twiddles(n2, sign);
+    double* tw3 = orc_twiddles(n3, sign);
 #pragma omp parallel num_threads(nthreads)
     {
         double* line = (double*)malloc(sizeof(double) * 2 * nmax);
@@ -712,7 +732,7 @@ void orc_fft3d(double* a, int n1, int n2, int n3, int sign, int nthreads) {
 #pragma omp for schedule(static)
         for (int64_t l = 0; l < (int64_t)n2 * n3; l++) {
             double* base = a + 2 * l * n1;
-            orc_fft_rec(n1, base, 1, y, sign, tmp);
+            orc_fft_rec(n1, base, 1, y, tmp, tw1, 1);
             memcpy(base, y, sizeof(double) * 2 * n1);
         }
         /* axis 2 (y) */
@@ -724,7 +744,7 @@ void orc_fft3d(double* a, int n1, int n2, int n3, int sign, int nthreads) {
                 line[2 * j] = base[2 * (int64_t)j * n1];
                 line[2 * j + 1] = base[2 * (int64_t)j * n1 + 1];
             }
-            orc_fft_rec(n2, line, 1, y, sign, tmp);
+            orc_fft_rec(n2, line, 1, y, tmp, tw2, 1);
             for (int j = 0; j < n2; j++) {
                 base[2 * (int64_t)j * n1] = y[2 * j];
                 base[2 * (int64_t)j * n1 + 1] = y[2 * j + 1];
@@ -739,7 +759,7 @@ void orc_fft3d(double* a, int n1, int n2, int n3, int sign, int nthreads) {
                 line[2 * j] = base[2 * j * st];
                 line[2 * j + 1] = base[2 * j * st + 1];
             }
-            orc_fft_rec(n3, line, 1, y, sign, tmp);
+            orc_fft_rec(n3, line, 1, y, tmp, tw3, 1);
             for (int j = 0; j < n3; j++) {
                 base[2 * j * st] = y[2 * j];
                 base[2 * j * st + 1] = y[2 * j + 1];
@@ -749,6 +769,9 @@ void orc_fft3d(double* a, int n1, int n2, int n3, int sign, int nthreads) {
         free(y);
         free(tmp);
     }
+    free(tw1);
+    free(tw2);
+    free(tw3);
 }
 
 /* ------------------------------------------------------------------------ */

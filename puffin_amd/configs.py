@@ -161,16 +161,32 @@ def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
     """Planar (ux = 1, uy = 0) elliptical FEL with natural focusing and a
     linear taper of -1% over 64 periods, rho = 0.005, temporal mesh
     129 x 129 x 1024, Gaussian seed (w0 = 2, amplitude 1e-3), Gaussian
-    current sigma = 200 nodes, Gaussian transverse sigma = 1.5, Gamma spread
-    5e-4.  The beam is centred at node 256 and truncated at +-256 nodes so
-    that the slippage of the run stays inside the temporal mesh."""
+    current sigma = 200 nodes, Gaussian transverse sigma = 1.5 (cut at
+    4 sigma), Gamma spread 5e-4.
+
+    Sized so that the 64-period module (1920 steps) stays on the temporal
+    mesh, as the reference requires (it STOPs when a macroparticle passes the
+    last node, system_interpolation.f90:70-75):
+      * 8 nodes per resonant wavelength, so 64 periods of resonant slippage
+        are 512 nodes;
+      * eta = 1e-6 (gamma_r = 866).  The slippage rate p2 grows with
+        aw^2 |p_perp|^2, and the natural-focusing betatron momentum of a
+        particle at amplitude a is about sqrt(eta) a / (2 sqrt(2) rho) in
+        scaled units.  At eta = 1e-4 a 4-sigma particle slips 5-6 times
+        faster than a resonant one; at 1e-6 the fastest slips 64.4 lambda_r
+        (515 nodes) over the module;
+      * the Gaussian current is cut at +-1.2 sigma (+-240 nodes) and centred
+        on node 244, so the beam spans nodes 4..484 and its front ends the
+        module at node ~1000 of 1024."""
     rng = np.random.Generator(np.random.Philox(SEED_BASE + 1))
     rho = 0.005
     ux, uy = 1.0, 0.0
     aw = 1.0
-    gamma_r = gamma_r_for_eta(1.0e-4, aw, ux, uy)
+    eta_target = 1.0e-6
+    gamma_r = gamma_r_for_eta(eta_target, aw, ux, uy)
     eta, kappa, fx, fy = scaling(rho, aw, gamma_r, ux, uy)
-    dz2 = 4.0 * PI_REF * rho / 16.0
+    nodes_per_lambda = 8.0
+    dz2 = 4.0 * PI_REF * rho / nodes_per_lambda
     lx = 19.2 * (nx - 1) / 128.0
     dx = lx / float(nx - 1)
     h = step_size(rho, 30)
@@ -179,11 +195,14 @@ def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
     mp = ModuleParams(nx=nx, ny=nx, nz2=nz2, dx=dx, dy=dx, dz2=dz2, rho=rho, eta=eta, kappa=kappa,
                       gamma_r=gamma_r, aw=aw, fx=fx, fy=fy, periodic=0, szfs=0.0, szfe=nsteps * h,
                       filt_cutoff=filter_cutoff(rho, nz2, dz2, False, 0.3))
-    zc = 256 * dz2 * (nz2 / 1024.0)
-    zs = 200 * dz2 * (nz2 / 1024.0)
-    z2 = zc + _truncated_normal(rng, n, zs, 1.28 * zs)
+    scale = nz2 / 1024.0
+    zc = 244 * dz2 * scale
+    zs = 200 * dz2 * scale
+    zcut = 1.2
+    z2 = zc + _truncated_normal(rng, n, zs, zcut * zs)
     sx = 1.5
-    p = dict(x=_truncated_normal(rng, n, sx, 4.0 * sx), y=_truncated_normal(rng, n, sx, 4.0 * sx),
+    xcut = 4.0
+    p = dict(x=_truncated_normal(rng, n, sx, xcut * sx), y=_truncated_normal(rng, n, sx, xcut * sx),
              z2=z2, px=np.zeros(n), py=np.zeros(n),
              gam=1.0 + 5.0e-4 * rng.standard_normal(n))
     # Gaussian current: chi = (2 pi sx^2 sqrt(2 pi) sz) / N (peak density 1)
@@ -192,39 +211,50 @@ def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
     ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0)
     run = nsteps if run_periods is None else 30 * run_periods
     return Case("c1_planar_taper_129x129x1024", mp, p, (ar, ai), h, run, steps_for_diffraction(rho, h),
-                n2col0=1.0, undgrad=undgrad, info=dict(gamma_r=gamma_r, periods=periods))
+                n2col0=1.0, undgrad=undgrad,
+                info=dict(gamma_r=gamma_r, periods=periods, eta=eta, nodes_per_lambda=nodes_per_lambda,
+                          sig_x=sx, x_cut_sigma=xcut, z2_centre=zc, z2_sigma=zs, z2_cut_sigma=zcut,
+                          gam_spread=5.0e-4))
 
 
 def case_c4(n: int = 1 << 27, nx: int = 257, nz2: int = 4096, periods: int = 128,
             run_periods: int | None = None) -> Case:
     """Large planar (ux = 1, uy = 0) FEL for the FP32-mixed form: temporal
     mesh 257 x 257 x 4096, 2^27 macroparticles, Gaussian transverse sigma = 2
-    (cut at 4 sigma), flat-top current over 2900 z2 nodes with Gaussian edges
-    (sigma 12 nodes), Gamma spread 1e-3, rho = 0.005, 8 nodes per resonant
-    wavelength, a per-period taper table of `periods` values
+    (cut at 4 sigma), flat-top current over 3000 z2 nodes with Gaussian edges
+    (sigma 12 nodes, cut at 3 sigma), Gamma spread 1e-3, rho = 0.005, a
+    per-period taper table of `periods` values
     n2col_k = 1 - 0.01 (k / (periods - 1))^2, Gaussian seed (w0 = 2,
-    amplitude 1e-3).  128 periods slip the beam by 1024 nodes: the flat top
-    spans nodes 40..2940 (edges to 3 sigma beyond), so the whole beam stays
-    on the mesh for the whole module."""
+    amplitude 1e-3).
+
+    Sized so that the 128-period module (3840 steps) stays on the temporal
+    mesh: 7 nodes per resonant wavelength and eta = 1e-6 (see case_c1), so
+    that the fastest 4-sigma particle slips 131.5 lambda_r = 921 nodes; the
+    flat top spans nodes 38..3038 and the front edge ends the module at node
+    ~3995 of 4096."""
     rng = np.random.Generator(np.random.Philox(SEED_BASE + 4))
     rho = 0.005
     ux, uy = 1.0, 0.0
     aw = 1.0
-    gamma_r = gamma_r_for_eta(1.0e-4, aw, ux, uy)
+    gamma_r = gamma_r_for_eta(1.0e-6, aw, ux, uy)
     eta, kappa, fx, fy = scaling(rho, aw, gamma_r, ux, uy)
-    dz2 = 4.0 * PI_REF * rho / 8.0
+    nodes_per_lambda = 7.0
+    dz2 = 4.0 * PI_REF * rho / nodes_per_lambda
     dx = 0.1
     h = step_size(rho, 30)
     nsteps = 30 * periods
     mp = ModuleParams(nx=nx, ny=nx, nz2=nz2, dx=dx, dy=dx, dz2=dz2, rho=rho, eta=eta, kappa=kappa,
                       gamma_r=gamma_r, aw=aw, fx=fx, fy=fy, periodic=0, szfs=0.0, szfe=nsteps * h,
                       filt_cutoff=filter_cutoff(rho, nz2, dz2, False, 0.3))
-    flat = 2900 * dz2 * (nz2 / 4096.0)
+    scale = nz2 / 4096.0
+    flat = 3000 * dz2 * scale
     edge = 12 * dz2
-    z0 = 40 * dz2
-    z2 = z0 + flat * rng.random(n) + _truncated_normal(rng, n, edge, 3.0 * edge)
+    ecut = 3.0
+    z0 = 38 * dz2 * scale
+    z2 = z0 + flat * rng.random(n) + _truncated_normal(rng, n, edge, ecut * edge)
     sx = 2.0
-    p = dict(x=_truncated_normal(rng, n, sx, 4.0 * sx), y=_truncated_normal(rng, n, sx, 4.0 * sx),
+    xcut = 4.0
+    p = dict(x=_truncated_normal(rng, n, sx, xcut * sx), y=_truncated_normal(rng, n, sx, xcut * sx),
              z2=z2, px=np.zeros(n), py=np.zeros(n),
              gam=1.0 + 1.0e-3 * rng.standard_normal(n))
     p["chi"] = np.full(n, 2 * math.pi * sx * sx * flat / n)
@@ -234,7 +264,11 @@ def case_c4(n: int = 1 << 27, nx: int = 257, nz2: int = 4096, periods: int = 128
     ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0)
     run = nsteps if run_periods is None else 30 * run_periods
     c = Case("c4_planar_large_257x257x4096", mp, p, (ar, ai), h, run, steps_for_diffraction(rho, h),
-             n2col0=float(table[0]), info=dict(gamma_r=gamma_r, periods=periods, seed_amp=1.0e-3))
+             n2col0=float(table[0]),
+             info=dict(gamma_r=gamma_r, periods=periods, seed_amp=1.0e-3, eta=eta,
+                       nodes_per_lambda=nodes_per_lambda, sig_x=sx, x_cut_sigma=xcut,
+                       z2_flat=(z0, z0 + flat), z2_edge_sigma=edge, z2_edge_cut_sigma=ecut,
+                       gam_spread=1.0e-3))
     c.taper_table = table
     c.taper_period = 4.0 * PI_REF * rho
     return c

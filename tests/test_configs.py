@@ -1,0 +1,108 @@
+"""The synthetic workloads fit their meshes for their whole module.
+
+The reference STOPs when a macroparticle passes the last z2 node of a temporal
+mesh (system_interpolation.f90:70-75) and raises qInnerXYOK_G when one leaves
+the transverse mesh (:60-68), so each workload must keep every macroparticle
+on the mesh for all of its steps.  Slippage is not bounded by the initial p2:
+p2 oscillates with the wiggle and grows with the natural-focusing betatron
+momentum.  So each test integrates the uncoupled orbits of the workload's
+extreme macroparticles (largest transverse amplitudes, the beam front, the
+lowest energy, which slips fastest) with the CPU oracle over the module's
+steps, on a small periodic mesh that raises no flag, and checks the whole
+orbit against the workload's own mesh with a 1% margin on the slippage for
+the energy the FEL interaction takes out of the beam.
+"""
+from __future__ import annotations
+
+import dataclasses
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from puffin_amd.configs import case_c0, case_c1, case_c3, case_c4, match_in
+
+
+def extreme_particles(case, n2col0):
+    """Corners of the beam's cut box: x, y at -cut, 0, +cut; z2 at the back
+    and front cuts; Gamma at 1 and 1 -/+ 5 sigma; matched as the case is."""
+    i = case.info
+    mp = case.mp
+    xc = i["x_cut_sigma"] * i["sig_x"]
+    if "z2_flat" in i:
+        lo = i["z2_flat"][0] - i["z2_edge_cut_sigma"] * i["z2_edge_sigma"]
+        hi = i["z2_flat"][1] + i["z2_edge_cut_sigma"] * i["z2_edge_sigma"]
+    else:
+        lo = i["z2_centre"] - i["z2_cut_sigma"] * i["z2_sigma"]
+        hi = i["z2_centre"] + i["z2_cut_sigma"] * i["z2_sigma"]
+    gs = i["gam_spread"]
+    rows = list(itertools.product((-xc, 0.0, xc), (-xc, 0.0, xc), (lo, hi),
+                                  (1.0 - 5 * gs, 1.0, 1.0 + 5 * gs)))
+    a = np.array(rows, dtype=np.float64)
+    n = a.shape[0]
+    p = dict(x=a[:, 0].copy(), y=a[:, 1].copy(), z2=a[:, 2].copy(), px=np.zeros(n), py=np.zeros(n),
+             gam=a[:, 3].copy(), chi=np.full(n, 1e-9))
+    match_in(mp, n2col0, p)
+    return p, lo, hi
+
+
+def uncoupled_orbits(case, parts):
+    """Integrate the module's steps with coupling and field evolution off, on a
+    3 x 3 x 3 periodic mesh wide enough for any orbit; returns the final
+    particles and the largest |x|, |y| seen."""
+    mp = dataclasses.replace(case.mp, nx=3, ny=3, nz2=3, dx=1.0e3, dy=1.0e3, periodic=1,
+                             coupling=0, field_evolve=0)
+    zero = (np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+    o = orc.Oracle(mp, parts, zero, taper=(case.n2col0, case.n2col0, case.undgrad))
+    if case.taper_table is not None:
+        orc.set_taper_table(case.taper_table, 0.0, case.taper_period)
+    try:
+        xmax = np.abs(o.parts["x"]).max()
+        ymax = np.abs(o.parts["y"]).max()
+        szl = 0.0
+        for _ in range(case.nsteps):
+            assert o.rk4par(szl, case.h) == 0
+            szl += case.h
+            xmax = max(xmax, np.abs(o.parts["x"]).max())
+            ymax = max(ymax, np.abs(o.parts["y"]).max())
+    finally:
+        orc.set_taper_table(None)
+    return o.parts, xmax, ymax
+
+
+@pytest.mark.parametrize("make,nsteps", [(case_c1, 1920), (case_c4, 3840)])
+def test_temporal_workload_stays_on_mesh(make, nsteps):
+    case = make(n=4096)
+    assert case.nsteps == nsteps
+    mp = case.mp
+    parts, lo, hi = extreme_particles(case, case.n2col0)
+    # the generated beam lies inside the cut box
+    assert case.particles["z2"].min() >= lo and case.particles["z2"].max() <= hi
+    assert lo >= 0.0
+    out, xmax, ymax = uncoupled_orbits(case, parts)
+    slip = out["z2"] - parts["z2"]
+    assert slip.min() > 0.0
+    front = (out["z2"] + 0.01 * slip).max()
+    # the last cell a macroparticle may occupy starts at node nz2 - 2
+    assert front < (mp.nz2 - 2) * mp.dz2, (front / mp.dz2, mp.nz2)
+    half = (mp.nx - 1) / 2.0 * mp.dx
+    assert xmax < half - mp.dx and ymax < half - mp.dy, (xmax, ymax, half)
+
+
+@pytest.mark.parametrize("make", [case_c0, case_c3])
+def test_periodic_workload_stays_inside_transverse_mesh(make):
+    case = make(n=4096)
+    mp = case.mp
+    sig = case.info["sig_x"]
+    xc = 4.5 * sig
+    assert np.abs(case.particles["x"] - case.particles["x"].mean()).max() <= xc + 1.0
+    rows = list(itertools.product((-xc, 0.0, xc), (-xc, 0.0, xc)))
+    a = np.array(rows)
+    n = a.shape[0]
+    p = dict(x=a[:, 0].copy(), y=a[:, 1].copy(), z2=np.full(n, 0.1), px=np.zeros(n), py=np.zeros(n),
+             gam=np.full(n, 1.0 - 5e-3), chi=np.full(n, 1e-9))
+    match_in(mp, case.n2col0, p)
+    out, xmax, ymax = uncoupled_orbits(case, p)
+    half = (mp.nx - 1) / 2.0 * mp.dx
+    assert xmax < half - mp.dx and ymax < half - mp.dy, (xmax, ymax, half)

@@ -1,0 +1,251 @@
+"""The HIP integrator on the five workloads at their full mesh sizes.
+
+* C0: the whole module (2^18 macroparticles, periodic 33 x 33 x (128 + 1),
+  240 steps on UndSection's schedule with its diffraction steps) against the
+  oracle, FP64, at 1e-10 relative: the bar the reference holds its own run
+  to (test/testMPIIntegration.pf:102-107, 126-127).
+* C1: the full 129 x 129 x 1024 mesh with a 2^16-macroparticle beam of the
+  same distribution for 30 steps of the schedule (two diffraction half
+  steps of the whole mesh) against the oracle at 1e-10; and the full 2^24
+  beam through size-independent properties: charge conservation of the
+  deposit, a flag-free module start, finite growing power.
+* C2: the FP32-mixed form on C1's mesh against the oracle's FP64 run, error
+  stated per output (power, bunching, on-axis spectrum, particles).
+* C3: the gradient of the full 2^20-macroparticle, 480-step module against
+  central differences of the forward.
+* C4: the full 2^27-macroparticle beam on the 257 x 257 x 4096 mesh (mixed):
+  charge conservation, a flag-free module start, finite power.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from puffin_amd import gpu
+from puffin_amd.configs import case_c0, case_c1, case_c3, case_c4
+from puffin_amd.params import FP64, MIXED
+from puffin_amd.undulator import und_section
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("x", "y", "z2", "px", "py", "gam")
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
+
+def say(msg):
+    print(f"[gpu-configs] {msg}", flush=True)
+
+
+def relerr(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(b).max(), 1e-300))
+
+
+def run_module_gpu(case, precision, nsteps):
+    with gpu.Integrator(case.mp, case.n, precision) as g:
+        g.upload_particles(case.particles)
+        g.upload_field(*case.field)
+        g.set_taper(case.n2col0, case.n2col0, case.undgrad)
+        if case.taper_table is not None:
+            g.set_taper_table(case.taper_table, 0.0, case.taper_period)
+        res = und_section(g, nsteps, case.h, diffraction=True, steps4diff=case.steps4diff,
+                          periodic=bool(case.mp.periodic))
+        assert res.status == 0, (res.status, res.steps_done)
+        out = g.download_particles()
+        ar, ai = g.download_field()
+        power = g.power()
+        lam = 4.0 * math.pi * case.mp.rho
+        nsl = int(math.ceil((case.mp.nz2 - 1) * case.mp.dz2 / lam))
+        bunch = g.bunching(1, nsl, lam)
+    return out, ar, ai, power, bunch, nsl, lam
+
+
+def run_module_oracle(case, nsteps):
+    o = orc.Oracle(case.mp, case.particles, case.field,
+                   taper=(case.n2col0, case.n2col0, case.undgrad), nthreads=THREADS)
+    st, _, _, _ = o.und_section(nsteps, case.h, True, case.steps4diff,
+                                60 if case.mp.periodic else 0)
+    assert st == 0
+    return o
+
+
+def on_axis_spectrum(mp, ar, ai):
+    ntr = mp.nx * mp.ny
+    c = (mp.ny // 2) * mp.nx + mp.nx // 2
+    axis = ar[c::ntr][: mp.nzp] + 1j * ai[c::ntr][: mp.nzp]
+    return (np.abs(np.fft.fft(axis)) ** 2)[: mp.nzp // 2 + 1]
+
+
+# ---------------------------------------------------------------------------
+# C0: the whole module against the oracle
+# ---------------------------------------------------------------------------
+
+def test_c0_whole_module_matches_oracle():
+    case = case_c0()
+    assert case.n == 1 << 18 and case.nsteps == 240
+    assert (case.mp.nx, case.mp.ny, case.mp.nz2, case.mp.periodic) == (33, 33, 129, 1)
+    say(f"C0 GPU {case.n} macroparticles, {case.nsteps} steps")
+    out, ar, ai, power, bunch, nsl, lam = run_module_gpu(case, FP64, case.nsteps)
+    say(f"C0 oracle, {THREADS} threads")
+    o = run_module_oracle(case, case.nsteps)
+    for k in KEYS:
+        e = relerr(out[k], o.parts[k])
+        say(f"C0 {k} rel err {e:.2e}")
+        assert e < 1e-10, k
+    assert relerr(ar, o.ar) < 1e-10 and relerr(ai, o.ai) < 1e-10
+    assert relerr(power, o.power()) < 1e-10
+    assert relerr(bunch, o.bunching(1, nsl, lam)) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# C1 / C2: the full mesh with a sampled beam against the oracle
+# ---------------------------------------------------------------------------
+
+C1_SAMPLE = 1 << 16
+C1_STEPS = 30
+
+
+@pytest.fixture(scope="module")
+def c1_sample_oracle():
+    case = case_c1(n=C1_SAMPLE)
+    say(f"C1 oracle: {case.n} macroparticles on the full mesh, {C1_STEPS} steps, {THREADS} threads")
+    o = run_module_oracle(case, C1_STEPS)
+    return case, o
+
+
+def test_c1_full_mesh_matches_oracle(c1_sample_oracle):
+    case, o = c1_sample_oracle
+    assert (case.mp.nx, case.mp.ny, case.mp.nz2) == (129, 129, 1024)
+    out, ar, ai, power, bunch, nsl, lam = run_module_gpu(case, FP64, C1_STEPS)
+    for k in KEYS:
+        e = relerr(out[k], o.parts[k])
+        say(f"C1 {k} rel err {e:.2e}")
+        assert e < 1e-10, k
+    assert relerr(ar, o.ar) < 1e-10 and relerr(ai, o.ai) < 1e-10
+    assert relerr(power, o.power()) < 1e-10
+
+
+def test_c2_mixed_outputs_against_oracle_fp64(c1_sample_oracle):
+    """Per-output error of the FP32-mixed form against the oracle's FP64 run
+    (30 steps with two diffraction half steps of the full mesh).  Stated
+    tolerances, relative to the largest FP64 value: power(z2), bunching and
+    on-axis spectrum 1e-4 (the north-star bar for integrated power), the
+    integrated power 1e-5, z2 and Gamma (FP64 state, driven by FP32 p_perp and
+    field) 1e-8, x, y, p_perp (FP32 state) 1e-5."""
+    case, o = c1_sample_oracle
+    out, ar, ai, power, bunch, nsl, lam = run_module_gpu(case, MIXED, C1_STEPS)
+    p64 = o.power()
+    e_pow = relerr(power, p64)
+    e_int = abs(np.trapezoid(power) - np.trapezoid(p64)) / abs(np.trapezoid(p64))
+    e_b = relerr(bunch, o.bunching(1, nsl, lam))
+    e_s = relerr(on_axis_spectrum(case.mp, ar, ai), on_axis_spectrum(case.mp, o.ar, o.ai))
+    say(f"C2 power(z) {e_pow:.2e} integrated {e_int:.2e} bunching {e_b:.2e} spectrum {e_s:.2e}")
+    assert e_pow < 1e-4 and e_b < 1e-4 and e_s < 1e-4 and e_int < 1e-5
+    for k in KEYS:
+        e = relerr(out[k], o.parts[k])
+        say(f"C2 {k} rel err {e:.2e}")
+        assert e < (1e-8 if k in ("z2", "gam") else 1e-5), k
+
+
+# ---------------------------------------------------------------------------
+# full beams: size-independent properties
+# ---------------------------------------------------------------------------
+
+def p2_host(mp, gam, pr, pi):
+    """getP2 (gamma_to_p2.f90:84-85) in FP64 on the host."""
+    d = (1.0 + mp.aw ** 2 * (pr * pr + pi * pi)) / (mp.gamma_r ** 2 * gam * gam)
+    return (1.0 / np.sqrt(1.0 - d) - 1.0) / mp.eta
+
+
+def check_charge_conservation(case, precision, tol):
+    """With frozen macroparticles every RK4 stage deposits the same source, so
+    one step adds h * deposit(y0) to the field; its node sum times dV3 is
+    sum_p chi (1 + eta p2) p_perp / Gamma (the weights of each macroparticle
+    sum to one).  p_perp gets an offset so that both components carry
+    current (the matched planar beam starts with p_perp = 0)."""
+    import dataclasses
+    mp = dataclasses.replace(case.mp, electrons_evolve=0)
+    p = dict(case.particles)
+    p["px"] = p["px"] + 0.3
+    p["py"] = p["py"] + 0.7
+    with gpu.Integrator(mp, case.n, precision) as g:
+        g.upload_particles(p)
+        g.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+        g.set_taper(case.n2col0, case.n2col0, case.undgrad)
+        assert g.rk4par(0.0, case.h) == 0
+        ar, ai = g.download_field()
+    fac = p["chi"] * (1.0 + mp.eta * p2_host(mp, p["gam"], p["px"], p["py"])) / p["gam"]
+    want_r = float(np.sum(fac * p["px"]))
+    want_i = float(np.sum(fac * p["py"]))
+    dv = mp.dx * mp.dy * mp.dz2
+    got_r = float(np.sum(ar)) * dv / case.h
+    got_i = float(np.sum(ai)) * dv / case.h
+    scale = float(np.sum(np.abs(fac) * np.hypot(p["px"], p["py"])))
+    say(f"{case.name} precision={precision} charge: re {got_r:.6e} vs {want_r:.6e}, "
+        f"im {got_i:.6e} vs {want_i:.6e}")
+    assert abs(got_r - want_r) <= tol * scale
+    assert abs(got_i - want_i) <= tol * scale
+
+
+def check_module_start(case, precision, nsteps):
+    out, ar, ai, power, bunch, nsl, lam = run_module_gpu(case, precision, nsteps)
+    assert np.isfinite(power).all() and np.isfinite(bunch).all()
+    for k in KEYS:
+        assert np.isfinite(out[k]).all(), k
+    p0 = np.sum(case.field[0] ** 2 + case.field[1] ** 2)
+    p1 = np.sum(ar * ar + ai * ai)
+    say(f"{case.name} precision={precision} after {nsteps} steps: |A|^2 {p0:.4e} -> {p1:.4e}")
+    assert p1 > p0
+    # the beam only slips forward
+    assert (out["z2"] >= case.particles["z2"]).mean() > 0.999
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_c1_full_beam_properties(precision):
+    case = case_c1()
+    assert case.n == 1 << 24
+    check_charge_conservation(case, precision, 1e-12 if precision == FP64 else 1e-5)
+    check_module_start(case, precision, 60)
+
+
+def test_c4_full_beam_properties():
+    case = case_c4()
+    assert case.n == 1 << 27 and (case.mp.nx, case.mp.nz2) == (257, 4096)
+    check_charge_conservation(case, MIXED, 1e-5)
+    check_module_start(case, MIXED, 30)
+
+
+# ---------------------------------------------------------------------------
+# C3: the full-size gradient against central differences
+# ---------------------------------------------------------------------------
+
+def test_c3_full_gradient_against_central_differences():
+    from puffin_amd.adjoint import ModuleGradient, module_program
+    c = case_c3()
+    assert c.n == 1 << 20 and c.nsteps == 480 and c.mp.nz2 == 257
+    mp = c.mp
+    ops = module_program(c.nsteps, c.h, c.steps4diff, diffraction=True, periodic=True, redist_every=60)
+    with gpu.Integrator(mp, c.n, FP64) as drv:
+        drv.upload_particles(c.particles)
+        drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+        shape = (c.field[0] / c.info["seed_amp"], c.field[1] / c.info["seed_amp"])
+        eng = ModuleGradient(drv, ops, shape, every=30, taper_period=c.taper_period)
+        table, seed = c.taper_table, c.info["seed_amp"]
+        J = eng.forward(table, seed)
+        g = eng.backward()
+        eps = 1e-6
+        fd_s = (eng.evaluate(table, seed * (1 + eps)) - eng.evaluate(table, seed * (1 - eps))) / (2 * eps * seed)
+        k = int(np.argmax(np.abs(g["table"])))
+        tp, tm = table.copy(), table.copy()
+        tp[k] += eps
+        tm[k] -= eps
+        fd_t = (eng.evaluate(tp, seed) - eng.evaluate(tm, seed)) / (2 * eps)
+        del eng
+    es = abs(fd_s - g["seed"]) / abs(g["seed"])
+    et = abs(fd_t - g["table"][k]) / np.abs(g["table"]).max()
+    say(f"C3 J={J:.6e} dJ/dseed {g['seed']:.6e} (fd {fd_s:.6e}, {es:.2e}); "
+        f"dJ/dtable[{k}] {g['table'][k]:.6e} (fd {fd_t:.6e}, {et:.2e})")
+    assert es < 1e-3 and et < 1e-3
