@@ -236,6 +236,29 @@ int puff_bunching(puff_ctx* ctx, int32_t harmonic, int32_t nslices,
    zero. */
 int puff_slice_stats(puff_ctx* ctx, int32_t nslices, double slice_len, double* out_host);
 
+/*
+ * Allocation-free slice statistics.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The same outputs as puff_slice_stats, in
+ * the same layout, for n macroparticles held in the caller's DEVICE arrays
+ * (FP64, the reference layout; PY holds -p_y), written to the caller's
+ * device array out_dev of 13 * nslices doubles, on the caller's HIP stream
+ * (hipStream_t cast to void*; NULL = the null stream) of the current device.
+ * No context, no device or host allocation, no wait: it returns once the
+ * kernels are queued.  The per-slice sums live in a library-owned static
+ * device buffer of PUFF_NOALLOC_MAX_SLICES * 13 doubles on each GPU, which
+ * the call clears first; nslices above PUFF_NOALLOC_MAX_SLICES is refused
+ * with PUFF_ERR_ARG.  Calls that overlap on one GPU (other streams or host
+ * threads) share that buffer and race; calls on one stream are ordered.
+ * Only params->periodic is read (the slice wrap).
+ */
+#define PUFF_NOALLOC_MAX_SLICES 4096
+int puff_slice_stats_noalloc(const puff_params* params, int64_t n, const double* x,
+                             const double* y, const double* z2, const double* px,
+                             const double* py, const double* gam, const double* chi,
+                             int32_t nslices, double slice_len, double* out_dev,
+                             void* hip_stream);
+
 /* Beam current nodes of getCurr (av_write.f90:297-368): chi-bar binned
    linearly onto npts nodes spaced sam_len apart (node j at (j - 1) sam_len),
    before the reference's unit factors npk_bar (x ata in 1-D) x q_e /

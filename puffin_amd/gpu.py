@@ -55,6 +55,8 @@ SIGNATURES = {
     "puff_tile_fallback": (ctypes.c_int, [_vp, _dp, _dp]),
     "puff_slice_stats": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, _dp]),
     "puff_current": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, _dp]),
+    "puff_slice_stats_noalloc": (ctypes.c_int, [ctypes.POINTER(PuffParams), ctypes.c_int64] + [_vp] * 7 +
+                                 [ctypes.c_int32, ctypes.c_double, _vp, _vp]),
     "puff_particle_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "puff_profile_enable": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "puff_profile_read": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32,
