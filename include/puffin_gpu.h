@@ -156,6 +156,21 @@ int puff_set_taper_table(puff_ctx* ctx, const double* values, int32_t nvals,
 int puff_set_sort_interval(puff_ctx* ctx, int32_t steps);
 
 /*
+ * Deterministic deposit (FP64 form only; no reference counterpart).  With
+ * on = 1 every contribution to the current deposit is summed as fixed-point
+ * integers (a 2 x 64-bit split resolving about 2^-68 of the step's largest
+ * source), so puff_rk4par's results do not depend on the order in which
+ * atomics land or on the macroparticles' slot order: the same inputs give
+ * the same state bit for bit, run after run.  The reference's own deposit
+ * (!$OMP ATOMIC, system_interpolation.f90:174-241) has no such mode.  Costs
+ * a pass over the macroparticles per step, a conversion pass over the
+ * active nodes per stage and 32 bytes of device memory per mesh node.  The
+ * adjoint steps keep their atomic deposits.  PUFF_ERR_ARG in the mixed form
+ * or above 2^30 macroparticles.
+ */
+int puff_set_deterministic(puff_ctx* ctx, int32_t on);
+
+/*
  * One RK4 step of particles and field together.
  * Replaces rk4par(sZ, h, qD) (puffin_mpi_RK4.f90:38-366) with its four
  * derivs / getrhs evaluations (derivative.f90:36-155, rhs.f90:60-330).

@@ -127,13 +127,19 @@ class ModuleGradient:
         self.ops = ops
         self.every = int(every)
         self.segs = segments(ops, self.every)
-        self.shape_r = np.ascontiguousarray(field_shape[0], dtype=np.float64).reshape(-1)
-        self.shape_i = np.ascontiguousarray(field_shape[1], dtype=np.float64).reshape(-1)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        # the seed field's unit shape stays on the device: each evaluation
+        # scales it there and uploads it device to device
+        self.shape_r = torch.as_tensor(np.ascontiguousarray(field_shape[0], dtype=np.float64).reshape(-1),
+                                       device=dev)
+        self.shape_i = torch.as_tensor(np.ascontiguousarray(field_shape[1], dtype=np.float64).reshape(-1),
+                                       device=dev)
+        self._seed_r = torch.empty_like(self.shape_r)
+        self._seed_i = torch.empty_like(self.shape_i)
         self.taper_period = taper_period if taper_period else 4.0 * PI_REF * self.mp.rho
         self.taper_z0 = taper_z0
         self.wz = power_weights(self.mp)
         nb = drv.state_bytes()
-        dev = torch.device("cuda", torch.cuda.current_device())
         self._blob = lambda: torch.empty(nb, dtype=torch.uint8, device=dev)
         # initial particles (taken from the driver as it is now)
         hdr = lambda: np.zeros(STATE_HEADER_BYTES, dtype=np.uint8)
@@ -179,7 +185,9 @@ class ModuleGradient:
         self.table = np.ascontiguousarray(table, dtype=np.float64)
         self.seed = float(seed)
         self.drv.state_load(self.initial.data_ptr(), self.initial_hdr)
-        self.drv.upload_field(self.seed * self.shape_r, self.seed * self.shape_i)
+        torch.mul(self.shape_r, self.seed, out=self._seed_r)
+        torch.mul(self.shape_i, self.seed, out=self._seed_i)
+        self.drv.upload_field_torch(self._seed_r, self._seed_i)
         self.drv.set_taper_table(self.table, self.taper_z0, self.taper_period)
 
     def _run(self, op: Op):
@@ -286,8 +294,8 @@ class ModuleGradient:
             g_table = drv.adjoint_taper_gradient(self.table.size + 1)[: self.table.size]
         finally:
             self._deferred(False)
-        _, lar, lai = drv.adjoint_download()
-        g_seed = float(np.dot(lar, self.shape_r) + np.dot(lai, self.shape_i))
+        lar, lai = drv.adjoint_download_field_torch()
+        g_seed = float(torch.dot(lar, self.shape_r) + torch.dot(lai, self.shape_i))
         return {"table": g_table, "seed": g_seed, "field": (lar, lai)}
 
 
@@ -296,17 +304,21 @@ if torch is not None:
     class IntegratedPower(torch.autograd.Function):
         """J(table, seed) of one module; gradients by the discrete adjoint.
 
-        Inputs: ``table`` (CPU float64 tensor of per-period n2col values),
-        ``seed`` (0-d float64 tensor, seed amplitude) and the ModuleGradient
-        ``engine``.  Output: 0-d float64 tensor J."""
+        Inputs: ``table`` (float64 tensor of per-period n2col values, on the
+        CPU or the device), ``seed`` (0-d float64 tensor, seed amplitude) and
+        the ModuleGradient ``engine``.  Output: 0-d float64 tensor J on
+        ``seed``'s device; the gradients come back on the inputs' devices."""
 
         @staticmethod
         def forward(ctx, table, seed, engine):
             J = engine.forward(table.detach().cpu().numpy(), float(seed))
             ctx.engine = engine
-            return torch.tensor(J, dtype=torch.float64)
+            ctx.devices = (table.device, seed.device)
+            return torch.tensor(J, dtype=torch.float64, device=seed.device)
 
         @staticmethod
         def backward(ctx, gJ):
             g = ctx.engine.backward(float(gJ))
-            return (torch.from_numpy(g["table"]), torch.tensor(g["seed"], dtype=torch.float64), None)
+            dt, ds = ctx.devices
+            return (torch.from_numpy(g["table"]).to(dt),
+                    torch.tensor(g["seed"], dtype=torch.float64, device=ds), None)

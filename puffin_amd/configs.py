@@ -157,7 +157,7 @@ def case_c3(n: int = 1 << 20, periods: int = 16, seed: float = 1.0e-3, aw_vec=No
 
 
 def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
-            run_periods: int | None = None) -> Case:
+            run_periods: int | None = None, with_field: bool = True) -> Case:
     """Planar (ux = 1, uy = 0) elliptical FEL with natural focusing and a
     linear taper of -1% over 64 periods, rho = 0.005, temporal mesh
     129 x 129 x 1024, Gaussian seed (w0 = 2, amplitude 1e-3), Gaussian
@@ -208,7 +208,7 @@ def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
     # Gaussian current: chi = (2 pi sx^2 sqrt(2 pi) sz) / N (peak density 1)
     p["chi"] = np.full(n, 2 * math.pi * sx * sx * math.sqrt(2 * math.pi) * zs / n)
     match_in(mp, 1.0, p)
-    ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0)
+    ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0) if with_field else (None, None)
     run = nsteps if run_periods is None else 30 * run_periods
     return Case("c1_planar_taper_129x129x1024", mp, p, (ar, ai), h, run, steps_for_diffraction(rho, h),
                 n2col0=1.0, undgrad=undgrad,
@@ -218,7 +218,7 @@ def case_c1(n: int = 1 << 24, nx: int = 129, nz2: int = 1024, periods: int = 64,
 
 
 def case_c4(n: int = 1 << 27, nx: int = 257, nz2: int = 4096, periods: int = 128,
-            run_periods: int | None = None) -> Case:
+            run_periods: int | None = None, with_field: bool = True) -> Case:
     """Large planar (ux = 1, uy = 0) FEL for the FP32-mixed form: temporal
     mesh 257 x 257 x 4096, 2^27 macroparticles, Gaussian transverse sigma = 2
     (cut at 4 sigma), flat-top current over 3000 z2 nodes with Gaussian edges
@@ -261,7 +261,7 @@ def case_c4(n: int = 1 << 27, nx: int = 257, nz2: int = 4096, periods: int = 128
     k = np.arange(periods, dtype=np.float64)
     table = 1.0 - 0.01 * (k / max(periods - 1, 1)) ** 2
     match_in(mp, float(table[0]), p)
-    ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0)
+    ar, ai = gaussian_seed(mp, 0.0, 1.0e-3, w0=2.0) if with_field else (None, None)
     run = nsteps if run_periods is None else 30 * run_periods
     c = Case("c4_planar_large_257x257x4096", mp, p, (ar, ai), h, run, steps_for_diffraction(rho, h),
              n2col0=float(table[0]),

@@ -45,6 +45,10 @@ __device__ __forceinline__ cplx<R> csub(cplx<R> a, cplx<R> b) {
 constexpr int FLAG_TRANSVERSE = 1;
 constexpr int FLAG_BUFFER = 2;
 constexpr int FLAG_Z2MESH = 4;
+// A source beyond the range the deterministic deposit's fixed point was
+// scaled for at the start of the step (never expected: the scale leaves a
+// factor 4 of headroom over the sources of the step's start state).
+constexpr int FLAG_DET_RANGE = 8;
 
 // Active box of one RK4 step: min/max cell index along x, y, z2.  The stage
 // kernels update BOX_COPIES copies (by workgroup) to spread the atomics.

@@ -125,6 +125,9 @@ struct Ctx {
     double* dn2_table = nullptr;   // per-slot dJ/dn2col (deferred adjoint steps)
     // deferred status: steps return without waiting for their flags
     bool deferred = false;
+    // deterministic deposit of the FP64 form (puff_set_deterministic)
+    bool det = false;
+    DetState dets{};
 
     int64_t nmesh() const { return (int64_t)p.nx * p.ny * p.nz2; }
     int nzp() const { return p.periodic ? p.nz2 - 1 : p.nz2; }
@@ -775,8 +778,16 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
     if (cap) capture_start<R>(c, cap, cap_hdr, v, saved);
     int slot = prof_start(c, 6);
     launch_step_begin(c->flags, c->box, s);
+    DetState det{};
+    if constexpr (sizeof(R) == 8) {
+        if (c->det) {
+            launch_det_scale(P.y, P.chi, c->n, c->phys, c->dets, s);
+            det = c->dets;
+        }
+    }
     prof_stop(c, slot);
     StageArgs<R> a{};
+    a.det = det;
     a.n = c->n;
     a.y0 = P.y;
     a.chi = P.chi;
@@ -812,6 +823,11 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
         a.dAout = D[st - 1];
         slot = prof_start(c, st - 1);
         launch_stage<R>(st, a, s);
+        if constexpr (sizeof(R) == 8) {
+            if (det.acc)
+                launch_det_convert(det, D[st - 1], c->box, c->p.nx, c->p.ny, c->nzp(), c->p.periodic,
+                                   c->p.one_d, s);
+        }
         prof_stop(c, slot);
     }
     CombineArgs<R> ca{};
@@ -851,6 +867,10 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
     if (f) {
         c->n2col = saved[0];
         c->n2col0 = saved[1];
+        if (f & FLAG_DET_RANGE) {
+            set_error("a source exceeded the range the deterministic deposit was scaled for");
+            return PUFF_ERR_STATE;
+        }
         if (f & FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
         if (f & FLAG_TRANSVERSE) return PUFF_OUT_TRANSVERSE;
         return PUFF_OUT_Z2_BUFFER;
@@ -1812,6 +1832,27 @@ int puff_set_stream(puff_ctx* ctx, void* stream) {
     return PUFF_OK;
 }
 
+int puff_set_deterministic(puff_ctx* ctx, int32_t on) {
+    CTX_CHECK(ctx);
+    puff::Ctx& c = ctx->c;
+    if (on && c.precision != PUFF_FP64) {
+        puff::set_error("the deterministic deposit is a mode of the FP64 form");
+        return PUFF_ERR_ARG;
+    }
+    if (on && c.n > (int64_t(1) << 30)) {
+        puff::set_error("the deterministic deposit holds at most 2^30 macroparticles");
+        return PUFF_ERR_ARG;
+    }
+    if (on && !c.dets.acc) {
+        PUFF_TRY(puff::dalloc(&c, (void**)&c.dets.acc,
+                              sizeof(unsigned long long) * puff::DET_WORDS * (size_t)c.nmesh()));
+        PUFF_TRY(puff::dalloc(&c, (void**)&c.dets.shift, sizeof(int)));
+        PUFF_TRY(puff::dalloc(&c, (void**)&c.dets.bound, sizeof(unsigned long long)));
+    }
+    c.det = on != 0;
+    return PUFF_OK;
+}
+
 int puff_synchronize(puff_ctx* ctx) {
     CTX_CHECK(ctx);
     return hipStreamSynchronize(ctx->c.stream) == hipSuccess ? PUFF_OK : PUFF_ERR_HIP;
@@ -2088,6 +2129,10 @@ int puff_deferred_status(puff_ctx* ctx) {
     PUFF_HIP(hipStreamSynchronize(c.stream));
     puff::prof_collect(&c);
     const int f = c.h_flags[puff::FLAG_WORD_STICKY];
+    if (f & puff::FLAG_DET_RANGE) {
+        puff::set_error("a source exceeded the range the deterministic deposit was scaled for");
+        return PUFF_ERR_STATE;
+    }
     if (f & puff::FLAG_Z2MESH) return PUFF_OUT_Z2_MESH;
     if (f & puff::FLAG_TRANSVERSE) return PUFF_OUT_TRANSVERSE;
     if (f) return PUFF_OUT_Z2_BUFFER;

@@ -37,6 +37,28 @@ struct PartPtrs {
     double* g;
 };
 
+// Deterministic deposit of the FP64 form.  Every contribution v = w_k s of a
+// macroparticle to a node is split into two fixed-point integers,
+// hi = rint(v 2^F) and lo = rint((v 2^F - hi) 2^DET_LO_BITS), and summed with
+// integer atomics: integer sums do not depend on the order in which lanes and
+// workgroups add them, so the deposit, and with it the whole step, is the
+// same bit for bit from run to run whatever the atomics' order or the
+// macroparticles' slot order.  DET_WORDS int64 per node: Re hi, Re lo, Im hi,
+// Im lo.  F is set per step from the largest source of the step's start
+// state (k_det_bound) so that n (4 max |s|) 2^F < 2^62: no node sum can
+// overflow, and the sums resolve 2^-(F + DET_LO_BITS), some 2^-68 of the
+// largest source at 2^24 macroparticles.  A conversion pass after each stage
+// turns the integers back into FP64 deposits.
+constexpr int DET_WORDS = 4;
+constexpr int DET_LO_BITS = 32;
+constexpr int DET_SAFETY = 4;
+
+struct DetState {
+    unsigned long long* acc;   // DET_WORDS per mesh node, zero between stages
+    int* shift;                // F of the current step (device word)
+    unsigned long long* bound; // bits of the largest |source| (device word)
+};
+
 template <typename R>
 struct StageArgs {
     int64_t n;
@@ -56,6 +78,7 @@ struct StageArgs {
     StageScalars s;
     int* flags;
     Box* box;
+    DetState det;       // acc == nullptr: the atomic FP64 deposit
 };
 
 template <typename R>
@@ -75,6 +98,14 @@ void launch_step_begin(int* flags, Box* box, hipStream_t s);
 template <typename R>
 void launch_combine(const CombineArgs<R>& a, hipStream_t s);
 void launch_wrap(double* z2, int64_t n, double lenz2, hipStream_t s);
+// Deterministic deposit: F of the step from the sources of the state y
+// (particles with a flag contribute nothing later, so all are bounded), and
+// the conversion of the integer sums of one stage into dA over the step's
+// active box (clearing the sums).
+void launch_det_scale(const PartPtrs<double>& y, const double* chi, int64_t n, const Physics& ph,
+                      const DetState& d, hipStream_t s);
+void launch_det_convert(const DetState& d, cplx<double>* dA, const Box* box, int nx, int ny, int nzp,
+                        int periodic, int one_d, hipStream_t s);
 
 // ---- discrete adjoint of one RK4 step -----------------------------------------
 // Stage s (4 .. 1) pulls the cotangent a_s = ws L + cnext mu_{s+1} of its

@@ -18,8 +18,9 @@
 
 namespace puff {
 
-// One RK4 stage.  STAGE = 1..4 (rk4par's four derivs calls).
-template <typename R, int STAGE>
+// One RK4 stage.  STAGE = 1..4 (rk4par's four derivs calls).  DET: the
+// deterministic fixed-point deposit (FP64 form only; see DetState).
+template <typename R, int STAGE, bool DET>
 __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // Flags raised by an earlier kernel of this step do not stop stages 1-3:
     // they write only the step's scratch (k_s, dA_s), and a particle whose
@@ -110,12 +111,15 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // The deposit tile accumulates in FP64 in both forms.  An FP32 tile for
     // the mixed form (ds_add_f32 in place of ds_add_f64) measured 3x slower
     // per stage on the C1 beam.
+    // The deterministic deposit's tile holds DET_WORDS 8-byte integers per
+    // node instead of two doubles.
     constexpr int TILE = stage_tile_nodes<R>();
+    constexpr int WORDS = DET ? DET_WORDS : 2;
     using D = double;
-    static_assert(2 * FIELD_TILE_NODES * sizeof(D) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
-                      2 * TILE * sizeof(D), "field tile inside the deposit tile");
-    __shared__ D s_dep[2 * TILE];
-    cplx<R>* s_fld = reinterpret_cast<cplx<R>*>(s_dep + 2 * FIELD_TILE_NODES);
+    static_assert(WORDS * FIELD_TILE_NODES * sizeof(D) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
+                      WORDS * TILE * sizeof(D), "field tile inside the deposit tile");
+    __shared__ D s_dep[WORDS * TILE];
+    cplx<R>* s_fld = reinterpret_cast<cplx<R>*>(s_dep + WORDS * FIELD_TILE_NODES);
     const R cA = (R)a.cin;
     const bool lds_field = use_tile && tb.tx * tb.ty * tb.tz <= FIELD_TILE_NODES;
     if (lds_field) {
@@ -164,7 +168,16 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     if (ph.field_evolve) {
         const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
         const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
-        scatter_cells<R>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout, s_dep);
+        if constexpr (DET) {
+            // n |s| 2^F <= 2^62 keeps every node's integer sums in range
+            const int F = *a.det.shift;
+            const double lim = ldexp(1.0, 62 - F) / (double)a.n;
+            if (ok && (fabs((double)sr) > lim || fabs((double)si) > lim)) atomicOr(a.flags, FLAG_DET_RANGE);
+            scatter_cells_det(m, tb, use_tile, ok, c, cr, (double)sr, (double)si, ldexp(1.0, F), a.det.acc,
+                              reinterpret_cast<unsigned long long*>(s_dep));
+        } else {
+            scatter_cells<R>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout, s_dep);
+        }
     }
     if (!in) return;
     if (!ph.coupling) { fr = 0; fi = 0; }   // rhs.f90:221-224
@@ -269,6 +282,77 @@ __global__ __launch_bounds__(256) void k_field_combine(CombineArgs<R> a) {
     }
 }
 
+// ---- deterministic deposit ---------------------------------------------------
+
+// Largest |source| chi / dV3 (1 + eta p2) |p_r|, |p_i| / Gamma of the state y,
+// reduced as the bits of a non-negative double: their integer order is the
+// value order, so the integer maximum is exact whatever the atomics' order.
+__global__ __launch_bounds__(256) void k_det_bound(PartPtrs<double> y, const double* chi, int64_t n,
+                                                   Physics ph, unsigned long long* bound) {
+    double v = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double pr = y.pr[i], pi = y.pi[i];
+        const double invg = 1.0 / y.g[i];
+        const double p2 = ((1.0 / sqrt(1.0 - ph.inv_gr2 * invg * invg * (1.0 + ph.aw * ph.aw * (pr * pr + pi * pi)))) -
+                           1.0) * ph.inv_eta;
+        const double s = chi[i] * ph.inv_dV3 * (1.0 + ph.eta * p2) * invg;
+        v = fmax(v, fmax(fabs(s * pr), fabs(s * pi)));
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(bound, (unsigned long long)__double_as_longlong(v));
+}
+
+// F of the step: DET_SAFETY n max|s| < 2^(62 - F), i.e. 62 minus the binary
+// exponent e of that bound (frexp: bound < 2^e).  Clears the bound word for
+// the next step.
+__global__ void k_det_shift(unsigned long long* bound, int* shift, int64_t n) {
+    const double m = __longlong_as_double((long long)*bound);
+    int F = 0;
+    if (m > 0.0) {
+        int e = 0;
+        frexp((double)DET_SAFETY * (double)n * m, &e);
+        F = 62 - e;
+        F = F > 1000 ? 1000 : (F < -1000 ? -1000 : F);
+    }
+    *shift = F;
+    *bound = 0ull;
+}
+
+// dA = Re hi 2^-F + Re lo 2^-(F + DET_LO_BITS) (and Im) over the step's
+// active box so far, clearing the integer sums there.
+__global__ __launch_bounds__(256) void k_det_convert(DetState d, cplx<double>* dA, const Box* box, int nx,
+                                                     int ny, int nzp, int periodic, int one_d) {
+    const Box b = merged_box(box);
+    if (b.hi[0] < 0) return;
+    const int x0 = b.lo[0], x1 = min(b.hi[0] + 1, nx - 1);
+    const int y0 = one_d ? 0 : b.lo[1], y1 = one_d ? 0 : min(b.hi[1] + 1, ny - 1);
+    const int z0 = periodic ? 0 : b.lo[2];
+    const int z1 = periodic ? nzp - 1 : min(b.hi[2] + 1, nzp - 1);
+    const int lx = x1 - x0 + 1, ly = y1 - y0 + 1, lz = z1 - z0 + 1;
+    const int64_t vol = (int64_t)lx * ly * lz;
+    const int F = *d.shift;
+    const double inv_hi = ldexp(1.0, -F), inv_lo = ldexp(1.0, -(F + DET_LO_BITS));
+    const int64_t ntr = (int64_t)nx * ny;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < vol;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int ix = x0 + (int)(t % lx);
+        const int64_t r = t / lx;
+        const int iy = y0 + (int)(r % ly);
+        const int iz = z0 + (int)(r / ly);
+        const int64_t j = (int64_t)iz * ntr + (int64_t)iy * nx + ix;
+        unsigned long long* w = d.acc + DET_WORDS * j;
+        const long long rh = (long long)w[0], rl = (long long)w[1], ih = (long long)w[2], il = (long long)w[3];
+        dA[j] = {(double)rh * inv_hi + (double)rl * inv_lo, (double)ih * inv_hi + (double)il * inv_lo};
+        if (rh | rl | ih | il) {
+            w[0] = 0ull;
+            w[1] = 0ull;
+            w[2] = 0ull;
+            w[3] = 0ull;
+        }
+    }
+}
+
 // Periodic z2 wrap (para_field.f90:212-215).
 __global__ void k_wrap_z2(double* z2, int64_t n, double lenz2) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -282,12 +366,37 @@ void launch_stage(int stage, const StageArgs<R>& a, hipStream_t s) {
     const int threads = STAGE_THREADS;
     const int64_t blocks = (a.n + threads - 1) / threads;
     dim3 grid((unsigned)(blocks > 0 ? blocks : 1));
-    switch (stage) {
-        case 1: hipLaunchKernelGGL((k_stage<R, 1>), grid, dim3(threads), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((k_stage<R, 2>), grid, dim3(threads), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((k_stage<R, 3>), grid, dim3(threads), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_stage<R, 4>), grid, dim3(threads), 0, s, a); break;
+    if constexpr (sizeof(R) == 8) {
+        if (a.det.acc) {
+            switch (stage) {
+                case 1: hipLaunchKernelGGL((k_stage<R, 1, true>), grid, dim3(threads), 0, s, a); break;
+                case 2: hipLaunchKernelGGL((k_stage<R, 2, true>), grid, dim3(threads), 0, s, a); break;
+                case 3: hipLaunchKernelGGL((k_stage<R, 3, true>), grid, dim3(threads), 0, s, a); break;
+                default: hipLaunchKernelGGL((k_stage<R, 4, true>), grid, dim3(threads), 0, s, a); break;
+            }
+            return;
+        }
     }
+    switch (stage) {
+        case 1: hipLaunchKernelGGL((k_stage<R, 1, false>), grid, dim3(threads), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_stage<R, 2, false>), grid, dim3(threads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((k_stage<R, 3, false>), grid, dim3(threads), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_stage<R, 4, false>), grid, dim3(threads), 0, s, a); break;
+    }
+}
+
+void launch_det_scale(const PartPtrs<double>& y, const double* chi, int64_t n, const Physics& ph,
+                      const DetState& d, hipStream_t s) {
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_det_bound, dim3((unsigned)blocks), dim3(256), 0, s, y, chi, n, ph, d.bound);
+    hipLaunchKernelGGL(k_det_shift, dim3(1), dim3(1), 0, s, d.bound, d.shift, n);
+}
+
+void launch_det_convert(const DetState& d, cplx<double>* dA, const Box* box, int nx, int ny, int nzp,
+                        int periodic, int one_d, hipStream_t s) {
+    hipLaunchKernelGGL(k_det_convert, dim3(2048), dim3(256), 0, s, d, dA, box, nx, ny, nzp, periodic, one_d);
 }
 
 void launch_step_begin(int* flags, Box* box, hipStream_t s) {

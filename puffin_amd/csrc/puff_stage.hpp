@@ -339,4 +339,97 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
     }
 }
 
+// ---- deterministic deposit (FP64 form; see DetState) -------------------------
+
+// Fixed-point pair of one contribution: t = v 2^F is exact (a power-of-two
+// scaling) and so is t - rint(t) (Sterbenz: |t - rint(t)| <= 1/2 with
+// rint(t) within a factor 2 of t, or rint(t) = 0), so hi + lo 2^-DET_LO_BITS
+// is v 2^F rounded once, at 2^-DET_LO_BITS.  Two's-complement words: integer
+// atomics add them modulo 2^64, which is exact while the sums stay in range.
+__device__ __forceinline__ void det_split(double v, double scale, unsigned long long& hi,
+                                          unsigned long long& lo) {
+    const double t = v * scale;
+    const double r = rint(t);
+    hi = (unsigned long long)(long long)r;
+    lo = (unsigned long long)(long long)rint((t - r) * (double)(1ull << DET_LO_BITS));
+}
+
+// out[node_k] += w_k (sr, si) in fixed point for every lane with a cell.
+// The same two paths as scatter_cells (LDS tile over the workgroup's node
+// box, or runs of lanes that share a cell), with integer adds throughout, so
+// neither the order of the LDS atomics nor that of the global ones changes a
+// bit.  s_tile: DET_WORDS x nt words (Re hi | Re lo | Im hi | Im lo); out:
+// DET_WORDS words per mesh node.
+__device__ __forceinline__ void scatter_cells_det(const MeshGeom& m, const TileBox& t, bool use_tile,
+                                                  bool ok, const Cell<double>& c, const CellRef& cr,
+                                                  double sr, double si, double scale,
+                                                  unsigned long long* out, unsigned long long* s_tile) {
+    const int lane = threadIdx.x & 63;
+    const int nnb = m.one_d ? 2 : 8;
+    if (use_tile) {
+        const int nt = t.tx * t.ty * t.tz;
+        for (int q = threadIdx.x; q < DET_WORDS * nt; q += blockDim.x) s_tile[q] = 0ull;
+        __syncthreads();
+        if (ok) {
+            for (int q = 0; q < nnb; ++q) {
+                const int k = (q + lane) & (nnb - 1);
+                const double wk = corner_weight<double>(m, c, k);
+                const int tk = corner_tile(m, t, cr, k);
+                unsigned long long h, l;
+                det_split(wk * sr, scale, h, l);
+                atomicAdd(&s_tile[tk], h);
+                atomicAdd(&s_tile[nt + tk], l);
+                det_split(wk * si, scale, h, l);
+                atomicAdd(&s_tile[2 * nt + tk], h);
+                atomicAdd(&s_tile[3 * nt + tk], l);
+            }
+        }
+        __syncthreads();
+        const int64_t ntr = (int64_t)m.nx * m.ny;
+        const BoxDiv bd = box_div(t);
+        for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+            const unsigned long long w[DET_WORDS] = {s_tile[q], s_tile[nt + q], s_tile[2 * nt + q],
+                                                     s_tile[3 * nt + q]};
+            if (w[0] | w[1] | w[2] | w[3]) {
+                int lx, ly, lz;
+                box_coords(bd, q, lx, ly, lz);
+                const int64_t node = (int64_t)(t.bz0 + lz) * ntr + (int64_t)(t.by0 + ly) * m.nx + (t.bx0 + lx);
+                unsigned long long* o = out + DET_WORDS * node;
+#pragma unroll
+                for (int j = 0; j < DET_WORDS; ++j)
+                    if (w[j]) atomicAdd(o + j, w[j]);
+            }
+        }
+    } else {
+        const int prev = __shfl_up(cr.key, 1, 64);
+        const bool head = (lane == 0) || (prev != cr.key);
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long after = (lane == 63) ? 0ull : (heads >> (lane + 1));
+        const int seg_end = after ? lane + __builtin_ctzll(after) : 63;
+        const int maxlen = wave_max(head ? seg_end - lane + 1 : 1);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k < nnb) {
+                const double wk = ok ? corner_weight<double>(m, c, k) : 0.0;
+                unsigned long long w[DET_WORDS];
+                det_split(wk * sr, scale, w[0], w[1]);
+                det_split(wk * si, scale, w[2], w[3]);
+                for (int o = 1; o < maxlen; o <<= 1) {
+#pragma unroll
+                    for (int j = 0; j < DET_WORDS; ++j) {
+                        const unsigned long long u = __shfl_down(w[j], o, 64);
+                        if (lane + o <= seg_end) w[j] += u;
+                    }
+                }
+                if (ok && head) {
+                    unsigned long long* o = out + DET_WORDS * (int64_t)corner_node(m, cr, k);
+#pragma unroll
+                    for (int j = 0; j < DET_WORDS; ++j)
+                        if (w[j]) atomicAdd(o + j, w[j]);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace puff

@@ -39,6 +39,7 @@ SIGNATURES = {
     "puff_get_taper": (ctypes.c_int, [_vp, _dp, _dp]),
     "puff_set_taper_table": (ctypes.c_int, [_vp, _dp, ctypes.c_int32, ctypes.c_double, ctypes.c_double]),
     "puff_set_sort_interval": (ctypes.c_int, [_vp, ctypes.c_int32]),
+    "puff_set_deterministic": (ctypes.c_int, [_vp, ctypes.c_int32]),
     "puff_rk4par": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
     "puff_diffraction_step": (ctypes.c_int, [_vp, ctypes.c_double]),
     "puff_transform": (ctypes.c_int, [_vp, ctypes.c_int32]),
@@ -223,6 +224,47 @@ class Integrator:
     def set_stream(self, stream_ptr: int | None):
         _check(lib().puff_set_stream(self._h, stream_ptr), "puff_set_stream")
 
+    # torch tensors on the context's device (FP64, reference layout).  The
+    # uploads first wait for torch's current stream, so that a tensor torch
+    # has just produced is complete before the context's stream reads it;
+    # downloads return once the copies are done.
+    @staticmethod
+    def _torch_ready(tensors):
+        import torch
+        for t in tensors:
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError("expected contiguous float64 device tensors")
+        torch.cuda.current_stream().synchronize()
+        return [t.data_ptr() for t in tensors]
+
+    def use_torch_stream(self):
+        """Run the context's work on torch's current stream."""
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    def upload_particles_torch(self, p: dict):
+        ts = [p[k] for k in PARTICLE_KEYS + ("chi",)]
+        for t in ts:
+            assert t.numel() == self.n
+        self.upload_particles_device(self._torch_ready(ts))
+
+    def download_particles_torch(self) -> dict:
+        import torch
+        out = {k: torch.empty(self.n, dtype=torch.float64, device="cuda") for k in PARTICLE_KEYS}
+        self.download_particles_device(self._torch_ready([out[k] for k in PARTICLE_KEYS]))
+        return out
+
+    def upload_field_torch(self, ar, ai):
+        assert ar.numel() == self.mp.nmesh and ai.numel() == self.mp.nmesh
+        self.upload_field_device(*self._torch_ready([ar, ai]))
+
+    def download_field_torch(self):
+        import torch
+        ar = torch.empty(self.mp.nmesh, dtype=torch.float64, device="cuda")
+        ai = torch.empty_like(ar)
+        self.download_field_device(*self._torch_ready([ar, ai]))
+        return ar, ai
+
     def synchronize(self):
         _check(lib().puff_synchronize(self._h), "puff_synchronize")
 
@@ -239,6 +281,10 @@ class Integrator:
         v = np.ascontiguousarray(values, dtype=np.float64)
         _check(lib().puff_set_taper_table(self._h, v.ctypes.data_as(_dp), v.size, z0, period),
                "puff_set_taper_table")
+
+    def set_deterministic(self, on: bool = True):
+        """Fixed-point deposit: results independent of the atomics' order (FP64)."""
+        _check(lib().puff_set_deterministic(self._h, int(on)), "puff_set_deterministic")
 
     def set_sort_interval(self, steps: int):
         _check(lib().puff_set_sort_interval(self._h, steps), "puff_set_sort_interval")
@@ -398,6 +444,16 @@ class Integrator:
         fi = None if lai is None else np.ascontiguousarray(lai, dtype=np.float64).reshape(-1)
         _check(lib().puff_adjoint_upload(self._h, MEM_HOST, *[_ptr(a) for a in arrs], _ptr(fr), _ptr(fi)),
                "puff_adjoint_upload")
+
+    def adjoint_download_field_torch(self):
+        """The field cotangent into torch device tensors (split Re / Im)."""
+        import torch
+        lar = torch.empty(self.mp.nmesh, dtype=torch.float64, device="cuda")
+        lai = torch.empty_like(lar)
+        ptrs = self._torch_ready([lar, lai])
+        _check(lib().puff_adjoint_download(self._h, MEM_DEVICE, None, None, None, None, None, None,
+                                           ptrs[0], ptrs[1]), "puff_adjoint_download")
+        return lar, lai
 
     def adjoint_download(self):
         out = {k: np.empty(self.n) for k in PARTICLE_KEYS}

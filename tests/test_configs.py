@@ -73,7 +73,7 @@ def uncoupled_orbits(case, parts):
 
 @pytest.mark.parametrize("make,nsteps", [(case_c1, 1920), (case_c4, 3840)])
 def test_temporal_workload_stays_on_mesh(make, nsteps):
-    case = make(n=4096)
+    case = make(n=4096, with_field=False)
     assert case.nsteps == nsteps
     mp = case.mp
     parts, lo, hi = extreme_particles(case, case.n2col0)

@@ -134,7 +134,8 @@ def test_c2_mixed_outputs_against_oracle_fp64(c1_sample_oracle):
     tolerances, relative to the largest FP64 value: power(z2), bunching and
     on-axis spectrum 1e-4 (the north-star bar for integrated power), the
     integrated power 1e-5, z2 and Gamma (FP64 state, driven by FP32 p_perp and
-    field) 1e-8, x, y, p_perp (FP32 state) 1e-5."""
+    field) 1e-8, x, y (FP32 state) 1e-5, p_perp (FP32) 1e-5 of the wiggle
+    amplitude fx n2col = 1."""
     case, o = c1_sample_oracle
     out, ar, ai, power, bunch, nsl, lam = run_module_gpu(case, MIXED, C1_STEPS)
     p64 = o.power()
@@ -144,9 +145,15 @@ def test_c2_mixed_outputs_against_oracle_fp64(c1_sample_oracle):
     e_s = relerr(on_axis_spectrum(case.mp, ar, ai), on_axis_spectrum(case.mp, o.ar, o.ai))
     say(f"C2 power(z) {e_pow:.2e} integrated {e_int:.2e} bunching {e_b:.2e} spectrum {e_s:.2e}")
     assert e_pow < 1e-4 and e_b < 1e-4 and e_s < 1e-4 and e_int < 1e-5
+    # p_perp is compared on the scale of the wiggle amplitude (fx n2col = 1):
+    # at the end of a period the planar wiggle puts p_y near zero
+    pscale = max(1.0, np.abs(o.parts["px"]).max(), np.abs(o.parts["py"]).max())
     for k in KEYS:
-        e = relerr(out[k], o.parts[k])
-        say(f"C2 {k} rel err {e:.2e}")
+        if k in ("px", "py"):
+            e = float(np.abs(out[k] - o.parts[k]).max() / pscale)
+        else:
+            e = relerr(out[k], o.parts[k])
+        say(f"C2 {k} error {e:.2e}")
         assert e < (1e-8 if k in ("z2", "gam") else 1e-5), k
 
 
