@@ -59,12 +59,14 @@ struct Box {
 constexpr int BOX_COPIES = 64;
 
 // Workgroup of the RK4 stage kernels (one macroparticle per thread) and the
-// node capacity of its LDS deposit tile (2 FP64 per node).  Four-wave
-// workgroups (one wave per SIMD each); the stage kernels hold 51-71 VGPRs,
-// so the LDS tiles set how many workgroups share a CU: 1280 nodes (20 KiB,
-// 6 per CU) in the FP64 form, 1536 (24 KiB) in the mixed form, whose
-// boxes overflowing a smaller tile cost more re-sorts than the occupancy
-// gains (measured on the C1 beam).
+// node capacity of its LDS deposit tile.  Four-wave workgroups (one wave per
+// SIMD each); the stage kernels hold 42-76 VGPRs, so the LDS tiles set how
+// many workgroups share a CU: 1280 nodes (2 FP64 each, 20 KiB) in the FP64
+// form, 3072 (one packed 8-byte word each, 24 KiB) in the mixed form.
+// Measured on the C1 beam: FP64 tiles of 768 and 2048 nodes gave 3.17 and
+// 3.07 ms steps against 2.87 ms; mixed FP64 tiles of 1024 and 2048 nodes
+// 2.39 and 2.54 ms against 2.28 ms at 1536; the packed mixed tile 2.21 ms
+// at 1536 nodes and 2.18 ms at 3072.
 #ifndef PUFF_STAGE_THREADS
 #define PUFF_STAGE_THREADS 256
 #endif
@@ -73,7 +75,12 @@ constexpr int STAGE_THREADS = PUFF_STAGE_THREADS;
 #define PUFF_TILE_NODES_FP64 1280
 #endif
 #ifndef PUFF_TILE_NODES_MIXED
-#define PUFF_TILE_NODES_MIXED 1536
+#define PUFF_TILE_NODES_MIXED 3072
+#endif
+// The mixed form's LDS deposit tile: packed fixed point (default) or, with
+// PUFF_MIXED_TILE_FP64 = 1, the FP64 tile of the FP64 form.
+#ifndef PUFF_MIXED_TILE_FP64
+#define PUFF_MIXED_TILE_FP64 0
 #endif
 template <typename R>
 constexpr int stage_tile_nodes() {

@@ -55,7 +55,9 @@ __device__ __forceinline__ void lds_fill(cplx<R>* dst, int n, F get) {
     }
 }
 
-// put(w, src[w]) for 0 <= w < n.
+// put(w, src[w]) for 0 <= w < n.  The LDS reads are unconditional (element 0
+// stands in past n), so that v stays in registers: with conditional reads
+// the compiler kept v in scratch memory (144 bytes per lane in FP64).
 template <int U, typename R, typename F>
 __device__ __forceinline__ void lds_drain(const cplx<R>* src, int n, F put) {
     for (int w0 = threadIdx.x; w0 < n; w0 += U * (int)blockDim.x) {
@@ -63,7 +65,7 @@ __device__ __forceinline__ void lds_drain(const cplx<R>* src, int n, F put) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int w = w0 + u * (int)blockDim.x;
-            if (w < n) v[u] = src[w];
+            v[u] = src[w < n ? w : 0];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -119,135 +121,172 @@ __device__ __forceinline__ void dft16(cplx<R>* v, int sign) {
     for (int q0 = 0; q0 < 4; ++q0) dft4(v[4 * q0], v[4 * q0 + 1], v[4 * q0 + 2], v[4 * q0 + 3], sign);
 }
 
-// In-LDS mixed-radix transform of length N over elements 0..N-1 of B lines
-// laid out with line length LD (N <= LD; elements N..LD-1 are left alone).
-// src holds the input; returns the buffer holding the output (src or dst).
+// ---- compile-time line plans --------------------------------------------------
+// The line lengths of the workloads get kernels with their length, radix
+// passes (factorize()'s order) and lines per workgroup as constants: every
+// index division and modulo by B, T, p and N then becomes a multiply and
+// shift, and the pass loops unroll.  Other lengths run the same code with
+// the runtime plan (NC = 0).
+template <int N>
+struct CtPlan {
+    static constexpr int np = 0;
+    static constexpr int rader = 0;
+    static constexpr int r[1] = {0};
+};
+#define PUFF_CT_PLAN(N_, RADER_, ...)                         \
+    template <>                                               \
+    struct CtPlan<N_> {                                       \
+        static constexpr int r[] = {__VA_ARGS__};             \
+        static constexpr int np = sizeof(r) / sizeof(int);    \
+        static constexpr int rader = RADER_;                  \
+    };
+PUFF_CT_PLAN(33, 0, 11, 3)
+PUFF_CT_PLAN(129, 0, 43, 3)
+PUFF_CT_PLAN(257, 1, 16, 16)   // Rader: transforms of M = 256
+PUFF_CT_PLAN(128, 0, 16, 4, 2)
+PUFF_CT_PLAN(256, 0, 16, 16)
+PUFF_CT_PLAN(1024, 0, 16, 16, 4)
+PUFF_CT_PLAN(4096, 0, 16, 16, 16)
+#undef PUFF_CT_PLAN
+
+// One radix-r pass (after passes whose radices multiply to p) of an in-LDS
+// mixed-radix transform of length N over elements 0..N-1 of B lines laid
+// out with line length LD (N <= LD; elements N..LD-1 are left alone): src
+// to dst.  With constant arguments (ct_passes) every index division folds.
 template <typename R, bool IL>
-__device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N, int npass,
-                               const int* radix, const cplx<R>* tw, int sign) {
+__device__ __forceinline__ void lds_pass(const cplx<R>* src, cplx<R>* dst, int B, int LD, int N, int r,
+                                         int p, const cplx<R>* tw, int sign) {
+    const int T = N / r;
+    // twiddle exponents j k twst (j < r, k < p) stay below N: no reduction
+    const int twst = N / (p * r);
+    if (r == 16) {
+        for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
+            int b, i;
+            if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
+            const int k = i % p;
+            const int ob = (i / p) * p * 16 + k;
+            cplx<R> v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
+                if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
+            }
+            dft16(v, sign);
+#pragma unroll
+            for (int q0 = 0; q0 < 4; ++q0)
+#pragma unroll
+                for (int q1 = 0; q1 < 4; ++q1)
+                    dst[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, LD)] = v[4 * q0 + q1];
+        }
+    } else if (r == 2 || r == 3 || r == 4) {
+        for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
+            int b, i;
+            if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
+            const int k = i % p;
+            const int ob = (i / p) * p * r + k;
+            cplx<R> v[4];
+            for (int j = 0; j < r; ++j) {
+                v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
+                if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
+            }
+            if (r == 2) {
+                dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], v[1]);
+                dst[lidx<IL>(b, ob + p, B, LD)] = csub(v[0], v[1]);
+            } else if (r == 3) {
+                const R s3 = (R)0.86602540378443864676 * (R)(sign < 0 ? -1 : 1);
+                const cplx<R> t1 = cadd(v[1], v[2]);
+                const cplx<R> t2 = {v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y};
+                const cplx<R> d = csub(v[1], v[2]);
+                const cplx<R> t3 = {-s3 * d.y, s3 * d.x};   // i s s3 (x1 - x2)
+                dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], t1);
+                dst[lidx<IL>(b, ob + p, B, LD)] = cadd(t2, t3);
+                dst[lidx<IL>(b, ob + 2 * p, B, LD)] = csub(t2, t3);
+            } else {
+                const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+                const cplx<R> b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+                // W4 b1: sign -1 -> -i b1, sign +1 -> +i b1
+                const cplx<R> wb = (sign < 0) ? cplx<R>{b1.y, -b1.x} : cplx<R>{-b1.y, b1.x};
+                dst[lidx<IL>(b, ob, B, LD)] = cadd(a0, b0);
+                dst[lidx<IL>(b, ob + p, B, LD)] = cadd(a1, wb);
+                dst[lidx<IL>(b, ob + 2 * p, B, LD)] = csub(a0, b0);
+                dst[lidx<IL>(b, ob + 3 * p, B, LD)] = csub(a1, wb);
+            }
+        }
+    } else {
+        // direct DFT pass for an odd prime r, folded: with the pre-twiddled
+        // inputs y_j = x_j W^(j k twst), u_j = y_j + y_(r-j),
+        // v_j = y_j - y_(r-j) (j = 1 .. (r-1)/2) and w^(jq) = c + i s
+        // (w^((r-j)q) = c - i s),
+        //   X_q     = y_0 + sum u_j c + i sum v_j s
+        //   X_(r-q) = y_0 + sum u_j c - i sum v_j s,
+        // four FMAs per input pair for both outputs.
+        const int nr = N / r;
+        const int hq = (r + 1) / 2;
+        // each work item forms the outputs of two values of q (qa = 2 qq
+        // and qb = qa + 1, with their mirrors r - q), sharing the input
+        // loads, the folding and the loop bookkeeping
+        const int nqq = (hq + 1) / 2;
+        for (int w = threadIdx.x; w < B * T * nqq; w += blockDim.x) {
+            int b, t;
+            if (IL) { b = w % B; t = w / B; } else { t = w % (T * nqq); b = w / (T * nqq); }
+            const int qa = 2 * (t / T);
+            const bool hasb = qa + 1 < hq;
+            const int qb = hasb ? qa + 1 : 0;   // qb = 0: a valid table index, result unused
+            const int i = t % T;
+            const int k = i % p;
+            const int dk = k * twst;
+            const int dqa = qa * nr, dqb = qb * nr;
+            const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
+            int e1 = dk, e2 = (r - 1) * dk, fa = dqa, fb = dqb;   // e1, e2 < N throughout
+            R sra = 0, sia = 0, tra = 0, tia = 0;
+            R srb = 0, sib = 0, trb = 0, tib = 0;
+#pragma unroll 4
+            for (int j = 1; j < hq; ++j) {
+                cplx<R> xa = src[lidx<IL>(b, i + j * T, B, LD)];
+                cplx<R> xb = src[lidx<IL>(b, i + (r - j) * T, B, LD)];
+                if (dk) {
+                    xa = cmul(xa, tw_get(tw, e1, sign));
+                    xb = cmul(xb, tw_get(tw, e2, sign));
+                }
+                const R ux = xa.x + xb.x, uy = xa.y + xb.y;
+                const R vx = xa.x - xb.x, vy = xa.y - xb.y;
+                const cplx<R> oa = tw_get(tw, fa, sign);
+                const cplx<R> obw = tw_get(tw, fb, sign);
+                sra = fma(ux, oa.x, sra);
+                sia = fma(uy, oa.x, sia);
+                tra = fma(vx, oa.y, tra);
+                tia = fma(vy, oa.y, tia);
+                srb = fma(ux, obw.x, srb);
+                sib = fma(uy, obw.x, sib);
+                trb = fma(vx, obw.y, trb);
+                tib = fma(vy, obw.y, tib);
+                e1 += dk;
+                e2 -= dk;
+                fa += dqa;
+                if (fa >= N) fa -= N;
+                fb += dqb;
+                if (fb >= N) fb -= N;
+            }
+            const int ob = (i / p) * p * r + k;
+            dst[lidx<IL>(b, ob + qa * p, B, LD)] = {y0.x + sra - tia, y0.y + sia + tra};
+            if (qa > 0) dst[lidx<IL>(b, ob + (r - qa) * p, B, LD)] = {y0.x + sra + tia, y0.y + sia - tra};
+            if (hasb) {
+                dst[lidx<IL>(b, ob + qb * p, B, LD)] = {y0.x + srb - tib, y0.y + sib + trb};
+                dst[lidx<IL>(b, ob + (r - qb) * p, B, LD)] = {y0.x + srb + tib, y0.y + sib - trb};
+            }
+        }
+    }
+}
+
+// The passes of a runtime plan.  src holds the input; returns the buffer
+// holding the output (src or dst).
+template <typename R, bool IL>
+__device__ __forceinline__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N, int npass,
+                                               const int* radix, const cplx<R>* tw, int sign) {
     int p = 1;
     for (int ps = 0; ps < npass; ++ps) {
         const int r = radix[ps];
-        const int T = N / r;
-        // twiddle exponents j k twst (j < r, k < p) stay below N: no reduction
-        const int twst = N / (p * r);
-        if (r == 16) {
-            for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
-                int b, i;
-                if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
-                const int k = i % p;
-                const int ob = (i / p) * p * 16 + k;
-                cplx<R> v[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
-                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
-                }
-                dft16(v, sign);
-#pragma unroll
-                for (int q0 = 0; q0 < 4; ++q0)
-#pragma unroll
-                    for (int q1 = 0; q1 < 4; ++q1)
-                        dst[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, LD)] = v[4 * q0 + q1];
-            }
-        } else if (r == 2 || r == 3 || r == 4) {
-            for (int w = threadIdx.x; w < B * T; w += blockDim.x) {
-                int b, i;
-                if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
-                const int k = i % p;
-                const int ob = (i / p) * p * r + k;
-                cplx<R> v[4];
-                for (int j = 0; j < r; ++j) {
-                    v[j] = src[lidx<IL>(b, i + j * T, B, LD)];
-                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
-                }
-                if (r == 2) {
-                    dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], v[1]);
-                    dst[lidx<IL>(b, ob + p, B, LD)] = csub(v[0], v[1]);
-                } else if (r == 3) {
-                    const R s3 = (R)0.86602540378443864676 * (R)(sign < 0 ? -1 : 1);
-                    const cplx<R> t1 = cadd(v[1], v[2]);
-                    const cplx<R> t2 = {v[0].x - (R)0.5 * t1.x, v[0].y - (R)0.5 * t1.y};
-                    const cplx<R> d = csub(v[1], v[2]);
-                    const cplx<R> t3 = {-s3 * d.y, s3 * d.x};   // i s s3 (x1 - x2)
-                    dst[lidx<IL>(b, ob, B, LD)] = cadd(v[0], t1);
-                    dst[lidx<IL>(b, ob + p, B, LD)] = cadd(t2, t3);
-                    dst[lidx<IL>(b, ob + 2 * p, B, LD)] = csub(t2, t3);
-                } else {
-                    const cplx<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
-                    const cplx<R> b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
-                    // W4 b1: sign -1 -> -i b1, sign +1 -> +i b1
-                    const cplx<R> wb = (sign < 0) ? cplx<R>{b1.y, -b1.x} : cplx<R>{-b1.y, b1.x};
-                    dst[lidx<IL>(b, ob, B, LD)] = cadd(a0, b0);
-                    dst[lidx<IL>(b, ob + p, B, LD)] = cadd(a1, wb);
-                    dst[lidx<IL>(b, ob + 2 * p, B, LD)] = csub(a0, b0);
-                    dst[lidx<IL>(b, ob + 3 * p, B, LD)] = csub(a1, wb);
-                }
-            }
-        } else {
-            // direct DFT pass for an odd prime r, folded: with the pre-twiddled
-            // inputs y_j = x_j W^(j k twst), u_j = y_j + y_(r-j),
-            // v_j = y_j - y_(r-j) (j = 1 .. (r-1)/2) and w^(jq) = c + i s
-            // (w^((r-j)q) = c - i s),
-            //   X_q     = y_0 + sum u_j c + i sum v_j s
-            //   X_(r-q) = y_0 + sum u_j c - i sum v_j s,
-            // four FMAs per input pair for both outputs.
-            const int nr = N / r;
-            const int hq = (r + 1) / 2;
-            // each work item forms the outputs of two values of q (qa = 2 qq
-            // and qb = qa + 1, with their mirrors r - q), sharing the input
-            // loads, the folding and the loop bookkeeping
-            const int nqq = (hq + 1) / 2;
-            for (int w = threadIdx.x; w < B * T * nqq; w += blockDim.x) {
-                int b, t;
-                if (IL) { b = w % B; t = w / B; } else { t = w % (T * nqq); b = w / (T * nqq); }
-                const int qa = 2 * (t / T);
-                const bool hasb = qa + 1 < hq;
-                const int qb = hasb ? qa + 1 : 0;   // qb = 0: a valid table index, result unused
-                const int i = t % T;
-                const int k = i % p;
-                const int dk = k * twst;
-                const int dqa = qa * nr, dqb = qb * nr;
-                const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
-                int e1 = dk, e2 = (r - 1) * dk, fa = dqa, fb = dqb;   // e1, e2 < N throughout
-                R sra = 0, sia = 0, tra = 0, tia = 0;
-                R srb = 0, sib = 0, trb = 0, tib = 0;
-#pragma unroll 4
-                for (int j = 1; j < hq; ++j) {
-                    cplx<R> xa = src[lidx<IL>(b, i + j * T, B, LD)];
-                    cplx<R> xb = src[lidx<IL>(b, i + (r - j) * T, B, LD)];
-                    if (dk) {
-                        xa = cmul(xa, tw_get(tw, e1, sign));
-                        xb = cmul(xb, tw_get(tw, e2, sign));
-                    }
-                    const R ux = xa.x + xb.x, uy = xa.y + xb.y;
-                    const R vx = xa.x - xb.x, vy = xa.y - xb.y;
-                    const cplx<R> oa = tw_get(tw, fa, sign);
-                    const cplx<R> obw = tw_get(tw, fb, sign);
-                    sra = fma(ux, oa.x, sra);
-                    sia = fma(uy, oa.x, sia);
-                    tra = fma(vx, oa.y, tra);
-                    tia = fma(vy, oa.y, tia);
-                    srb = fma(ux, obw.x, srb);
-                    sib = fma(uy, obw.x, sib);
-                    trb = fma(vx, obw.y, trb);
-                    tib = fma(vy, obw.y, tib);
-                    e1 += dk;
-                    e2 -= dk;
-                    fa += dqa;
-                    if (fa >= N) fa -= N;
-                    fb += dqb;
-                    if (fb >= N) fb -= N;
-                }
-                const int ob = (i / p) * p * r + k;
-                dst[lidx<IL>(b, ob + qa * p, B, LD)] = {y0.x + sra - tia, y0.y + sia + tra};
-                if (qa > 0) dst[lidx<IL>(b, ob + (r - qa) * p, B, LD)] = {y0.x + sra + tia, y0.y + sia - tra};
-                if (hasb) {
-                    dst[lidx<IL>(b, ob + qb * p, B, LD)] = {y0.x + srb - tib, y0.y + sib + trb};
-                    dst[lidx<IL>(b, ob + (r - qb) * p, B, LD)] = {y0.x + srb + tib, y0.y + sib - trb};
-                }
-            }
-        }
+        lds_pass<R, IL>(src, dst, B, LD, N, r, p, tw, sign);
         __syncthreads();
         cplx<R>* t = src;
         src = dst;
@@ -257,15 +296,38 @@ __device__ cplx<R>* lds_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
     return src;
 }
 
+// The passes K.. of CtPlan<NC> (P: product of the earlier radices), with
+// the radix and its offsets as constants.
+template <typename R, bool IL, int NC, int K = 0, int P = 1>
+__device__ __forceinline__ cplx<R>* ct_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
+                                              const cplx<R>* tw, int sign) {
+    if constexpr (K == CtPlan<NC>::np) {
+        return src;
+    } else {
+        constexpr int r = CtPlan<NC>::r[K];
+        lds_pass<R, IL>(src, dst, B, LD, N, r, P, tw, sign);
+        __syncthreads();
+        return ct_passes<R, IL, NC, K + 1, P * r>(dst, src, B, LD, N, tw, sign);
+    }
+}
+
+// Plan NC's passes when NC > 0, else the runtime plan of ax.
+template <typename R, bool IL, int NC>
+__device__ __forceinline__ cplx<R>* plan_passes(cplx<R>* src, cplx<R>* dst, int B, int LD, int N,
+                                                const FftAxis& ax, const cplx<R>* tw, int sign) {
+    if constexpr (NC > 0) return ct_passes<R, IL, NC>(src, dst, B, LD, N, tw, sign);
+    else return lds_passes<R, IL>(src, dst, B, LD, N, ax.npass, ax.radix, tw, sign);
+}
+
 // Rader's algorithm for a prime N with N - 1 = M a power of two (FftAxis):
 // a_j = x[g^j], A = DFT_M(a), C = A . rb_s, c = IDFT_M(C) (unnormalised; rb
 // carries the 1/M), X[g^-q] = x_0 + c_q, X_0 = x_0 + A_0.  Element M of each
 // line buffer is outside the M-point transforms: it keeps x_0 in dst and
 // X_0 in src until the final permutation.
-template <typename R, bool IL>
-__device__ cplx<R>* lds_rader(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
-                              const cplx<R>* tw, int sign) {
-    const int N = ax.n, M = N - 1;
+template <typename R, bool IL, int NC>
+__device__ __forceinline__ cplx<R>* lds_rader(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
+                                              const cplx<R>* tw, int sign) {
+    const int N = NC ? NC : ax.n, M = N - 1;
     const cplx<R>* rb = reinterpret_cast<const cplx<R>*>(ax.rb) + (sign > 0 ? M : 0);
     cplx<R>* P = dst;   // element M: x_0
     cplx<R>* Q = src;   // element M: X_0
@@ -275,7 +337,7 @@ __device__ cplx<R>* lds_rader(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& 
         P[lidx<IL>(b, j, B, N)] = src[lidx<IL>(b, j < M ? ax.rperm[j] : 0, B, N)];
     }
     __syncthreads();
-    cplx<R>* res = lds_passes<R, IL>(P, Q, B, N, M, ax.npass, ax.radix, tw, -1);
+    cplx<R>* res = plan_passes<R, IL, NC>(P, Q, B, N, M, ax, tw, -1);
     for (int w = threadIdx.x; w < B * M; w += blockDim.x) {
         int b, k;
         if (IL) { b = w % B; k = w / B; } else { k = w % M; b = w / M; }
@@ -288,7 +350,7 @@ __device__ cplx<R>* lds_rader(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& 
     }
     __syncthreads();
     cplx<R>* other = (res == P) ? Q : P;
-    res = lds_passes<R, IL>(res, other, B, N, M, ax.npass, ax.radix, tw, +1);
+    res = plan_passes<R, IL, NC>(res, other, B, N, M, ax, tw, +1);
     cplx<R>* out = (res == P) ? Q : P;
     // every output element except element M (which holds x_0 or X_0 in out)
     const int qm = M / 2;   // g^-(M/2) = -1 = M (mod N)
@@ -342,88 +404,113 @@ __device__ __forceinline__ void small_radix(cplx<R>* v, int r, int sign) {
 // the same buffer.  Half the LDS of the ping-pong form, hence more
 // workgroups per CU.
 template <typename R, bool IL>
-__device__ void lds_passes_inplace(cplx<R>* buf, int B, int N, int npass, const int* radix,
-                                   const cplx<R>* tw, int sign) {
-    int p = 1;
-    for (int ps = 0; ps < npass; ++ps) {
-        const int r = radix[ps];
-        const int T = N / r;
-        const int twst = N / (p * r);   // j k twst < N, as in lds_passes
-        const int w = threadIdx.x;
-        const bool act = w < B * T;
-        int b = 0, ob = 0;
-        cplx<R> v[16];
-        if (act) {
-            int i;
-            if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
-            const int k = i % p;
-            ob = (i / p) * p * r + k;
-            if (r == 16) {
+__device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, int p, const cplx<R>* tw,
+                                             int sign) {
+    const int T = N / r;
+    const int twst = N / (p * r);   // j k twst < N, as in lds_pass
+    const int w = threadIdx.x;
+    const bool act = w < B * T;
+    int b = 0, ob = 0;
+    cplx<R> v[16];
+    if (act) {
+        int i;
+        if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
+        const int k = i % p;
+        ob = (i / p) * p * r + k;
+        if (r == 16) {
 #pragma unroll
-                for (int j = 0; j < 16; ++j) {
+            for (int j = 0; j < 16; ++j) {
+                v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
+                if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
+            }
+            dft16(v, sign);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < r) {
                     v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
                     if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
                 }
-                dft16(v, sign);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (j < r) {
-                        v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
-                        if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
-                    }
-                }
-                small_radix(v, r, sign);
             }
+            small_radix(v, r, sign);
         }
-        __syncthreads();
-        if (act) {
-            if (r == 16) {
+    }
+    __syncthreads();
+    if (act) {
+        if (r == 16) {
 #pragma unroll
-                for (int q0 = 0; q0 < 4; ++q0)
+            for (int q0 = 0; q0 < 4; ++q0)
 #pragma unroll
-                    for (int q1 = 0; q1 < 4; ++q1)
-                        buf[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, N)] = v[4 * q0 + q1];
-            } else {
+                for (int q1 = 0; q1 < 4; ++q1)
+                    buf[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, N)] = v[4 * q0 + q1];
+        } else {
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if (q < r) buf[lidx<IL>(b, ob + q * p, B, N)] = v[q];
-            }
+            for (int q = 0; q < 4; ++q)
+                if (q < r) buf[lidx<IL>(b, ob + q * p, B, N)] = v[q];
         }
+    }
+}
+
+template <typename R, bool IL>
+__device__ __forceinline__ void lds_passes_inplace(cplx<R>* buf, int B, int N, int npass, const int* radix,
+                                                   const cplx<R>* tw, int sign) {
+    int p = 1;
+    for (int ps = 0; ps < npass; ++ps) {
+        const int r = radix[ps];
+        inplace_pass<R, IL>(buf, B, N, r, p, tw, sign);
         __syncthreads();
         p *= r;
     }
 }
 
-template <typename R, bool IL>
-__device__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
-                            const cplx<R>* tw, int sign) {
-    if (ax.rader) return lds_rader<R, IL>(src, dst, B, ax, tw, sign);
-    return lds_passes<R, IL>(src, dst, B, ax.n, ax.n, ax.npass, ax.radix, tw, sign);
+template <typename R, bool IL, int NC, int K = 0, int P = 1>
+__device__ __forceinline__ void ct_passes_inplace(cplx<R>* buf, int B, int N, const cplx<R>* tw, int sign) {
+    if constexpr (K < CtPlan<NC>::np) {
+        constexpr int r = CtPlan<NC>::r[K];
+        inplace_pass<R, IL>(buf, B, N, r, P, tw, sign);
+        __syncthreads();
+        ct_passes_inplace<R, IL, NC, K + 1, P * r>(buf, B, N, tw, sign);
+    }
+}
+
+template <typename R, bool IL, int NC>
+__device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
+                                            const cplx<R>* tw, int sign) {
+    if constexpr (NC > 0) {
+        if constexpr (CtPlan<NC>::rader) return lds_rader<R, IL, NC>(src, dst, B, ax, tw, sign);
+        else return ct_passes<R, IL, NC>(src, dst, B, NC, NC, tw, sign);
+    } else {
+        if (ax.rader) return lds_rader<R, IL, 0>(src, dst, B, ax, tw, sign);
+        return lds_passes<R, IL>(src, dst, B, ax.n, ax.n, ax.npass, ax.radix, tw, sign);
+    }
 }
 
 // z2-line transform: in place (one LDS buffer) or ping-pong.
-template <typename R, bool INPLACE>
+template <typename R, bool INPLACE, int NC>
 __device__ __forceinline__ cplx<R>* z_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
                                           const cplx<R>* tw, int sign) {
-    if (INPLACE) {
-        lds_passes_inplace<R, true>(src, B, ax.n, ax.npass, ax.radix, tw, sign);
+    if constexpr (INPLACE) {
+        if constexpr (NC > 0) ct_passes_inplace<R, true, NC>(src, B, NC, tw, sign);
+        else lds_passes_inplace<R, true>(src, B, ax.n, ax.npass, ax.radix, tw, sign);
         return src;
+    } else {
+        return lds_fft<R, true, NC>(src, dst, B, ax, tw, sign);
     }
-    return lds_fft<R, true>(src, dst, B, ax, tw, sign);
 }
 
 // ---- x lines (contiguous) --------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int sign) {
+template <typename R, int NC, int BC>
+__global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int op, int sign) {
     extern __shared__ unsigned char smem_raw[];
-    const int N = a.nx;
+    const int N = NC ? NC : a.nx;
+    const int B = BC ? BC : Bp;
+    const FftAxis& ax = a.ax;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* s1 = s0 + B * N;
     const int64_t nlines = (int64_t)a.ny * a.nzp;
     const int64_t L0 = (int64_t)blockIdx.x * B;
     const int nb = (int)min<int64_t>(B, nlines - L0);
-    const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(a.ax.tw);
+    const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(ax.tw);
     cplx<R>* tw = s1 + B * N;   // twiddle table staged in LDS
     for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
     cplx<R>* base = a.a + L0 * N;
@@ -455,7 +542,7 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
     }
     __syncthreads();
     const int first_sign = (op == XOP_FORWARD) ? sign : (op == XOP_ADJ_FOLD_SAVE_FWD ? -1 : +1);
-    cplx<R>* res = lds_fft<R, false>(s0, s1, B, a.ax, tw, first_sign);
+    cplx<R>* res = lds_fft<R, false, NC>(s0, s1, B, ax, tw, first_sign);
     if (op == XOP_ADJ_INV_MIX_FWD) {
         // adjoint of the absorption split: (1 - m) cotangent + m W^H G W cotangent / ffact
         const cplx<R>* lam = a.anb + L0 * N;
@@ -472,7 +559,7 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = lds_fft<R, false>(res, other, B, a.ax, tw, -1);
+        res = lds_fft<R, false, NC>(res, other, B, ax, tw, -1);
     } else if (op == XOP_ADJ_INV_SCALE) {
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
             const cplx<R> v = res[w];
@@ -495,7 +582,7 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = lds_fft<R, false>(res, other, B, a.ax, tw, -1);
+        res = lds_fft<R, false, NC>(res, other, B, ax, tw, -1);
     } else if (op == XOP_INV_RECOMBINE) {
         // /ffact + interior (diffraction.f90:513-517)
         const cplx<R>* anb = a.anb + L0 * N;
@@ -521,16 +608,18 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int B, int op, int
 }
 
 // ---- y lines (stride nx), B consecutive x per block -------------------------
-template <typename R>
-__global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int B, int sign) {
+template <typename R, int NC, int BC>
+__global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) {
     extern __shared__ unsigned char smem_raw[];
-    const int N = a.ny;
+    const int N = NC ? NC : a.ny;
+    const int B = BC ? BC : Bp;
+    const FftAxis& ay = a.ay;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* s1 = s0 + B * N;
     const int x0 = blockIdx.x * B;
     const int iz = blockIdx.y;
     const int nb = min(B, a.nx - x0);
-    const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(a.ay.tw);
+    const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(ay.tw);
     cplx<R>* tw = s1 + B * N;   // twiddle table staged in LDS
     for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
     cplx<R>* base = a.a + (int64_t)iz * a.nx * a.ny + x0;
@@ -539,7 +628,7 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int B, int sign) {
         return (b < nb) ? base[(int64_t)j * a.nx + b] : cplx<R>{0, 0};
     });
     __syncthreads();
-    cplx<R>* res = lds_fft<R, true>(s0, s1, B, a.ay, tw, sign);
+    cplx<R>* res = lds_fft<R, true, NC>(s0, s1, B, ay, tw, sign);
     lds_drain<STAGE_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
         if (b < nb) base[(int64_t)j * a.nx + b] = v;
@@ -554,10 +643,12 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int B, int sign) {
 #ifndef PUFF_Z_THREADS
 #define PUFF_Z_THREADS 256
 #endif
-template <typename R, bool INPLACE>
-__global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, int op, int sign) {
+template <typename R, bool INPLACE, int NC, int BC>
+__global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp, int op, int sign) {
     extern __shared__ unsigned char smem_raw[];
-    const int N = a.nzp;
+    const int N = NC ? NC : a.nzp;
+    const int B = BC ? BC : Bp;
+    const FftAxis& az = a.az;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
     cplx<R>* s1 = INPLACE ? s0 : s0 + B * N;
     // blocks of B x-columns, x fastest; neighbouring blocks share the cache
@@ -570,7 +661,7 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, 
     const int nb = min(B, a.nx - x0);
     const int64_t ntr = (int64_t)a.nx * a.ny;
     // few lines per workgroup here: the twiddles stay in global memory (L2)
-    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(a.az.tw);
+    const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(az.tw);
     cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
     lds_fill<1>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
@@ -579,9 +670,9 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, 
     __syncthreads();
     cplx<R>* res;
     if (op < 0) {
-        res = z_fft<R, INPLACE>(s0, s1, B, a.az, tw, sign);
+        res = z_fft<R, INPLACE, NC>(s0, s1, B, az, tw, sign);
     } else {
-        res = z_fft<R, INPLACE>(s0, s1, B, a.az, tw, -1);
+        res = z_fft<R, INPLACE, NC>(s0, s1, B, az, tw, -1);
         const double ky = a.ky[iy];
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w % B, q = w / B;
@@ -613,7 +704,7 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int B, 
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = z_fft<R, INPLACE>(res, other, B, a.az, tw, +1);
+        res = z_fft<R, INPLACE, NC>(res, other, B, az, tw, +1);
     }
     lds_drain<1>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
@@ -644,7 +735,7 @@ static void lds_opt_in(const void* kernel, size_t bytes) {
 #ifndef PUFF_RADER_THREADS
 #define PUFF_RADER_THREADS 256
 #endif
-static int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_LDS_BUDGET) {
+static constexpr int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_LDS_BUDGET) {
     const size_t tw = (size_t)n * elem;
     int b = budget > tw ? (int)((budget - tw) / (2 * (size_t)n * elem)) : 0;
     if (b < 1) b = 1;
@@ -652,64 +743,109 @@ static int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_
     return b;
 }
 
+// z2 lines per workgroup and whether the pass runs in place (one LDS
+// buffer): in place when every pass is radix 16, 4, 3 or 2 with at most one
+// butterfly per thread, and that leaves at least as many lines per workgroup
+// as two buffers do: C1's 1024-point lines in FP64 (616 -> 454 us per pass)
+// and C4's 4096-point lines (7.50 -> 6.92 ms).  Otherwise two line buffers,
+// no twiddle table (wider workgroups with longer runs along x measured
+// slower: occupancy matters more than coalescing here).
+struct ZPlan {
+    int B;
+    bool inplace;
+};
+static constexpr ZPlan z_plan(int n, size_t elem, bool rader, int npass, const int* radix) {
+    bool inplace = !rader;
+    int minr = 16;
+    for (int k = 0; k < npass; ++k) {
+        const int r = radix[k];
+        if (r != 2 && r != 3 && r != 4 && r != 16) inplace = false;
+        if (r < minr) minr = r;
+    }
+    int B = (int)((PUFF_Z_LDS_BUDGET) / (2 * (size_t)n * elem));
+    B = B < 1 ? 1 : (B > 16 ? 16 : B);   // lines per workgroup with two buffers
+    if (inplace) {
+        int Bi = (int)((PUFF_Z_LDS_BUDGET) / ((size_t)n * elem));
+        Bi = Bi > 16 ? 16 : Bi;
+        const int bt = PUFF_Z_THREADS / (n / minr);   // one butterfly per thread
+        Bi = Bi > bt ? bt : Bi;
+        inplace = Bi >= 1 && Bi >= B;
+        if (inplace) B = Bi;
+    }
+    return ZPlan{B, inplace};
+}
+
+template <typename R, int NC>
+static constexpr int ct_lines_x() { return lines_per_block(NC, sizeof(cplx<R>), 64); }
+template <typename R, int NC>
+static constexpr int ct_lines_y() { return lines_per_block(NC, sizeof(cplx<R>), 16); }
+template <typename R, int NC>
+static constexpr ZPlan ct_zplan() {
+    return z_plan(NC, sizeof(cplx<R>), CtPlan<NC>::rader, CtPlan<NC>::np, CtPlan<NC>::r);
+}
+
+template <typename R, int NC, int BC>
+static void launch_x(const DiffArgs<R>& a, int B, int op, int sign, hipStream_t s) {
+    const int64_t nlines = (int64_t)a.ny * a.nzp;
+    const size_t sh = (2 * (size_t)B + 1) * a.nx * sizeof(cplx<R>);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_x<R, NC, BC>), sh);
+    const int thr = a.ax.rader ? PUFF_RADER_THREADS : 256;
+    hipLaunchKernelGGL((k_fft_x<R, NC, BC>), dim3((unsigned)((nlines + B - 1) / B)), dim3(thr), sh, s, a, B,
+                       op, sign);
+}
+
 template <typename R>
 static void run_x(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
     const int B = lines_per_block(a.nx, sizeof(cplx<R>), 64);
-    const int64_t nlines = (int64_t)a.ny * a.nzp;
-    const size_t sh = (2 * (size_t)B + 1) * a.nx * sizeof(cplx<R>);
-    lds_opt_in(reinterpret_cast<const void*>(&k_fft_x<R>), sh);
-    const int thr = a.ax.rader ? PUFF_RADER_THREADS : 256;
-    hipLaunchKernelGGL((k_fft_x<R>), dim3((unsigned)((nlines + B - 1) / B)), dim3(thr), sh, s, a, B, op, sign);
+    if (a.nx == 129) return launch_x<R, 129, ct_lines_x<R, 129>()>(a, B, op, sign, s);
+    if (a.nx == 257) return launch_x<R, 257, ct_lines_x<R, 257>()>(a, B, op, sign, s);
+    if (a.nx == 33) return launch_x<R, 33, ct_lines_x<R, 33>()>(a, B, op, sign, s);
+    launch_x<R, 0, 0>(a, B, op, sign, s);
+}
+
+template <typename R, int NC, int BC>
+static void launch_y(const DiffArgs<R>& a, int B, int sign, hipStream_t s) {
+    const size_t sh = (2 * (size_t)B + 1) * a.ny * sizeof(cplx<R>);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_y<R, NC, BC>), sh);
+    const int thr = a.ay.rader ? PUFF_RADER_THREADS : 256;
+    hipLaunchKernelGGL((k_fft_y<R, NC, BC>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(thr),
+                       sh, s, a, B, sign);
 }
 
 template <typename R>
 static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
     const int B = lines_per_block(a.ny, sizeof(cplx<R>), 16);
-    const size_t sh = (2 * (size_t)B + 1) * a.ny * sizeof(cplx<R>);
-    lds_opt_in(reinterpret_cast<const void*>(&k_fft_y<R>), sh);
-    const int thr = a.ay.rader ? PUFF_RADER_THREADS : 256;
-    hipLaunchKernelGGL((k_fft_y<R>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(thr), sh, s,
-                       a, B, sign);
+    if (a.ny == 129) return launch_y<R, 129, ct_lines_y<R, 129>()>(a, B, sign, s);
+    if (a.ny == 257) return launch_y<R, 257, ct_lines_y<R, 257>()>(a, B, sign, s);
+    if (a.ny == 33) return launch_y<R, 33, ct_lines_y<R, 33>()>(a, B, sign, s);
+    launch_y<R, 0, 0>(a, B, sign, s);
+}
+
+template <typename R, bool IP, int NC, int BC>
+static void launch_z(const DiffArgs<R>& a, int B, int op, int sign, hipStream_t s) {
+    const size_t sh = (IP ? 1 : 2) * (size_t)B * a.nzp * sizeof(cplx<R>);
+    const int64_t blocks = xcd_grid((int64_t)((a.nx + B - 1) / B) * a.ny);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R, IP, NC, BC>), sh);
+    hipLaunchKernelGGL((k_fft_z<R, IP, NC, BC>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op,
+                       sign);
+}
+
+template <typename R, int NC>
+static bool launch_z_ct(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
+    if (a.nzp != NC) return false;
+    constexpr ZPlan zp = ct_zplan<R, NC>();
+    launch_z<R, zp.inplace, NC, zp.B>(a, zp.B, op, sign, s);
+    return true;
 }
 
 template <typename R>
 static void run_z(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
-    // In place (one LDS buffer) when every pass is radix 16, 4, 3 or 2 with
-    // at most one butterfly per thread, and that leaves at least as many
-    // lines per workgroup as two buffers do: C1's 1024-point lines in FP64
-    // (616 -> 454 us per pass) and C4's 4096-point lines (7.50 -> 6.92 ms).
-    // Otherwise two line buffers, no twiddle table (wider workgroups with
-    // longer runs along x measured slower: occupancy matters more than
-    // coalescing here).
-    const FftAxis& ax = a.az;
-    bool inplace = !ax.rader;
-    int minr = 16;
-    for (int k = 0; k < ax.npass; ++k) {
-        const int r = ax.radix[k];
-        if (r != 2 && r != 3 && r != 4 && r != 16) inplace = false;
-        if (r < minr) minr = r;
-    }
-    int B = (int)((PUFF_Z_LDS_BUDGET) / (2 * (size_t)a.nzp * sizeof(cplx<R>)));
-    B = B < 1 ? 1 : (B > 16 ? 16 : B);   // lines per workgroup with two buffers
-    if (inplace) {
-        int Bi = (int)((PUFF_Z_LDS_BUDGET) / ((size_t)a.nzp * sizeof(cplx<R>)));
-        Bi = Bi > 16 ? 16 : Bi;
-        const int bt = PUFF_Z_THREADS / (a.nzp / minr);   // one butterfly per thread
-        Bi = Bi > bt ? bt : Bi;
-        inplace = Bi >= 1 && Bi >= B;
-        if (inplace) B = Bi;
-    }
-    const size_t sh = (inplace ? 1 : 2) * (size_t)B * a.nzp * sizeof(cplx<R>);
-    const int64_t blocks = xcd_grid((int64_t)((a.nx + B - 1) / B) * a.ny);
-    if (inplace) {
-        lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R, true>), sh);
-        hipLaunchKernelGGL((k_fft_z<R, true>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op,
-                           sign);
-    } else {
-        lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R, false>), sh);
-        hipLaunchKernelGGL((k_fft_z<R, false>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op,
-                           sign);
-    }
+    if (launch_z_ct<R, 1024>(a, op, sign, s) || launch_z_ct<R, 4096>(a, op, sign, s) ||
+        launch_z_ct<R, 128>(a, op, sign, s) || launch_z_ct<R, 256>(a, op, sign, s))
+        return;
+    const ZPlan zp = z_plan(a.nzp, sizeof(cplx<R>), a.az.rader, a.az.npass, a.az.radix);
+    if (zp.inplace) launch_z<R, true, 0, 0>(a, zp.B, op, sign, s);
+    else launch_z<R, false, 0, 0>(a, zp.B, op, sign, s);
 }
 
 template <typename R>

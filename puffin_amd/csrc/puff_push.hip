@@ -90,9 +90,16 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     const bool ok = in && flag == 0;
     if (flag) atomicOr(a.flags, flag);
 
-    // ---- node box of this workgroup's cells and the step's active box
+    // ---- sources (getSource): s = chi / dV3 (1 + eta p2) p / Gamma
+    const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
+    const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
+
+    // ---- node box of this workgroup's cells and the step's active box (and,
+    // for the packed fixed-point tile, the workgroup's largest |s|)
     const int lane = threadIdx.x & 63;
-    const TileBox tb = workgroup_box<R>(m, ok, c, a.box);
+    constexpr bool PACKED = sizeof(R) == 4 && !PUFF_MIXED_TILE_FP64 && !DET;
+    const TileBox tb = workgroup_box<R, PACKED ? 7 : 6>(
+        m, ok, c, a.box, PACKED ? __float_as_int(fmaxf(fabsf((float)sr), fabsf((float)si))) : 0);
     const bool use_tile = ph.field_evolve && tile_fits<stage_tile_nodes<R>()>(m, tb);
     if (threadIdx.x == 0 && ph.field_evolve && tb.any && !use_tile) atomicAdd(a.flags + FLAG_WORD_FALLBACK, 1);
 
@@ -107,15 +114,19 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // either way.
     // The field tile shares the deposit tile's LDS: it is used only for boxes
     // of at most FIELD_TILE_NODES nodes, whose deposit needs the first
-    // 2 FIELD_TILE_NODES doubles, so it sits right after them.
-    // The deposit tile accumulates in FP64 in both forms.  An FP32 tile for
-    // the mixed form (ds_add_f32 in place of ds_add_f64) measured 3x slower
-    // per stage on the C1 beam.
-    // The deterministic deposit's tile holds DET_WORDS 8-byte integers per
-    // node instead of two doubles.
+    // WORDS x FIELD_TILE_NODES words, so it sits right after them.
+    // Deposit tiles: two doubles per node in the FP64 form; DET_WORDS 8-byte
+    // integers in its deterministic mode; one 8-byte packed fixed-point word
+    // in the mixed form (scatter_cells_packed: one ds_add_u64 per corner for
+    // both components).  Measured on the C1 beam, mixed form: an FP32 tile
+    // (ds_add_f32) ran each stage 3.1x slower than the FP64 tile, and a
+    // compensated FP32 tile (ds_add_rtn_f32 + TwoSum) 5.7x slower, with the
+    // same count of LDS instructions per wave but 6x the LDS-active and 24x
+    // the LDS-wait cycles of ds_add_f64; the packed tile cut the LDS-wait
+    // cycles per wave from 398 to 6 and the step by 3-4 %.
     constexpr int TILE = stage_tile_nodes<R>();
-    constexpr int WORDS = DET ? DET_WORDS : 2;
-    using D = double;
+    using D = typename std::conditional<PACKED, unsigned long long, double>::type;
+    constexpr int WORDS = DET ? DET_WORDS : (PACKED ? 1 : 2);
     static_assert(WORDS * FIELD_TILE_NODES * sizeof(D) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
                       WORDS * TILE * sizeof(D), "field tile inside the deposit tile");
     __shared__ D s_dep[WORDS * TILE];
@@ -166,8 +177,6 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
 
     // ---- deposit w_k s into dA_s (getSource)
     if (ph.field_evolve) {
-        const R sr = ok ? (R)((double)chi * ph.inv_dV3 * fac * prd) : (R)0;
-        const R si = ok ? (R)((double)chi * ph.inv_dV3 * fac * pid) : (R)0;
         if constexpr (DET) {
             // n |s| 2^F <= 2^62 keeps every node's integer sums in range
             const int F = *a.det.shift;
@@ -175,8 +184,15 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
             if (ok && (fabs((double)sr) > lim || fabs((double)si) > lim)) atomicOr(a.flags, FLAG_DET_RANGE);
             scatter_cells_det(m, tb, use_tile, ok, c, cr, (double)sr, (double)si, ldexp(1.0, F), a.det.acc,
                               reinterpret_cast<unsigned long long*>(s_dep));
+        } else if constexpr (PACKED) {
+            if (use_tile) {
+                scatter_cells_packed(m, tb, ok, c, cr, (float)sr, (float)si, packed_scale(tb.extra),
+                                     reinterpret_cast<cplx<float>*>(a.dAout), s_dep);
+            } else {
+                scatter_cells<R, float>(m, tb, false, ok, c, cr, sr, si, a.dAout, (float*)nullptr);
+            }
         } else {
-            scatter_cells<R>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout, s_dep);
+            scatter_cells<R, D>(m, tb, use_tile, ok, c, cr, sr, si, a.dAout, s_dep);
         }
     }
     if (!in) return;

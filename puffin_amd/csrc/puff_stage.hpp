@@ -127,32 +127,40 @@ struct TileBox {
     int tx, ty, tz;
     int hz;        // last cell plane
     bool any;
+    int extra;     // block maximum of workgroup_box's `extra` (NX = 7)
 };
 
-template <typename R>
+// NX = 7 also reduces `extra` (a non-negative int per lane) to its block
+// maximum, returned in TileBox::extra.
+template <typename R, int NX = 6>
 __device__ __forceinline__ TileBox workgroup_box(const MeshGeom& m, bool ok, const Cell<R>& c,
-                                                 Box* box) {
-    __shared__ int s_red[6][MAX_BOX_WAVES];
-    __shared__ int s_bb[6];
+                                                 Box* box, int extra = 0) {
+    __shared__ int s_red[NX][MAX_BOX_WAVES];
+    __shared__ int s_bb[NX];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     // max values are reduced as min of their negatives
-    int v[6] = {ok ? c.ix : 0x7fffffff, ok ? -c.ix : 1,
-                ok ? c.iy : 0x7fffffff, ok ? -c.iy : 1,
-                ok ? c.iz0 : 0x7fffffff, ok ? -c.iz0 : 1};
+    int v[NX];
+    v[0] = ok ? c.ix : 0x7fffffff;
+    v[1] = ok ? -c.ix : 1;
+    v[2] = ok ? c.iy : 0x7fffffff;
+    v[3] = ok ? -c.iy : 1;
+    v[4] = ok ? c.iz0 : 0x7fffffff;
+    v[5] = ok ? -c.iz0 : 1;
+    if constexpr (NX > 6) v[6] = -extra;
 #pragma unroll
-    for (int d = 0; d < 6; ++d) {
+    for (int d = 0; d < NX; ++d) {
         v[d] = wave_min(v[d]);
         if (lane == 0) s_red[d][wv] = v[d];
     }
     __syncthreads();
-    if (threadIdx.x < 6) {
+    if (threadIdx.x < NX) {
         const int d = threadIdx.x;
         int r = s_red[d][0];
         for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = min(r, s_red[d][k]);
         s_bb[d] = (d & 1) ? -r : r;
         const bool empty = (d & 1) ? (r == 1) : (r == 0x7fffffff);
-        if (!empty) {
+        if (d < 6 && !empty) {
             Box* b = box + (blockIdx.x % BOX_COPIES);
             if (d & 1) atomicMax(&b->hi[d >> 1], -r);
             else atomicMin(&b->lo[d >> 1], r);
@@ -160,6 +168,8 @@ __device__ __forceinline__ TileBox workgroup_box(const MeshGeom& m, bool ok, con
     }
     __syncthreads();
     TileBox t;
+    t.extra = 0;
+    if constexpr (NX > 6) t.extra = -s_bb[NX - 1];
     t.any = s_bb[1] >= 0;
     t.bx0 = s_bb[0];
     t.by0 = s_bb[2];
@@ -335,6 +345,63 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
                     atomic_add(&out[node].y, vi);
                 }
             }
+        }
+    }
+}
+
+// ---- packed fixed-point tile (mixed form) -------------------------------------
+
+// One LDS atomic per corner for both components: each contribution becomes
+// the integers qr = rint(w sr S), qi = rint(w si S) with the workgroup's
+// power-of-two scale S, and qr 2^32 + qi is added to one 64-bit word of the
+// tile.  S keeps every node's sums below 2^30 in magnitude (256 lanes, each
+// adding at most the workgroup's largest |s| to a node), so the low half
+// never carries into the high half beyond what its sign extension undoes:
+// the word is sum qr 2^32 + sum qi exactly, and sum qi is its low 32 bits
+// read as signed.  Integer sums are exact (and independent of the atomics'
+// order); each contribution is rounded once, to 2^-31 of the largest node
+// sum the scale allows.  The flush divides by S and adds in FP32.
+__device__ __forceinline__ float packed_scale(int max_bits) {
+    const float mx = __int_as_float(max_bits);
+    if (!(mx > 0.0f)) return 1.0f;
+    int e = 0;
+    frexpf(mx, &e);                   // mx < 2^e
+    return ldexpf(1.0f, 30 - 8 - e);  // 256 mx S < 2^30 (STAGE_THREADS <= 256)
+}
+
+__device__ __forceinline__ void scatter_cells_packed(const MeshGeom& m, const TileBox& t, bool ok,
+                                                     const Cell<float>& c, const CellRef& cr, float sr,
+                                                     float si, float S, cplx<float>* out,
+                                                     unsigned long long* s_tile) {
+    const int lane = threadIdx.x & 63;
+    const int nnb = m.one_d ? 2 : 8;
+    const int nt = t.tx * t.ty * t.tz;
+    for (int q = threadIdx.x; q < nt; q += blockDim.x) s_tile[q] = 0ull;
+    __syncthreads();
+    if (ok) {
+        const float ssr = sr * S, ssi = si * S;
+        for (int q = 0; q < nnb; ++q) {
+            const int k = (q + lane) & (nnb - 1);
+            const float wk = corner_weight<float>(m, c, k);
+            const long long qr = (long long)(int)rintf(wk * ssr);
+            const long long qi = (long long)(int)rintf(wk * ssi);
+            atomicAdd(&s_tile[corner_tile(m, t, cr, k)], (unsigned long long)(qr * 4294967296LL + qi));
+        }
+    }
+    __syncthreads();
+    const int ntr = m.nx * m.ny;
+    const BoxDiv bd = box_div(t);
+    const float inv = 1.0f / S;
+    for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+        const long long w = (long long)s_tile[q];
+        if (w != 0) {
+            const long long lo = (long long)(int)(unsigned int)(w & 0xffffffffLL);
+            const long long hi = (w - lo) / 4294967296LL;
+            int lx, ly, lz;
+            box_coords(bd, q, lx, ly, lz);
+            const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
+            if (hi) atomic_add(&out[node].x, (float)hi * inv);
+            if (lo) atomic_add(&out[node].y, (float)lo * inv);
         }
     }
 }

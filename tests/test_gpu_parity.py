@@ -73,10 +73,16 @@ def test_f1main_golden_vectors_on_gpu():
 # ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("shape", [(9, 7, 8), (33, 33, 128), (129, 5, 16), (5, 6, 257), (11, 4, 43),
-                                   (17, 17, 17), (257, 3, 4), (3, 257, 4), (85, 7, 6)])
+                                   (17, 17, 17), (257, 3, 4), (3, 257, 4), (85, 7, 6),
+                                   # the line lengths with compile-time plans (x, y and z2);
+                                   # FP64 lines of 4096 points exceed a workgroup's LDS
+                                   (129, 129, 4), (33, 33, 256), (3, 5, 1024), (2, 3, 4096),
+                                   (257, 257, 4)])
 @pytest.mark.parametrize("precision", [FP64, MIXED])
 def test_transform_matches_numpy(shape, precision):
     nx, ny, nz = shape
+    if precision == FP64 and nz == 4096:
+        pytest.skip("FP64 lines of 4096 points do not fit a workgroup's LDS (refused by puff_create)")
     print(f"[gpu] transform shape={shape} precision={precision}", flush=True)
     mp = ModuleParams(nx=nx, ny=ny, nz2=nz, dx=0.1, dy=0.1, dz2=0.01, rho=0.01, eta=1e-4,
                       kappa=0.5, gamma_r=100.0, aw=1.0)
