@@ -507,6 +507,86 @@ def c4_full_module(precision: int = 1):
     return out
 
 
+def c4_full_gradient(fd_rel: float = 1e-3, reserve_gb: float = 48.0):
+    """d(integrated power)/d(per-period taper table, seed amplitude) of the
+    whole 128-period module of the large workload (3840 steps, 2^27
+    macroparticles, 257 x 257 x 4096, FP32-mixed) with binomial
+    checkpointing over as many device snapshots as fit beside the context and
+    the adjoint buffers.  Times a plain forward, the gradient, and two more
+    forwards for a central difference on the seed amplitude; the device
+    memory in use is read while every snapshot and adjoint buffer is live."""
+    import torch
+    from puffin_amd import gpu
+    from puffin_amd.adjoint import BinomialGradient, checkpoint_schedule, integrated_power as ipow, module_program
+    from puffin_amd.configs import case_c4
+    from puffin_amd.undulator import und_section
+    c = case_c4()
+    mp = c.mp
+    seed = c.info["seed_amp"]
+    shape = (c.field[0] / seed, c.field[1] / seed)
+    ops = module_program(c.nsteps, c.h, c.steps4diff, diffraction=True, periodic=False)
+    drv = gpu.Integrator(mp, c.n, 1)
+    out = dict(workload="planar FEL, temporal mesh 257x257x4096, 2^27 macroparticles, FP32-mixed, "
+                        "whole 128-period module (3840 steps)")
+    try:
+        drv.upload_particles(c.particles)
+        drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+
+        def forward(s):
+            drv.upload_particles(c.particles)
+            drv.upload_field(c.field[0] * (s / seed), c.field[1] * (s / seed))
+            drv.set_taper_table(c.taper_table, 0.0, c.taper_period)
+            drv.synchronize()
+            t0 = time.perf_counter()
+            res = und_section(drv, c.nsteps, c.h, diffraction=True, steps4diff=c.steps4diff)
+            drv.synchronize()
+            if res.status != 0:
+                raise RuntimeError(f"c4 forward status {res.status}")
+            return ipow(mp, drv.power()), time.perf_counter() - t0
+
+        J_fwd, t_fwd = forward(seed)
+        log(f"c4 gradient: plain forward {t_fwd:.1f} s, J = {J_fwd:.6e}")
+        drv.upload_particles(c.particles)
+        blob = drv.state_bytes()
+        free, total = torch.cuda.mem_get_info()
+        shape_bytes = 4 * mp.nmesh * 8   # unit seed shape and its scaled copy, FP64 device tensors
+        slots = int((free - shape_bytes - reserve_gb * 1e9) // blob) - 1
+        slots = max(2, min(slots, 40))
+        F, _ = checkpoint_schedule(3840, slots - 1)
+        log(f"c4 gradient: {slots} snapshot slots of {blob / 1e9:.2f} GB, schedule {F[3840, slots - 1]} "
+            f"forward steps for 3840")
+        t0 = time.perf_counter()
+
+        def progress(st):
+            log(f"c4 gradient: {st['forward_steps']} forward, {st['adjoint_steps']} adjoint steps, "
+                f"{time.perf_counter() - t0:.0f} s")
+
+        eng = BinomialGradient(drv, ops, shape, slots=slots, taper_period=c.taper_period, progress=progress,
+                               progress_every=480)
+        t0 = time.perf_counter()
+        g = eng.gradient(c.taper_table, seed)
+        t_grad = time.perf_counter() - t0
+        log(f"c4 gradient: {t_grad:.1f} s ({t_grad / t_fwd:.2f}x the forward), peak device "
+            f"{eng.peak_device_bytes / 2**30:.1f} GiB")
+        out.update(forward_s=t_fwd, J_forward=J_fwd, J_gradient_run=g["J"], gradient_s=t_grad,
+                   gradient_cost_vs_forward=t_grad / t_fwd, snapshot_slots=slots, snapshot_bytes=blob,
+                   schedule_forward_steps=int(F[3840, slots - 1]), stats=g["stats"],
+                   peak_device_bytes=eng.peak_device_bytes, device_total_bytes=int(total),
+                   grad_seed=g["seed"], grad_table_norm=float(np.linalg.norm(g["table"])),
+                   grad_table_first=float(g["table"][0]), grad_table_last=float(g["table"][-1]))
+        del eng
+        torch.cuda.empty_cache()
+        Jp, _ = forward(seed * (1 + fd_rel))
+        Jm, _ = forward(seed * (1 - fd_rel))
+        fd = (Jp - Jm) / (2 * fd_rel * seed)
+        out["fd_check"] = dict(rel_step=fd_rel, fd_seed=fd, seed_rel_err=abs(fd - g["seed"]) / abs(g["seed"]))
+        log(f"c4 gradient: dJ/dseed {g['seed']:.6e}, central difference {fd:.6e} "
+            f"(rel err {out['fd_check']['seed_rel_err']:.2e})")
+    finally:
+        drv.close()
+    return out
+
+
 def copy_rate(mib: int = 1024, reps: int = 10) -> float:
     """Achievable streaming rate (TB/s): a device-to-device copy of mib MiB,
     read plus write bytes over the time, timed with events on torch's
@@ -567,6 +647,8 @@ def main():
                     help="only the large FP32-mixed workload (its JSON object on stdout)")
     ap.add_argument("--c4-module", action="store_true",
                     help="only the whole 3840-step module of the large workload (JSON on stdout)")
+    ap.add_argument("--c4-gradient", action="store_true",
+                    help="only the gradient of the whole large module (binomial checkpointing; JSON on stdout)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -579,6 +661,9 @@ def main():
         return
     if args.c4_module:
         print(json.dumps(c4_full_module()), flush=True)
+        return
+    if args.c4_gradient:
+        print(json.dumps(c4_full_gradient()), flush=True)
         return
     if args.diffraction_only:
         print(json.dumps(diffraction_bench()), flush=True)

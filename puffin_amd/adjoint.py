@@ -299,6 +299,212 @@ class ModuleGradient:
         return {"table": g_table, "seed": g_seed, "field": (lar, lai)}
 
 
+
+def checkpoint_schedule(n: int, s: int):
+    """Optimal snapshot placement for reversing n steps with s free snapshot
+    slots besides the one holding the start state (the recursion of
+    Griewank and Walther's binomial checkpointing, "Algorithm 799: revolve",
+    ACM TOMS 26 (2000) 19-45): advance m steps, snapshot, reverse the last
+    n - m steps with s - 1 slots, then the first m with s.  Its forward
+    steps obey F(k, 0) = k (k - 1) / 2 (replays from the start for each step)
+    and F(k, j) = min_m [m + F(k - m, j - 1) + F(m, j)], solved here by
+    dynamic programming.  Returns (F, M): F[k, j] forward steps and M[k, j]
+    the best m, for k <= n, j <= s."""
+    F = np.zeros((n + 1, s + 1), dtype=np.int64)
+    M = np.ones((n + 1, s + 1), dtype=np.int64)
+    k = np.arange(n + 1, dtype=np.int64)
+    F[:, 0] = k * (k - 1) // 2
+    for j in range(1, s + 1):
+        for kk in range(2, n + 1):
+            m = np.arange(1, kk)
+            c = m + F[kk - m, j - 1] + F[m, j]
+            i = int(np.argmin(c))
+            F[kk, j] = c[i]
+            M[kk, j] = m[i]
+    return F, M
+
+
+class BinomialGradient:
+    """Gradient of one module's integrated power with binomial checkpointing
+    over ``slots`` device snapshots, for modules whose every-step captures do
+    not fit the device (C4: 3840 steps, 7.6 GB per snapshot, 27 GB per step
+    capture).  A unit is one RK4 step and the operations that follow it
+    before the next step (diffraction, wrap); the module's leading operations
+    (its first half diffraction) come before unit 0.  The reverse sweep of
+    Griewank and Walther's recursion restores a snapshot, advances to the
+    next one, and reverses the units after it first; each adjoint step
+    recomputes its forward stages from the step's start state
+    (puff_rk4par_adjoint), so no capture is kept.  The first advance doubles
+    as the forward pass: J and the cotangent seed come from the state after
+    the last unit, reached once on the way.  The snapshot positions follow
+    checkpoint_schedule, which minimises the forward steps."""
+
+    def __init__(self, drv: Integrator, ops: list[Op], field_shape: tuple, *, slots: int,
+                 taper_period: float | None = None, taper_z0: float = 0.0, progress=None,
+                 progress_every: int = 240, device=None):
+        if torch is None:
+            raise RuntimeError("the gradient engine keeps its snapshots in torch device memory")
+        if slots < 2:
+            raise ValueError("binomial checkpointing needs at least two snapshot slots")
+        self.drv = drv
+        self.mp = drv.mp
+        self.taper_period = taper_period if taper_period else 4.0 * PI_REF * self.mp.rho
+        self.taper_z0 = taper_z0
+        self.wz = power_weights(self.mp)
+        first = next(i for i, op in enumerate(ops) if op.kind == "rk4")
+        self.lead = ops[:first]
+        self.units = []
+        for op in ops[first:]:
+            if op.kind == "rk4":
+                self.units.append([op])
+            else:
+                self.units[-1].append(op)
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.dev = dev
+        self.shape_r = torch.as_tensor(np.ascontiguousarray(field_shape[0], dtype=np.float64).reshape(-1),
+                                       device=dev)
+        self.shape_i = torch.as_tensor(np.ascontiguousarray(field_shape[1], dtype=np.float64).reshape(-1),
+                                       device=dev)
+        self._seed_r = torch.empty_like(self.shape_r)
+        self._seed_i = torch.empty_like(self.shape_i)
+        nb = drv.state_bytes()
+        self.initial = torch.empty(nb, dtype=torch.uint8, device=dev)
+        self.initial_hdr = np.zeros(STATE_HEADER_BYTES, dtype=np.uint8)
+        drv.state_save(self.initial.data_ptr(), self.initial_hdr)
+        self.blobs = [torch.empty(nb, dtype=torch.uint8, device=dev) for _ in range(slots)]
+        self.hdrs = [np.zeros(STATE_HEADER_BYTES, dtype=np.uint8) for _ in range(slots)]
+        self.progress = progress
+        self.progress_every = progress_every
+        self.stats = dict(forward_steps=0, adjoint_steps=0, restores=0, snapshots=0)
+        self.sched_F, self.sched_M = checkpoint_schedule(len(self.units), slots - 1)
+        self.peak_device_bytes = 0
+
+    def memory_bytes(self) -> int:
+        return (1 + len(self.blobs)) * self.initial.numel()
+
+    # ---- primitives
+    def _run(self, op: Op):
+        if op.kind == "rk4":
+            self.drv.rk4par(op.sz, op.h)
+            self.stats["forward_steps"] += 1
+        elif op.kind == "diff":
+            self.drv.diffraction_step(op.h)
+        else:
+            self.drv.wrap_z2()
+
+    def _advance(self, lo: int, m: int):
+        for j in range(lo, lo + m):
+            for op in self.units[j]:
+                self._run(op)
+            self._tick()
+
+    def _save(self, slot: int):
+        self.drv.state_save(self.blobs[slot].data_ptr(), self.hdrs[slot])
+        self.stats["snapshots"] += 1
+
+    def _load(self, slot: int):
+        self.drv.state_load(self.blobs[slot].data_ptr(), self.hdrs[slot])
+        self.stats["restores"] += 1
+
+    def _tick(self):
+        done = self.stats["forward_steps"] + self.stats["adjoint_steps"]
+        if self.progress and done % self.progress_every == 0:
+            st = self.drv.deferred_status()
+            if st != 0:
+                raise RuntimeError(f"a step raised status {st}")
+            self.progress(dict(self.stats))
+
+    def _seed_at_end(self, slot: int):
+        """The state is at the start of the last unit (held in `slot`): run it,
+        take J and the cotangent of the final state, restore the start."""
+        for op in self.units[-1]:
+            self._run(op)
+        st = self.drv.deferred_status()
+        if st != 0:
+            raise RuntimeError(f"forward: a step raised status {st}")
+        self.J = integrated_power(self.mp, self.drv.power())
+        self.drv.adjoint_enable()
+        self.drv.adjoint_zero()
+        self.drv.adjoint_seed_power(self.wz, self.scale)
+        if self.dev.type == "cuda":
+            # every buffer of the gradient is allocated from here on
+            free, total = torch.cuda.mem_get_info(self.dev)
+            self.peak_device_bytes = max(self.peak_device_bytes, int(total - free))
+        self._load(slot)
+
+    def _adjoint_unit(self, j: int):
+        unit = self.units[j]
+        for op in reversed(unit[1:]):
+            if op.kind == "diff":
+                self.drv.diffraction_adjoint(op.h)
+        self.drv.rk4par_adjoint(unit[0].sz, unit[0].h)
+        self.stats["adjoint_steps"] += 1
+        self._tick()
+
+    def _rev(self, lo: int, n: int, base: int, free: list):
+        """Reverse units lo .. lo + n - 1; the state at unit lo is held in slot
+        `base`; `free` lists the unused slots."""
+        if n == 1:
+            self._load(base)
+            if lo == len(self.units) - 1:
+                self._seed_at_end(base)
+            self._adjoint_unit(lo)
+            return
+        if not free:
+            # no slot left: replay from the base for each unit (quadratic)
+            for j in range(n - 1, -1, -1):
+                self._load(base)
+                self._advance(lo, j)
+                if lo + j == len(self.units) - 1:
+                    for op in self.units[-1]:
+                        self._run(op)
+                    self.J = integrated_power(self.mp, self.drv.power())
+                    self.drv.adjoint_enable()
+                    self.drv.adjoint_zero()
+                    self.drv.adjoint_seed_power(self.wz, self.scale)
+                    self._load(base)
+                    self._advance(lo, j)
+                self._adjoint_unit(lo + j)
+            return
+        m = int(self.sched_M[n, len(free)])
+        self._load(base)
+        self._advance(lo, m)
+        slot = free.pop()
+        self._save(slot)
+        self._rev(lo + m, n - m, slot, free)
+        free.append(slot)
+        self._rev(lo, m, base, free)
+
+    # ---- the gradient
+    def gradient(self, table, seed: float, scale: float = 1.0) -> dict:
+        """J and dJ/d(table), dJ/d(seed amplitude) (times ``scale``)."""
+        self.scale = scale
+        self.table = np.ascontiguousarray(table, dtype=np.float64)
+        drv = self.drv
+        drv.state_load(self.initial.data_ptr(), self.initial_hdr)
+        torch.mul(self.shape_r, float(seed), out=self._seed_r)
+        torch.mul(self.shape_i, float(seed), out=self._seed_i)
+        drv.upload_field_torch(self._seed_r, self._seed_i)
+        drv.set_taper_table(self.table, self.taper_z0, self.taper_period)
+        drv.set_deferred(True)
+        try:
+            for op in self.lead:
+                self._run(op)
+            self._save(0)
+            self._rev(0, len(self.units), 0, list(range(1, len(self.blobs))))
+            for op in reversed(self.lead):
+                if op.kind == "diff":
+                    drv.diffraction_adjoint(op.h)
+            st = drv.deferred_status()
+            if st != 0:
+                raise RuntimeError(f"adjoint: a step raised status {st}")
+            g_table = drv.adjoint_taper_gradient(self.table.size + 1)[: self.table.size]
+        finally:
+            drv.set_deferred(False)
+        lar, lai = drv.adjoint_download_field_torch()
+        g_seed = float(torch.dot(lar, self.shape_r) + torch.dot(lai, self.shape_i))
+        return {"J": self.J, "table": g_table, "seed": g_seed, "stats": dict(self.stats)}
+
 if torch is not None:
 
     class IntegratedPower(torch.autograd.Function):

@@ -21,6 +21,13 @@
 
 namespace puff {
 
+// Threads per z2 workgroup.  512 and 1024 (more loads in flight per line
+// at the same LDS) measured 6 % and 30 % slower on the C1 mesh in FP64;
+// 512 gained 2 % on the C4 mesh.
+#ifndef PUFF_Z_THREADS
+#define PUFF_Z_THREADS 256
+#endif
+
 // LDS index of element j of line b.  Interleaved layout (b fastest) keeps
 // the strided y / z2 loads conflict free; line-major suits contiguous x lines.
 template <bool IL>
@@ -36,6 +43,9 @@ __device__ __forceinline__ int lidx(int b, int j, int B, int N) {
 // z2 passes, whose elements lie NX NY apart, ran 4-5 % slower with it and
 // keep U = 1.
 constexpr int STAGE_UNROLL = 8;
+#ifndef PUFF_Z_UNROLL
+#define PUFF_Z_UNROLL 1
+#endif
 
 // dst[w] = get(w) for 0 <= w < n.
 template <int U, typename R, typename F>
@@ -403,50 +413,61 @@ __device__ __forceinline__ void small_radix(cplx<R>* v, int r, int sign) {
 // into registers, the workgroup passes a barrier, and the outputs overwrite
 // the same buffer.  Half the LDS of the ping-pong form, hence more
 // workgroups per CU.
-template <typename R, bool IL>
+template <typename R, bool IL, int KB = 1>
 __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, int p, const cplx<R>* tw,
                                              int sign) {
+    // KB butterflies per thread: all their inputs are read into registers
+    // before the barrier, all outputs written after it
     const int T = N / r;
     const int twst = N / (p * r);   // j k twst < N, as in lds_pass
-    const int w = threadIdx.x;
-    const bool act = w < B * T;
-    int b = 0, ob = 0;
-    cplx<R> v[16];
-    if (act) {
-        int i;
-        if (IL) { b = w % B; i = w / B; } else { i = w % T; b = w / T; }
-        const int k = i % p;
-        ob = (i / p) * p * r + k;
-        if (r == 16) {
+    int b[KB], ob[KB];
+    bool act[KB];
+    cplx<R> v[KB][16];
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
-                if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
-            }
-            dft16(v, sign);
-        } else {
+    for (int it = 0; it < KB; ++it) {
+        const int w = threadIdx.x + it * (int)blockDim.x;
+        act[it] = w < B * T;
+        b[it] = 0;
+        ob[it] = 0;
+        if (act[it]) {
+            int i;
+            if (IL) { b[it] = w % B; i = w / B; } else { i = w % T; b[it] = w / T; }
+            const int k = i % p;
+            ob[it] = (i / p) * p * r + k;
+            if (r == 16) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j < r) {
-                    v[j] = buf[lidx<IL>(b, i + j * T, B, N)];
-                    if (j > 0 && k > 0) v[j] = cmul(v[j], tw_get(tw, j * k * twst, sign));
+                for (int j = 0; j < 16; ++j) {
+                    v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, N)];
+                    if (j > 0 && k > 0) v[it][j] = cmul(v[it][j], tw_get(tw, j * k * twst, sign));
                 }
+                dft16(v[it], sign);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (j < r) {
+                        v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, N)];
+                        if (j > 0 && k > 0) v[it][j] = cmul(v[it][j], tw_get(tw, j * k * twst, sign));
+                    }
+                }
+                small_radix(v[it], r, sign);
             }
-            small_radix(v, r, sign);
         }
     }
     __syncthreads();
-    if (act) {
-        if (r == 16) {
 #pragma unroll
-            for (int q0 = 0; q0 < 4; ++q0)
+    for (int it = 0; it < KB; ++it) {
+        if (act[it]) {
+            if (r == 16) {
 #pragma unroll
-                for (int q1 = 0; q1 < 4; ++q1)
-                    buf[lidx<IL>(b, ob + (q0 + 4 * q1) * p, B, N)] = v[4 * q0 + q1];
-        } else {
+                for (int q0 = 0; q0 < 4; ++q0)
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (q < r) buf[lidx<IL>(b, ob + q * p, B, N)] = v[q];
+                    for (int q1 = 0; q1 < 4; ++q1)
+                        buf[lidx<IL>(b[it], ob[it] + (q0 + 4 * q1) * p, B, N)] = v[it][4 * q0 + q1];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (q < r) buf[lidx<IL>(b[it], ob[it] + q * p, B, N)] = v[it][q];
+            }
         }
     }
 }
@@ -463,13 +484,19 @@ __device__ __forceinline__ void lds_passes_inplace(cplx<R>* buf, int B, int N, i
     }
 }
 
-template <typename R, bool IL, int NC, int K = 0, int P = 1>
-__device__ __forceinline__ void ct_passes_inplace(cplx<R>* buf, int B, int N, const cplx<R>* tw, int sign) {
+// Butterflies per thread of an in-place pass of radix r over B lines of N.
+constexpr int inplace_kb(int B, int N, int r, int threads) {
+    return (B * (N / r) + threads - 1) / threads;
+}
+
+template <typename R, bool IL, int NC, int BC, int K = 0, int P = 1>
+__device__ __forceinline__ void ct_passes_inplace(cplx<R>* buf, const cplx<R>* tw, int sign) {
     if constexpr (K < CtPlan<NC>::np) {
         constexpr int r = CtPlan<NC>::r[K];
-        inplace_pass<R, IL>(buf, B, N, r, P, tw, sign);
+        constexpr int KB = inplace_kb(BC, NC, r, PUFF_Z_THREADS);
+        inplace_pass<R, IL, KB>(buf, BC, NC, r, P, tw, sign);
         __syncthreads();
-        ct_passes_inplace<R, IL, NC, K + 1, P * r>(buf, B, N, tw, sign);
+        ct_passes_inplace<R, IL, NC, BC, K + 1, P * r>(buf, tw, sign);
     }
 }
 
@@ -486,11 +513,11 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, c
 }
 
 // z2-line transform: in place (one LDS buffer) or ping-pong.
-template <typename R, bool INPLACE, int NC>
+template <typename R, bool INPLACE, int NC, int BC>
 __device__ __forceinline__ cplx<R>* z_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
                                           const cplx<R>* tw, int sign) {
     if constexpr (INPLACE) {
-        if constexpr (NC > 0) ct_passes_inplace<R, true, NC>(src, B, NC, tw, sign);
+        if constexpr (NC > 0) ct_passes_inplace<R, true, NC, BC>(src, tw, sign);
         else lds_passes_inplace<R, true>(src, B, ax.n, ax.npass, ax.radix, tw, sign);
         return src;
     } else {
@@ -637,12 +664,6 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) 
 
 // ---- z2 lines (stride nx ny), B consecutive x per block ----------------------
 // op < 0: plain transform with `sign`; else forward, spectral op, inverse.
-// Threads per z2 workgroup.  512 and 1024 (more loads in flight per line
-// at the same LDS) measured 6 % and 30 % slower on the C1 mesh in FP64;
-// 512 gained 2 % on the C4 mesh.
-#ifndef PUFF_Z_THREADS
-#define PUFF_Z_THREADS 256
-#endif
 template <typename R, bool INPLACE, int NC, int BC>
 __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp, int op, int sign) {
     extern __shared__ unsigned char smem_raw[];
@@ -663,16 +684,16 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
     // few lines per workgroup here: the twiddles stay in global memory (L2)
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(az.tw);
     cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
-    lds_fill<1>(s0, B * N, [&](int w) {
+    lds_fill<PUFF_Z_UNROLL>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
         return (b < nb) ? base[(int64_t)j * ntr + b] : cplx<R>{0, 0};
     });
     __syncthreads();
     cplx<R>* res;
     if (op < 0) {
-        res = z_fft<R, INPLACE, NC>(s0, s1, B, az, tw, sign);
+        res = z_fft<R, INPLACE, NC, BC>(s0, s1, B, az, tw, sign);
     } else {
-        res = z_fft<R, INPLACE, NC>(s0, s1, B, az, tw, -1);
+        res = z_fft<R, INPLACE, NC, BC>(s0, s1, B, az, tw, -1);
         const double ky = a.ky[iy];
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w % B, q = w / B;
@@ -704,9 +725,9 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = z_fft<R, INPLACE, NC>(res, other, B, az, tw, +1);
+        res = z_fft<R, INPLACE, NC, BC>(res, other, B, az, tw, +1);
     }
-    lds_drain<1>(res, B * N, [&](int w, const cplx<R>& v) {
+    lds_drain<PUFF_Z_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
         if (b < nb) base[(int64_t)j * ntr + b] = v;
     });
@@ -779,9 +800,34 @@ template <typename R, int NC>
 static constexpr int ct_lines_x() { return lines_per_block(NC, sizeof(cplx<R>), 64); }
 template <typename R, int NC>
 static constexpr int ct_lines_y() { return lines_per_block(NC, sizeof(cplx<R>), 16); }
+// Compile-time plans may give a thread several butterflies of an in-place
+// pass (inplace_pass<KB>), up to Z_MAX_HELD complex values per thread
+// (64 VGPRs), so their lines per workgroup follow the LDS budget alone.
+template <typename R>
+constexpr int z_max_held() { return sizeof(R) == 8 ? 16 : 32; }
 template <typename R, int NC>
 static constexpr ZPlan ct_zplan() {
-    return z_plan(NC, sizeof(cplx<R>), CtPlan<NC>::rader, CtPlan<NC>::np, CtPlan<NC>::r);
+    bool inplace = !CtPlan<NC>::rader;
+    for (int k = 0; k < CtPlan<NC>::np; ++k) {
+        const int r = CtPlan<NC>::r[k];
+        if (r != 2 && r != 3 && r != 4 && r != 16) inplace = false;
+    }
+    const size_t elem = sizeof(cplx<R>);
+    int B2 = (int)((PUFF_Z_LDS_BUDGET) / (2 * (size_t)NC * elem));
+    B2 = B2 < 1 ? 1 : (B2 > 16 ? 16 : B2);
+    if (!inplace) return ZPlan{B2, false};
+    int Bi = (int)((PUFF_Z_LDS_BUDGET) / ((size_t)NC * elem));
+    Bi = Bi > 16 ? 16 : Bi;
+    for (; Bi > 1; --Bi) {
+        bool fits = true;
+        for (int k = 0; k < CtPlan<NC>::np; ++k) {
+            const int r = CtPlan<NC>::r[k];
+            if (inplace_kb(Bi, NC, r, PUFF_Z_THREADS) * r > z_max_held<R>()) fits = false;
+        }
+        if (fits) break;
+    }
+    if (Bi >= 1 && Bi >= B2) return ZPlan{Bi, true};
+    return ZPlan{B2, false};
 }
 
 template <typename R, int NC, int BC>

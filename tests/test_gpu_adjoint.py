@@ -220,3 +220,28 @@ def test_integrated_power_autograd_function():
     assert abs(float(J.detach()) - ref) <= 1e-12 * abs(ref)
     assert np.allclose(table.grad.numpy(), g["table"], rtol=1e-9, atol=0)
     assert abs(float(seed.grad) - g["seed"]) <= 1e-9 * abs(g["seed"])
+
+
+@pytest.mark.parametrize("slots", [3, 6])
+@pytest.mark.parametrize("periodic", [1, 0])
+def test_binomial_checkpointing_gradient_matches_oracle(periodic, slots):
+    """BinomialGradient (few snapshot slots, adjoint steps that recompute their
+    forward stages) against the oracle's module gradient."""
+    print(f"[gpu] binomial gradient periodic={periodic} slots={slots}", flush=True)
+    gpu = _gpu()
+    from puffin_amd.adjoint import BinomialGradient
+    c = small_module(periodic=periodic)
+    J0, gt0, gs0, _ = OracleModule(c).gradient(c["table"], c["seed"])
+    mp = c["mp"]
+    drv = gpu.Integrator(mp, c["parts"]["x"].size, FP64)
+    try:
+        drv.upload_particles(c["parts"])
+        drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+        eng = BinomialGradient(drv, c["ops"], c["shape"], slots=slots, taper_period=c["period"])
+        g = eng.gradient(c["table"], c["seed"])
+    finally:
+        drv.close()
+    assert abs(g["J"] - J0) <= 1e-10 * abs(J0)
+    assert np.abs(g["table"] - gt0).max() <= 1e-7 * np.abs(gt0).max()
+    assert abs(g["seed"] - gs0) <= 1e-7 * abs(gs0)
+    assert g["stats"]["adjoint_steps"] == sum(1 for op in c["ops"] if op.kind == "rk4")
