@@ -414,9 +414,15 @@ class BinomialGradient:
                 raise RuntimeError(f"a step raised status {st}")
             self.progress(dict(self.stats))
 
-    def _seed_at_end(self, slot: int):
-        """The state is at the start of the last unit (held in `slot`): run it,
-        take J and the cotangent of the final state, restore the start."""
+    def _note_memory(self):
+        if self.dev.type == "cuda":
+            free, total = torch.cuda.mem_get_info(self.dev)
+            self.peak_device_bytes = max(self.peak_device_bytes, int(total - free))
+
+    def _seed(self):
+        """The state is at the start of the last unit: run it, take J and the
+        cotangent of the final state (every buffer of the gradient is
+        allocated from here on: the device memory in use is read here)."""
         for op in self.units[-1]:
             self._run(op)
         st = self.drv.deferred_status()
@@ -426,11 +432,7 @@ class BinomialGradient:
         self.drv.adjoint_enable()
         self.drv.adjoint_zero()
         self.drv.adjoint_seed_power(self.wz, self.scale)
-        if self.dev.type == "cuda":
-            # every buffer of the gradient is allocated from here on
-            free, total = torch.cuda.mem_get_info(self.dev)
-            self.peak_device_bytes = max(self.peak_device_bytes, int(total - free))
-        self._load(slot)
+        self._note_memory()
 
     def _adjoint_unit(self, j: int):
         unit = self.units[j]
@@ -447,7 +449,8 @@ class BinomialGradient:
         if n == 1:
             self._load(base)
             if lo == len(self.units) - 1:
-                self._seed_at_end(base)
+                self._seed()
+                self._load(base)
             self._adjoint_unit(lo)
             return
         if not free:
@@ -456,12 +459,7 @@ class BinomialGradient:
                 self._load(base)
                 self._advance(lo, j)
                 if lo + j == len(self.units) - 1:
-                    for op in self.units[-1]:
-                        self._run(op)
-                    self.J = integrated_power(self.mp, self.drv.power())
-                    self.drv.adjoint_enable()
-                    self.drv.adjoint_zero()
-                    self.drv.adjoint_seed_power(self.wz, self.scale)
+                    self._seed()
                     self._load(base)
                     self._advance(lo, j)
                 self._adjoint_unit(lo + j)
@@ -502,6 +500,7 @@ class BinomialGradient:
         finally:
             drv.set_deferred(False)
         lar, lai = drv.adjoint_download_field_torch()
+        self._note_memory()
         g_seed = float(torch.dot(lar, self.shape_r) + torch.dot(lai, self.shape_i))
         return {"J": self.J, "table": g_table, "seed": g_seed, "stats": dict(self.stats)}
 
