@@ -291,9 +291,13 @@ def case_power0(case):
 def adjoint_bench(steps_check: bool = True):
     """d(integrated power)/d(per-period a_w, seed amplitude) on the adjoint
     workload (helical, periodic 33 x 33 x 256, 2^20 macroparticles, 16
-    periods; every step's capture kept when they fit the engine's budget,
-    else checkpoints every 30 steps): forward and gradient wall time and
-    central differences (h = 1e-6) of two components."""
+    periods), under two memory plans: every step's capture kept in the
+    forward (no replay), and checkpoints every 30 steps with each segment
+    replayed (with step captures) before it is reversed.  Forward and
+    gradient wall time, the device memory in use while the snapshots and
+    adjoint buffers are live, and central differences (h = 1e-6) of two
+    components."""
+    import torch
     from puffin_amd import gpu
     from puffin_amd.adjoint import ModuleGradient, module_program
     from puffin_amd.configs import case_c3
@@ -305,37 +309,60 @@ def adjoint_bench(steps_check: bool = True):
         drv.upload_particles(c.particles)
         drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
         shape = (c.field[0] / c.info["seed_amp"], c.field[1] / c.info["seed_amp"])
-        eng = ModuleGradient(drv, ops, shape, every=30, taper_period=c.taper_period)
         table, seed = c.taper_table, c.info["seed_amp"]
-        eng.evaluate(table, seed)             # warm-up
-        drv.synchronize()
-        t0 = time.perf_counter()
-        J = eng.evaluate(table, seed)
-        t1 = time.perf_counter()
-        eng.forward(table, seed)
-        t2 = time.perf_counter()
-        g = eng.backward()
-        t3 = time.perf_counter()
-        out = dict(workload=WORKLOADS["c3"] + ", 16 periods of 30 steps", J=J,
-                   forward_s=t1 - t0, forward_checkpointed_s=t2 - t1, backward_s=t3 - t2,
-                   gradient_cost_vs_forward=(t3 - t1) / (t1 - t0),
-                   memory_plan=("every step captured in the forward, no replay" if eng.keep_all
-                                else "checkpoints every 30 steps, segments replayed"),
-                   snapshot_bytes=eng.memory_bytes(), grad_seed=g["seed"],
-                   grad_table=[float(v) for v in g["table"]])
-        if steps_check:
-            eps = 1e-6
-            fd_s = (eng.evaluate(table, seed * (1 + eps)) - eng.evaluate(table, seed * (1 - eps))) / (2 * eps * seed)
-            k = int(np.argmax(np.abs(g["table"])))
-            tp, tm = table.copy(), table.copy()
-            tp[k] += eps
-            tm[k] -= eps
-            fd_t = (eng.evaluate(tp, seed) - eng.evaluate(tm, seed)) / (2 * eps)
-            out["fd_check"] = dict(h=eps, seed_rel_err=abs(fd_s - g["seed"]) / abs(g["seed"]),
-                                   table_index=k,
-                                   table_rel_err=abs(fd_t - g["table"][k]) / np.abs(g["table"]).max())
-        log(f"adjoint: forward {out['forward_s']:.3f} s, gradient {t3 - t1:.3f} s "
-            f"({out['gradient_cost_vs_forward']:.2f}x)")
+        plans = {}
+        grads = {}
+        for name, keep_all in (("captures", None), ("checkpoints_every_30", False)):
+            # each engine snapshots the driver's particles as the module's start
+            drv.upload_particles(c.particles)
+            eng = ModuleGradient(drv, ops, shape, every=30, taper_period=c.taper_period, keep_all=keep_all)
+            eng.evaluate(table, seed)             # warm-up
+            drv.synchronize()
+            t0 = time.perf_counter()
+            J = eng.evaluate(table, seed)
+            t1 = time.perf_counter()
+            eng.forward(table, seed)
+            t2 = time.perf_counter()
+            g = eng.backward()
+            t3 = time.perf_counter()
+            plans[name] = dict(
+                memory_plan=("every step captured in the forward, no replay" if eng.keep_all
+                             else "checkpoints every 30 steps, each segment replayed with step captures"
+                             if eng.capture else "checkpoints every 30 steps, segments replayed"),
+                J=J, forward_s=t1 - t0, forward_checkpointed_s=t2 - t1, backward_s=t3 - t2,
+                gradient_cost_vs_forward=(t3 - t1) / (t1 - t0),
+                snapshot_bytes=eng.memory_bytes(), peak_device_bytes=eng.peak_device_bytes,
+                grad_seed=g["seed"])
+            grads[name] = g
+            log(f"adjoint ({name}): forward {t1 - t0:.3f} s, gradient {t3 - t1:.3f} s "
+                f"({plans[name]['gradient_cost_vs_forward']:.2f}x), "
+                f"{eng.peak_device_bytes / 2**30:.1f} GiB in use")
+            if name == "captures" and steps_check:
+                eps = 1e-6
+                fd_s = ((eng.evaluate(table, seed * (1 + eps)) - eng.evaluate(table, seed * (1 - eps)))
+                        / (2 * eps * seed))
+                k = int(np.argmax(np.abs(g["table"])))
+                tp, tm = table.copy(), table.copy()
+                tp[k] += eps
+                tm[k] -= eps
+                fd_t = (eng.evaluate(tp, seed) - eng.evaluate(tm, seed)) / (2 * eps)
+                plans[name]["fd_check"] = dict(
+                    h=eps, seed_rel_err=abs(fd_s - g["seed"]) / abs(g["seed"]), table_index=k,
+                    table_rel_err=abs(fd_t - g["table"][k]) / np.abs(g["table"]).max())
+            del eng, g
+            torch.cuda.empty_cache()
+        a, b = grads["captures"], grads["checkpoints_every_30"]
+        out = dict(workload=WORKLOADS["c3"] + ", 16 periods of 30 steps", plans=plans,
+                   plans_table_rel_diff=float(np.abs(a["table"] - b["table"]).max()
+                                              / np.abs(a["table"]).max()),
+                   plans_seed_rel_diff=abs(a["seed"] - b["seed"]) / abs(a["seed"]))
+        # the headline keys are the captures plan's
+        for k in ("J", "forward_s", "forward_checkpointed_s", "backward_s", "gradient_cost_vs_forward",
+                  "memory_plan", "snapshot_bytes", "peak_device_bytes", "grad_seed"):
+            out[k] = plans["captures"][k]
+        if "fd_check" in plans["captures"]:
+            out["fd_check"] = plans["captures"]["fd_check"]
+        out["grad_table"] = [float(v) for v in a["table"]]
         return out
     finally:
         drv.close()

@@ -176,6 +176,8 @@ class ModuleGradient:
         self.seed = None
         self.J = None
         self.rk4_calls = 0
+        self.dev = dev
+        self.peak_device_bytes = 0
 
     def memory_bytes(self) -> int:
         return (1 + len(self.ckpts)) * self.initial.numel() + sum(t.numel() for t in self.steps)
@@ -253,6 +255,9 @@ class ModuleGradient:
         drv.adjoint_enable()
         drv.adjoint_zero()
         drv.adjoint_seed_power(self.wz, scale)
+        # the snapshots, the captures and the adjoint buffers are all live now
+        free, total = torch.cuda.mem_get_info(self.dev)
+        self.peak_device_bytes = max(self.peak_device_bytes, int(total - free))
         self._deferred(True)
         try:
             if self.keep_all:

@@ -211,10 +211,11 @@ static void factorize(int n, FftAxis& ax) {
 }
 
 template <typename R>
-static int make_axis(Ctx* c, int n, FftAxis& ax) {
+static int make_axis(Ctx* c, int n, FftAxis& ax, bool z2_axis = false) {
     // one line, its ping-pong buffer and the twiddle table share a
-    // workgroup's LDS (160 KiB on gfx950)
-    if (3 * (size_t)n * sizeof(cplx<R>) > 160 * 1024) {
+    // workgroup's LDS (160 KiB on gfx950); 4096-point z2 lines run as
+    // 64 x 64 (four-step) in either form
+    if (3 * (size_t)n * sizeof(cplx<R>) > 160 * 1024 && !(z2_axis && n == 4096)) {
         set_error("FFT line of " + std::to_string(n) + " points does not fit one workgroup's LDS");
         return PUFF_ERR_ARG;
     }
@@ -532,7 +533,7 @@ static int alloc_particles(Ctx* c) {
         PUFF_TRY(dalloc(c, &c->anb, sizeof(cplx<R>) * nm));
         PUFF_TRY(make_axis<R>(c, c->p.nx, c->ax));
         PUFF_TRY(make_axis<R>(c, c->p.ny, c->ay));
-        PUFF_TRY(make_axis<R>(c, c->nzp(), c->az));
+        PUFF_TRY(make_axis<R>(c, c->nzp(), c->az, true));
     }
     return PUFF_OK;
 }
@@ -1020,8 +1021,8 @@ static int download_field(Ctx* c, int kind, double* ar, double* ai) {
 template <typename R>
 static int transform(Ctx* c, int sign) {
     if (c->p.one_d) return PUFF_ERR_STATE;
-    launch_fft3d<R>(reinterpret_cast<cplx<R>*>(c->A), c->p.nx, c->p.ny, c->nzp(), c->ax, c->ay, c->az, sign,
-                    c->stream);
+    launch_fft3d<R>(reinterpret_cast<cplx<R>*>(c->A), reinterpret_cast<cplx<R>*>(c->anb), c->p.nx, c->p.ny,
+                    c->nzp(), c->ax, c->ay, c->az, sign, c->stream);
     PUFF_HIP(hipGetLastError());
     PUFF_HIP(hipStreamSynchronize(c->stream));
     return PUFF_OK;

@@ -16,6 +16,7 @@
 //   x fwd | y fwd | z fwd . multiplyexp . z inv | y inv |
 //   x inv . /ffact . mask split . x fwd | y fwd | z fwd . damping . z inv |
 //   y inv | x inv . /ffact . + interior
+// (4096-point z2 lines: each z2 pass as three four-step passes, k_fft_z4).
 #include "puff_common.hpp"
 #include "puff_kernels.hpp"
 
@@ -27,12 +28,31 @@ namespace puff {
 #ifndef PUFF_Z_THREADS
 #define PUFF_Z_THREADS 256
 #endif
+// Threads per x/y workgroup on a Rader axis.  Its radix-16 sub-passes keep
+// B x 16 of them busy; with the ping-pong transform (7 lines of C4's 257
+// points, 112 busy) 128 and 192 threads measured 15 % and 7 % slower C4
+// events than 256.
+#ifndef PUFF_RADER_THREADS
+#define PUFF_RADER_THREADS 256
+#endif
+
+// LDS word of flat element e of a line-major buffer (IL false): one spare
+// slot after every 16 elements.  A radix-16 pass with p = 1 has each lane
+// write 16 consecutive outputs, so lanes write 16 elements apart, all on the
+// same LDS banks; with the spare slots they land 17 apart and 16 lanes cover
+// all the banks.  Interleaved buffers (IL: y and z2 lines, b fastest) put
+// consecutive lanes on consecutive elements and are not padded.
+#ifndef PUFF_LDS_PAD_X
+#define PUFF_LDS_PAD_X 1
+#endif
+template <bool IL>
+__host__ __device__ constexpr int lpad(int e) { return (!IL && PUFF_LDS_PAD_X) ? e + (e >> 4) : e; }
 
 // LDS index of element j of line b.  Interleaved layout (b fastest) keeps
 // the strided y / z2 loads conflict free; line-major suits contiguous x lines.
 template <bool IL>
 __device__ __forceinline__ int lidx(int b, int j, int B, int N) {
-    return IL ? j * B + b : b * N + j;
+    return lpad<IL>(IL ? j * B + b : b * N + j);
 }
 
 // Line staging between global memory and LDS with U loads (or stores) per
@@ -47,8 +67,8 @@ constexpr int STAGE_UNROLL = 8;
 #define PUFF_Z_UNROLL 1
 #endif
 
-// dst[w] = get(w) for 0 <= w < n.
-template <int U, typename R, typename F>
+// dst[lpad<IL>(w)] = get(w) for 0 <= w < n.
+template <bool IL, int U, typename R, typename F>
 __device__ __forceinline__ void lds_fill(cplx<R>* dst, int n, F get) {
     for (int w0 = threadIdx.x; w0 < n; w0 += U * (int)blockDim.x) {
         cplx<R> v[U];
@@ -60,22 +80,23 @@ __device__ __forceinline__ void lds_fill(cplx<R>* dst, int n, F get) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int w = w0 + u * (int)blockDim.x;
-            if (w < n) dst[w] = v[u];
+            if (w < n) dst[lpad<IL>(w)] = v[u];
         }
     }
 }
 
-// put(w, src[w]) for 0 <= w < n.  The LDS reads are unconditional (element 0
-// stands in past n), so that v stays in registers: with conditional reads
-// the compiler kept v in scratch memory (144 bytes per lane in FP64).
-template <int U, typename R, typename F>
+// put(w, src[lpad<IL>(w)]) for 0 <= w < n.  The LDS reads are unconditional
+// (element 0 stands in past n), so that v stays in registers: with
+// conditional reads the compiler kept v in scratch memory (144 bytes per
+// lane in FP64).
+template <bool IL, int U, typename R, typename F>
 __device__ __forceinline__ void lds_drain(const cplx<R>* src, int n, F put) {
     for (int w0 = threadIdx.x; w0 < n; w0 += U * (int)blockDim.x) {
         cplx<R> v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int w = w0 + u * (int)blockDim.x;
-            v[u] = src[w < n ? w : 0];
+            v[u] = src[lpad<IL>(w < n ? w : 0)];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -83,6 +104,17 @@ __device__ __forceinline__ void lds_drain(const cplx<R>* src, int n, F put) {
             if (w < n) put(w, v[u]);
         }
     }
+}
+
+// threadIdx.x through an empty asm statement: index arithmetic built on it is
+// not merged with the same arithmetic of another call site, so it is
+// recomputed where it is used instead of held in registers between the
+// passes of a kernel (with two in-place Rader transforms per line, 181
+// VGPRs without it).
+__device__ __forceinline__ int opaque_tid() {
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
 }
 
 template <typename R>
@@ -157,6 +189,7 @@ PUFF_CT_PLAN(128, 0, 16, 4, 2)
 PUFF_CT_PLAN(256, 0, 16, 16)
 PUFF_CT_PLAN(1024, 0, 16, 16, 4)
 PUFF_CT_PLAN(4096, 0, 16, 16, 16)
+PUFF_CT_PLAN(64, 0, 16, 4)   // the four-step sub-lines of 4096-point z2 lines
 #undef PUFF_CT_PLAN
 
 // One radix-r pass (after passes whose radices multiply to p) of an in-LDS
@@ -415,7 +448,7 @@ __device__ __forceinline__ void small_radix(cplx<R>* v, int r, int sign) {
 // workgroups per CU.
 template <typename R, bool IL, int KB = 1>
 __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, int p, const cplx<R>* tw,
-                                             int sign) {
+                                             int sign, int LD) {
     // KB butterflies per thread: all their inputs are read into registers
     // before the barrier, all outputs written after it
     const int T = N / r;
@@ -423,9 +456,10 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
     int b[KB], ob[KB];
     bool act[KB];
     cplx<R> v[KB][16];
+    const int tid = opaque_tid();
 #pragma unroll
     for (int it = 0; it < KB; ++it) {
-        const int w = threadIdx.x + it * (int)blockDim.x;
+        const int w = tid + it * (int)blockDim.x;
         act[it] = w < B * T;
         b[it] = 0;
         ob[it] = 0;
@@ -437,7 +471,7 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
             if (r == 16) {
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, N)];
+                    v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, LD)];
                     if (j > 0 && k > 0) v[it][j] = cmul(v[it][j], tw_get(tw, j * k * twst, sign));
                 }
                 dft16(v[it], sign);
@@ -445,7 +479,7 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     if (j < r) {
-                        v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, N)];
+                        v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, LD)];
                         if (j > 0 && k > 0) v[it][j] = cmul(v[it][j], tw_get(tw, j * k * twst, sign));
                     }
                 }
@@ -462,11 +496,11 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
                 for (int q0 = 0; q0 < 4; ++q0)
 #pragma unroll
                     for (int q1 = 0; q1 < 4; ++q1)
-                        buf[lidx<IL>(b[it], ob[it] + (q0 + 4 * q1) * p, B, N)] = v[it][4 * q0 + q1];
+                        buf[lidx<IL>(b[it], ob[it] + (q0 + 4 * q1) * p, B, LD)] = v[it][4 * q0 + q1];
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (q < r) buf[lidx<IL>(b[it], ob[it] + q * p, B, N)] = v[it][q];
+                    if (q < r) buf[lidx<IL>(b[it], ob[it] + q * p, B, LD)] = v[it][q];
             }
         }
     }
@@ -478,7 +512,7 @@ __device__ __forceinline__ void lds_passes_inplace(cplx<R>* buf, int B, int N, i
     int p = 1;
     for (int ps = 0; ps < npass; ++ps) {
         const int r = radix[ps];
-        inplace_pass<R, IL>(buf, B, N, r, p, tw, sign);
+        inplace_pass<R, IL>(buf, B, N, r, p, tw, sign, N);
         __syncthreads();
         p *= r;
     }
@@ -489,14 +523,16 @@ constexpr int inplace_kb(int B, int N, int r, int threads) {
     return (B * (N / r) + threads - 1) / threads;
 }
 
-template <typename R, bool IL, int NC, int BC, int K = 0, int P = 1>
+// The passes of CtPlan<NC> in place over BC lines stored LD apart, with THR
+// threads per workgroup.
+template <typename R, bool IL, int NC, int BC, int LD = NC, int THR = PUFF_Z_THREADS, int K = 0, int P = 1>
 __device__ __forceinline__ void ct_passes_inplace(cplx<R>* buf, const cplx<R>* tw, int sign) {
     if constexpr (K < CtPlan<NC>::np) {
         constexpr int r = CtPlan<NC>::r[K];
-        constexpr int KB = inplace_kb(BC, NC, r, PUFF_Z_THREADS);
-        inplace_pass<R, IL, KB>(buf, BC, NC, r, P, tw, sign);
+        constexpr int KB = inplace_kb(BC, NC, r, THR);
+        inplace_pass<R, IL, KB>(buf, BC, NC, r, P, tw, sign, LD);
         __syncthreads();
-        ct_passes_inplace<R, IL, NC, BC, K + 1, P * r>(buf, tw, sign);
+        ct_passes_inplace<R, IL, NC, BC, LD, THR, K + 1, P * r>(buf, tw, sign);
     }
 }
 
@@ -509,6 +545,109 @@ __device__ __forceinline__ cplx<R>* lds_fft(cplx<R>* src, cplx<R>* dst, int B, c
     } else {
         if (ax.rader) return lds_rader<R, IL, 0>(src, dst, B, ax, tw, sign);
         return lds_passes<R, IL>(src, dst, B, ax.n, ax.n, ax.npass, ax.radix, tw, sign);
+    }
+}
+
+// Rader's algorithm in place, for the compile-time prime lengths (257): one
+// LDS buffer of BC lines instead of two, so twice the lines per workgroup at
+// the same LDS, and the radix-16 passes of the (N - 1)-point transforms
+// keep BC x 16 threads busy.  The two permutations go through registers
+// (each thread reads its elements, the workgroup passes a barrier, the
+// elements go to their new places); X_0 = x_0 + A_0 waits in x0s[b].  The
+// permutation tables are read from LDS (rps: g^j at j < N - 1, then g^-q at
+// N + q), staged by the kernel: read from global memory, the compiler kept
+// both tables' entries in registers across the kernel (191 VGPRs).
+template <typename R, int NC>
+constexpr bool rader_ip() { return sizeof(R) == 4 && NC > 0 && CtPlan<NC>::rader; }
+
+// rps[j] = g^j (j < n - 1), rps[n + q] = g^-q (q < n - 1).
+__device__ __forceinline__ void stage_rader_perm(int* rps, const FftAxis& ax) {
+    for (int w = threadIdx.x; w < ax.n - 1; w += blockDim.x) {
+        rps[w] = ax.rperm[w];
+        rps[ax.n + w] = ax.riperm[w];
+    }
+}
+
+template <typename R, bool IL, int NC, int BC>
+__device__ __forceinline__ void lds_rader_ip(cplx<R>* buf, cplx<R>* x0s, const int* rps, const FftAxis& ax,
+                                             const cplx<R>* tw, int sign) {
+    constexpr int N = NC, M = N - 1;
+    constexpr int THR = PUFF_RADER_THREADS;
+    constexpr int KP = (BC * N + THR - 1) / THR;   // elements per thread in a permutation
+    const cplx<R>* rb = reinterpret_cast<const cplx<R>*>(ax.rb) + (sign > 0 ? M : 0);
+    // a_j = x[g^j] at element j < M; x_0 to element M
+    {
+        cplx<R> v[KP];
+        const int tid = opaque_tid();
+#pragma unroll
+        for (int u = 0; u < KP; ++u) {
+            const int w = tid + u * THR;
+            int b, j;
+            if (IL) { b = w % BC; j = w / BC; } else { j = w % N; b = w / N; }
+            const bool in = w < BC * N;
+            v[u] = buf[in ? lidx<IL>(b, j < M ? rps[j] : 0, BC, N) : 0];
+        }
+        __syncthreads();
+        const int tid2 = opaque_tid();
+#pragma unroll
+        for (int u = 0; u < KP; ++u) {
+            const int w = tid2 + u * THR;
+            int b, j;
+            if (IL) { b = w % BC; j = w / BC; } else { j = w % N; b = w / N; }
+            if (w < BC * N) buf[lidx<IL>(b, j < M ? j : M, BC, N)] = v[u];
+        }
+        __syncthreads();
+    }
+    ct_passes_inplace<R, IL, M, BC, N, THR>(buf, tw, -1);
+    // C = A . rb_s (rb carries 1/M); X_0 = x_0 + A_0
+    for (int w = threadIdx.x; w < BC * M; w += THR) {
+        int b, k;
+        if (IL) { b = w % BC; k = w / BC; } else { k = w % M; b = w / M; }
+        const cplx<R> v = buf[lidx<IL>(b, k, BC, N)];
+        if (k == 0) x0s[b] = cadd(buf[lidx<IL>(b, M, BC, N)], v);
+        buf[lidx<IL>(b, k, BC, N)] = cmul(v, rb[k]);
+    }
+    __syncthreads();
+    ct_passes_inplace<R, IL, M, BC, N, THR>(buf, tw, +1);
+    // X[g^-q] = x_0 + c_q; X_0 to element 0
+    {
+        cplx<R> v[KP];
+        const int tid = opaque_tid();
+#pragma unroll
+        for (int u = 0; u < KP; ++u) {
+            const int w = tid + u * THR;
+            int b, q;
+            if (IL) { b = w % BC; q = w / BC; } else { q = w % N; b = w / N; }
+            const bool in = w < BC * N;
+            const int bb = in ? b : 0;
+            const cplx<R> c = buf[lidx<IL>(bb, q < M ? q : 0, BC, N)];
+            const cplx<R> x0 = buf[lidx<IL>(bb, M, BC, N)];
+            v[u] = q < M ? cadd(x0, c) : x0s[bb];
+        }
+        __syncthreads();
+        const int tid2 = opaque_tid();
+#pragma unroll
+        for (int u = 0; u < KP; ++u) {
+            const int w = tid2 + u * THR;
+            int b, q;
+            if (IL) { b = w % BC; q = w / BC; } else { q = w % N; b = w / N; }
+            if (w < BC * N) buf[lidx<IL>(b, q < M ? rps[N + q] : 0, BC, N)] = v[u];
+        }
+        __syncthreads();
+    }
+}
+
+// Line transform of the x / y passes: in place for rader_ip lengths (dst
+// unused; x0s holds BC values and then the permutation tables), else
+// lds_fft.  Returns the buffer holding the result.
+template <typename R, bool IL, int NC, int BC>
+__device__ __forceinline__ cplx<R>* line_fft(cplx<R>* src, cplx<R>* dst, cplx<R>* x0s, int B, const FftAxis& ax,
+                                             const cplx<R>* tw, int sign) {
+    if constexpr (rader_ip<R, NC>()) {
+        lds_rader_ip<R, IL, NC, BC>(src, x0s, reinterpret_cast<const int*>(x0s + BC), ax, tw, sign);
+        return src;
+    } else {
+        return lds_fft<R, IL, NC>(src, dst, B, ax, tw, sign);
     }
 }
 
@@ -526,20 +665,25 @@ __device__ __forceinline__ cplx<R>* z_fft(cplx<R>* src, cplx<R>* dst, int B, con
 }
 
 // ---- x lines (contiguous) --------------------------------------------------
-template <typename R, int NC, int BC>
-__global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int op, int sign) {
+// OP (an XOP_ value) is a template parameter: each fused operation gets its
+// own register allocation.
+template <typename R, int NC, int BC, int OP>
+__global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int sign) {
+    constexpr int op = OP;
     extern __shared__ unsigned char smem_raw[];
     const int N = NC ? NC : a.nx;
     const int B = BC ? BC : Bp;
     const FftAxis& ax = a.ax;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* s1 = s0 + B * N;
+    cplx<R>* s1 = rader_ip<R, NC>() ? s0 : s0 + lpad<false>(B * N);
     const int64_t nlines = (int64_t)a.ny * a.nzp;
     const int64_t L0 = (int64_t)blockIdx.x * B;
     const int nb = (int)min<int64_t>(B, nlines - L0);
     const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(ax.tw);
-    cplx<R>* tw = s1 + B * N;   // twiddle table staged in LDS
+    cplx<R>* tw = s1 + lpad<false>(B * N);   // twiddle table staged in LDS
+    cplx<R>* x0s = tw + N;            // in-place Rader's X_0 per line and permutations
     for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
+    if constexpr (rader_ip<R, NC>()) stage_rader_perm(reinterpret_cast<int*>(x0s + B), ax);
     cplx<R>* base = a.a + L0 * N;
     const int64_t ntr = (int64_t)a.nx * a.ny;
     // plane and row of the block's first line; line L0 + b is row iy0 + b
@@ -561,15 +705,22 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int op, in
                 }
                 a.anb[L0 * N + w] = v;
             }
-            s0[w] = v;
+            s0[lpad<false>(w)] = v;
         }
     } else {
         const int nv = nb * N;
-        lds_fill<STAGE_UNROLL>(s0, B * N, [&](int w) { return w < nv ? base[w] : cplx<R>{0, 0}; });
+        lds_fill<false, STAGE_UNROLL>(s0, B * N, [&](int w) { return w < nv ? base[w] : cplx<R>{0, 0}; });
     }
     __syncthreads();
     const int first_sign = (op == XOP_FORWARD) ? sign : (op == XOP_ADJ_FOLD_SAVE_FWD ? -1 : +1);
-    cplx<R>* res = lds_fft<R, false, NC>(s0, s1, B, ax, tw, first_sign);
+    // the backward transform's 1/ffact: a divide in FP64 (the reference's
+    // arithmetic), a reciprocal multiply in the mixed form
+    const double rff = 1.0 / a.ffact;
+    auto unscale = [&](R v) -> double {
+        if constexpr (sizeof(R) == 8) return (double)v / a.ffact;
+        else return (double)v * rff;
+    };
+    cplx<R>* res = line_fft<R, false, NC, BC>(s0, s1, x0s, B, ax, tw, first_sign);
     if (op == XOP_ADJ_INV_MIX_FWD) {
         // adjoint of the absorption split: (1 - m) cotangent + m W^H G W cotangent / ffact
         const cplx<R>* lam = a.anb + L0 * N;
@@ -579,18 +730,18 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int op, in
             const int iy = lb % a.ny, iz = iz0 + lb / a.ny;
             const double mxy = a.mxy[(int64_t)iy * a.nx + j];
             const double m = mxy + (a.mz[iz] * (1.0 - mxy));
-            const cplx<R> v = res[w];
+            const cplx<R> v = res[lpad<false>(w)];
             const cplx<R> l = lam[w];
-            res[w] = {(R)((1.0 - m) * (double)l.x + m * ((double)v.x / a.ffact)),
-                      (R)((1.0 - m) * (double)l.y + m * ((double)v.y / a.ffact))};
+            res[lpad<false>(w)] = {(R)((1.0 - m) * (double)l.x + m * unscale(v.x)),
+                            (R)((1.0 - m) * (double)l.y + m * unscale(v.y))};
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = lds_fft<R, false, NC>(res, other, B, ax, tw, -1);
+        res = line_fft<R, false, NC, BC>(res, other, x0s, B, ax, tw, -1);
     } else if (op == XOP_ADJ_INV_SCALE) {
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
-            const cplx<R> v = res[w];
-            res[w] = {(R)((double)v.x / a.ffact), (R)((double)v.y / a.ffact)};
+            const cplx<R> v = res[lpad<false>(w)];
+            res[lpad<false>(w)] = {(R)unscale(v.x), (R)unscale(v.y)};
         }
         __syncthreads();
     } else if (op == XOP_INV_SPLIT_FWD) {
@@ -602,36 +753,64 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int op, in
             const int iy = lb % a.ny, iz = iz0 + lb / a.ny;
             const double mxy = a.mxy[(int64_t)iy * a.nx + j];
             const double m = mxy + (a.mz[iz] * (1.0 - mxy));
-            cplx<R> v = res[w];
-            const double vr = (double)v.x / a.ffact, vi = (double)v.y / a.ffact;
+            cplx<R> v = res[lpad<false>(w)];
+            const double vr = unscale(v.x), vi = unscale(v.y);
             anb[w] = {(R)((1.0 - m) * vr), (R)((1.0 - m) * vi)};
-            res[w] = {(R)(m * vr), (R)(m * vi)};
+            res[lpad<false>(w)] = {(R)(m * vr), (R)(m * vi)};
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
-        res = lds_fft<R, false, NC>(res, other, B, ax, tw, -1);
+        res = line_fft<R, false, NC, BC>(res, other, x0s, B, ax, tw, -1);
     } else if (op == XOP_INV_RECOMBINE) {
         // /ffact + interior (diffraction.f90:513-517)
         const cplx<R>* anb = a.anb + L0 * N;
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
-            const cplx<R> v = res[w];
+            const cplx<R> v = res[lpad<false>(w)];
             const cplx<R> q = anb[w];
             if (sizeof(R) == 8) {
-                res[w] = {(R)((double)v.x / a.ffact + (double)q.x), (R)((double)v.y / a.ffact + (double)q.y)};
+                res[lpad<false>(w)] = {(R)(unscale(v.x) + (double)q.x), (R)(unscale(v.y) + (double)q.y)};
             } else {
-                res[w] = {(R)((double)v.x / a.ffact) + q.x, (R)((double)v.y / a.ffact) + q.y};
+                res[lpad<false>(w)] = {(R)unscale(v.x) + q.x, (R)unscale(v.y) + q.y};
             }
         }
         __syncthreads();
     }
-    lds_drain<STAGE_UNROLL>(res, nb * N, [&](int w, const cplx<R>& v) { base[w] = v; });
+    lds_drain<false, STAGE_UNROLL>(res, nb * N, [&](int w, const cplx<R>& v) { base[w] = v; });
     if (op == XOP_INV_RECOMBINE && a.periodic) {
         // keep the aliased plane NZ2 equal to plane 1 (para_field.f90:3197-3244)
         for (int w = threadIdx.x; w < nb * N; w += blockDim.x) {
             const int64_t L = L0 + w / N;
-            if (L < a.ny) a.a[(int64_t)(a.nz2 - 1) * ntr + L * N + (w % N)] = res[w];
+            if (L < a.ny) a.a[(int64_t)(a.nz2 - 1) * ntr + L * N + (w % N)] = res[lpad<false>(w)];
         }
     }
+}
+
+// ---- persistent tile loops -----------------------------------------------------
+// With compile-time plans the x and y passes run persistent workgroups (as
+// many as fit the device at once): each loops over its tiles of B lines,
+// loading tile t + stride into registers while tile t is transformed in LDS
+// and stored.  A one-tile workgroup has its loads in flight only between
+// its start and its first barrier; here they overlap the transform and the
+// stores of the previous tile.  The tiles are dealt to the XCDs in
+// contiguous ranges (workgroup g runs on XCD g mod 8, the grid being a
+// multiple of 8), so that neighbouring tiles, which share cache lines at
+// their edges, meet in one L2.
+// Off by default: on the C1 and C4 meshes the persistent y pass measured
+// 1-2 % slower events than one tile per workgroup (its prefetch registers
+// cost a wave per SIMD of occupancy).
+#ifndef PUFF_PERSIST_Y
+#define PUFF_PERSIST_Y 0
+#endif
+struct TileRange {
+    int64_t t, end;
+    int stride;
+};
+__device__ __forceinline__ TileRange tile_range(int64_t ntile) {
+    const int per_xcd = (int)gridDim.x / NXCD;
+    const int xcd = (int)blockIdx.x % NXCD, l = (int)blockIdx.x / NXCD;
+    const int64_t per = (ntile + NXCD - 1) / NXCD;
+    const int64_t lo = xcd * per;
+    return {lo + l, min(lo + per, ntile), per_xcd};
 }
 
 // ---- y lines (stride nx), B consecutive x per block -------------------------
@@ -642,24 +821,100 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) 
     const int B = BC ? BC : Bp;
     const FftAxis& ay = a.ay;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* s1 = s0 + B * N;
-    const int x0 = blockIdx.x * B;
-    const int iz = blockIdx.y;
+    cplx<R>* s1 = rader_ip<R, NC>() ? s0 : s0 + lpad<true>(B * N);
+    // blocks of B x-columns, x fastest; neighbouring blocks share the cache
+    // lines at their edges, so they go to one XCD
+    const int nxb = (a.nx + B - 1) / B;
+    constexpr bool persist = NC > 0 && PUFF_PERSIST_Y;
+    const int64_t blk = persist ? 0 : xcd_block((int64_t)nxb * a.nzp);
+    if (blk >= (int64_t)nxb * a.nzp) return;
+    const int x0 = (int)(blk % nxb) * B;
+    const int iz = (int)(blk / nxb);
     const int nb = min(B, a.nx - x0);
     const cplx<R>* twg = reinterpret_cast<const cplx<R>*>(ay.tw);
-    cplx<R>* tw = s1 + B * N;   // twiddle table staged in LDS
+    cplx<R>* tw = s1 + lpad<true>(B * N);   // twiddle table staged in LDS
+    cplx<R>* x0s = tw + N;            // in-place Rader's X_0 per line and permutations
     for (int w = threadIdx.x; w < N; w += blockDim.x) tw[w] = twg[w];
+    if constexpr (rader_ip<R, NC>()) stage_rader_perm(reinterpret_cast<int*>(x0s + B), ay);
+    if constexpr (persist) {
+        // persistent: tiles of B x-columns of one plane, x fastest
+        static_assert(PUFF_RADER_THREADS == 256, "the tile loop assumes 256 threads");
+        constexpr int U = (BC * NC + 255) / 256;   // elements per thread (256 threads)
+        const int64_t ntile = (int64_t)nxb * a.nzp;
+        const TileRange tr = tile_range(ntile);
+        cplx<R> v[U];
+        auto load = [&](int64_t t) {
+            const int tx0 = (int)(t % nxb) * BC, tz = (int)(t / nxb);
+            const int tnb = min(BC, a.nx - tx0);
+            const cplx<R>* tb = a.a + (int64_t)tz * a.nx * a.ny + tx0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int w = (int)threadIdx.x + u * 256;
+                const int b = w % BC, j = w / BC;
+                v[u] = (w < BC * NC && b < tnb) ? tb[(int64_t)j * a.nx + b] : cplx<R>{0, 0};
+            }
+        };
+        if (tr.t < tr.end) load(tr.t);
+        for (int64_t t = tr.t; t < tr.end; t += tr.stride) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int w = (int)threadIdx.x + u * 256;
+                if (w < BC * NC) s0[lpad<true>(w)] = v[u];
+            }
+            __syncthreads();
+            if (t + tr.stride < tr.end) load(t + tr.stride);
+            cplx<R>* res = line_fft<R, true, NC, BC>(s0, s1, x0s, BC, ay, tw, sign);
+            const int tx0 = (int)(t % nxb) * BC, tz = (int)(t / nxb);
+            const int tnb = min(BC, a.nx - tx0);
+            cplx<R>* tb = a.a + (int64_t)tz * a.nx * a.ny + tx0;
+            lds_drain<true, STAGE_UNROLL>(res, BC * NC, [&](int w, const cplx<R>& vv) {
+                const int b = w % BC, j = w / BC;
+                if (b < tnb) tb[(int64_t)j * a.nx + b] = vv;
+            });
+            __syncthreads();
+        }
+        return;
+    }
     cplx<R>* base = a.a + (int64_t)iz * a.nx * a.ny + x0;
-    lds_fill<STAGE_UNROLL>(s0, B * N, [&](int w) {
+    lds_fill<true, STAGE_UNROLL>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
         return (b < nb) ? base[(int64_t)j * a.nx + b] : cplx<R>{0, 0};
     });
     __syncthreads();
-    cplx<R>* res = lds_fft<R, true, NC>(s0, s1, B, ay, tw, sign);
-    lds_drain<STAGE_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
+    cplx<R>* res = line_fft<R, true, NC, BC>(s0, s1, x0s, B, ay, tw, sign);
+    lds_drain<true, STAGE_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
         if (b < nb) base[(int64_t)j * a.nx + b] = v;
     });
+}
+
+// The spectral factor between the forward and inverse z2 transforms, at
+// wavenumbers (kx, ky, kz): multiplyexp's propagator (with its filter) or
+// AbsorptionStep's damping, in FP64 and rounded once.
+template <typename R>
+__device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, double kx, double ky, double kz,
+                                               cplx<R> v) {
+    if (op == SPEC_PROPAGATE) {
+        // multiplyexp (diffraction.f90:150-168)
+        if ((kz > a.cutoff) || (kz < -a.cutoff)) {
+            if (kz != 0.0) {
+                const double th = a.h * (kx * kx + ky * ky) / (2.0 * kz);
+                double sn, cs;
+                sincos(th, &sn, &cs);
+                const double vr = (double)v.x, vi = (double)v.y;
+                v = {(R)(cs * vr - sn * vi), (R)(sn * vr + cs * vi)};
+            }
+        } else if (a.filter) {
+            v = {(R)0, (R)0};
+        }
+    } else {
+        // AbsorptionStep damping (diffraction.f90:484-497)
+        if (kz != 0.0) {
+            const double f = exp(-a.h * a.beta * (fabs(kx) + fabs(ky)) / (sqrt(fabs(2.0 * kz))));
+            v = {(R)(f * (double)v.x), (R)(f * (double)v.y)};
+        }
+    }
+    return v;
 }
 
 // ---- z2 lines (stride nx ny), B consecutive x per block ----------------------
@@ -671,7 +926,7 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
     const int B = BC ? BC : Bp;
     const FftAxis& az = a.az;
     cplx<R>* s0 = reinterpret_cast<cplx<R>*>(smem_raw);
-    cplx<R>* s1 = INPLACE ? s0 : s0 + B * N;
+    cplx<R>* s1 = INPLACE ? s0 : s0 + lpad<true>(B * N);
     // blocks of B x-columns, x fastest; neighbouring blocks share the cache
     // lines of every z2 plane, so they go to one XCD
     const int nxb = (a.nx + B - 1) / B;
@@ -684,7 +939,7 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
     // few lines per workgroup here: the twiddles stay in global memory (L2)
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(az.tw);
     cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
-    lds_fill<PUFF_Z_UNROLL>(s0, B * N, [&](int w) {
+    lds_fill<true, PUFF_Z_UNROLL>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
         return (b < nb) ? base[(int64_t)j * ntr + b] : cplx<R>{0, 0};
     });
@@ -698,38 +953,97 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w % B, q = w / B;
             if (b >= nb) continue;
-            const double kx = a.kx[x0 + b];
-            const double kz = a.kz[q];
-            cplx<R> v = res[w];
-            if (op == SPEC_PROPAGATE) {
-                // multiplyexp (diffraction.f90:150-168)
-                if ((kz > a.cutoff) || (kz < -a.cutoff)) {
-                    if (kz != 0.0) {
-                        const double th = a.h * (kx * kx + ky * ky) / (2.0 * kz);
-                        double sn, cs;
-                        sincos(th, &sn, &cs);
-                        const double vr = (double)v.x, vi = (double)v.y;
-                        v = {(R)(cs * vr - sn * vi), (R)(sn * vr + cs * vi)};
-                    }
-                } else if (a.filter) {
-                    v = {(R)0, (R)0};
-                }
-            } else {
-                // AbsorptionStep damping (diffraction.f90:484-497)
-                if (kz != 0.0) {
-                    const double f = exp(-a.h * a.beta * (fabs(kx) + fabs(ky)) / (sqrt(fabs(2.0 * kz))));
-                    v = {(R)(f * (double)v.x), (R)(f * (double)v.y)};
-                }
-            }
-            res[w] = v;
+            res[lpad<true>(w)] = spectral_op(a, op, a.kx[x0 + b], ky, a.kz[q], res[lpad<true>(w)]);
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
         res = z_fft<R, INPLACE, NC, BC>(res, other, B, az, tw, +1);
     }
-    lds_drain<PUFF_Z_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
+    lds_drain<true, PUFF_Z_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
         if (b < nb) base[(int64_t)j * ntr + b] = v;
+    });
+}
+
+// ---- z2 lines of 4096 points as 64 x 64 (four-step) ---------------------------
+// A 4096-point line fills 32 KiB of LDS (FP32), so a single-pass workgroup
+// holds one or two x-columns and reads 8 or 16 useful bytes of every cache
+// line of the 4096 planes it visits (C4's z2 pass fetched 1.4x and wrote
+// 3.5x its bytes).  Split as n = 64 n1 + n2, k = k1 + 64 k2:
+//   X[k1 + 64 k2] = sum_n2 W64^(n2 k2) W^(n2 k1) sum_n1 x[64 n1 + n2] W64^(n1 k1)
+// (W = exp(-2 pi i / 4096)), the transform runs as passes over 64-point
+// sub-lines, each workgroup holding BX adjacent x-columns (one 128-byte row
+// of every plane it visits) of SETS sub-lines:
+//   Z4_FIRST, set n2: transform over n1 (planes 64 n1 + n2), times
+//     W^(s n2 k1); result k1 back in plane 64 k1 + n2.
+//   Z4_MID, set k1: transform over n2 (planes 64 k1 + n2) -> X[k1 + 64 k2].
+//     With a spectral op: the op at q = k1 + 64 k2, the inverse transform
+//     over k2 and W^(-n2 k1) (the first steps of the inverse four-step),
+//     back in plane 64 k1 + n2.  Without one (plain transform): X written
+//     to plane k1 + 64 k2 of `out`.
+//   Z4_LAST, set n2: transform over k1 (planes 64 k1 + n2) -> x[64 n1 + n2].
+// Each pass moves the field once through HBM in whole rows.
+enum { Z4_FIRST = 0, Z4_MID = 1, Z4_LAST = 2 };
+constexpr int Z4_M = 64;
+template <typename R, int BX, int SETS, int PASS>
+__global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z4(DiffArgs<R> a, int op, int sign, cplx<R>* out) {
+    constexpr int M = Z4_M;
+    constexpr int N = M * M;
+    constexpr int L = BX * SETS;   // sub-lines per workgroup: b = xi + BX si
+    constexpr int NSG = M / SETS;
+    extern __shared__ unsigned char smem_raw[];
+    cplx<R>* buf = reinterpret_cast<cplx<R>*>(smem_raw);
+    cplx<R>* tw = buf + lpad<true>(L * M);   // W64^e = W^(64 e), e < 64
+    const cplx<R>* twn = reinterpret_cast<const cplx<R>*>(a.az.tw);
+    for (int w = threadIdx.x; w < M; w += blockDim.x) tw[w] = twn[w * (N / M)];
+    // x-blocks fastest, then set groups, then rows: neighbouring x-blocks
+    // share the cache lines at their edges and go to one XCD
+    const int nxb = (a.nx + BX - 1) / BX;
+    const int64_t nblk = (int64_t)nxb * NSG * a.ny;
+    const int64_t blk = xcd_block(nblk);
+    if (blk >= nblk) return;
+    const int x0 = (int)(blk % nxb) * BX;
+    const int sg = (int)((blk / nxb) % NSG) * SETS;
+    const int iy = (int)(blk / ((int64_t)nxb * NSG));
+    const int nb = min(BX, a.nx - x0);
+    const int64_t ntr = (int64_t)a.nx * a.ny;
+    const int64_t row = (int64_t)iy * a.nx + x0;
+    cplx<R>* base = a.a + row;
+    // plane of element j of set s
+    auto plane = [](int s, int j) { return PASS == Z4_MID ? M * s + j : M * j + s; };
+    lds_fill<true, (L * M) / PUFF_Z_THREADS>(buf, L * M, [&](int w) {
+        const int b = w % L, j = w / L;
+        const int xi = b % BX, si = b / BX;
+        return xi < nb ? base[(int64_t)plane(sg + si, j) * ntr + xi] : cplx<R>{0, 0};
+    });
+    __syncthreads();
+    const bool fused = PASS == Z4_MID && op >= 0;
+    ct_passes_inplace<R, true, M, L>(buf, tw, fused ? -1 : sign);
+    if (fused) {
+        const double ky = a.ky[iy];
+        for (int w = threadIdx.x; w < L * M; w += blockDim.x) {
+            const int b = w % L, k2 = w / L;
+            const int xi = b % BX, si = b / BX;
+            if (xi >= nb) continue;
+            buf[lpad<true>(w)] = spectral_op(a, op, a.kx[x0 + xi], ky, a.kz[(sg + si) + M * k2], buf[lpad<true>(w)]);
+        }
+        __syncthreads();
+        ct_passes_inplace<R, true, M, L>(buf, tw, +1);
+    }
+    lds_drain<true, (L * M) / PUFF_Z_THREADS>(buf, L * M, [&](int w, const cplx<R>& v) {
+        const int b = w % L, j = w / L;
+        const int xi = b % BX, si = b / BX;
+        if (xi >= nb) return;
+        const int st = sg + si;
+        if (PASS == Z4_FIRST) {
+            base[(int64_t)plane(st, j) * ntr + xi] = cmul(v, tw_get(twn, st * j, sign));   // W^(s n2 k1)
+        } else if (PASS == Z4_MID && !fused) {
+            out[(int64_t)(st + M * j) * ntr + row + xi] = v;   // X[k1 + 64 k2]
+        } else if (PASS == Z4_MID) {
+            base[(int64_t)plane(st, j) * ntr + xi] = cmul(v, tw_get(twn, st * j, +1));     // W^(-n2 k1)
+        } else {
+            base[(int64_t)plane(st, j) * ntr + xi] = v;
+        }
     });
 }
 
@@ -750,17 +1064,10 @@ static void lds_opt_in(const void* kernel, size_t bytes) {
 #ifndef PUFF_XY_LDS_BUDGET
 #define PUFF_XY_LDS_BUDGET (32 * 1024)
 #endif
-// Threads per x/y workgroup on a Rader axis.  Its radix-16 sub-passes keep
-// B x 16 of them busy (112 for C4's 257-point lines), but 128 and 192
-// threads measured 15 % and 7 % slower C4 events than 256.
-#ifndef PUFF_RADER_THREADS
-#define PUFF_RADER_THREADS 256
-#endif
-static constexpr int lines_per_block(int n, size_t elem, int cap, size_t budget = PUFF_XY_LDS_BUDGET) {
+static constexpr int lines_per_block(int n, size_t elem, int cap, size_t budget, bool il) {
     const size_t tw = (size_t)n * elem;
-    int b = budget > tw ? (int)((budget - tw) / (2 * (size_t)n * elem)) : 0;
-    if (b < 1) b = 1;
-    if (b > cap) b = cap;
+    int b = cap;
+    while (b > 1 && (2 * (size_t)(il ? lpad<true>(b * n) : lpad<false>(b * n)) * elem + tw) > budget) --b;
     return b;
 }
 
@@ -796,10 +1103,39 @@ static constexpr ZPlan z_plan(int n, size_t elem, bool rader, int npass, const i
     return ZPlan{B, inplace};
 }
 
+// Lines per workgroup of the in-place Rader passes: one line buffer, the
+// twiddle table and X_0 within `budget`; at most 16 lines (one butterfly per
+// thread in the radix-16 passes).
+#ifndef PUFF_RADER_LDS_BUDGET
+#define PUFF_RADER_LDS_BUDGET (40 * 1024)
+#endif
+static constexpr int lines_per_block_ip(int n, size_t elem, bool il, size_t budget = PUFF_RADER_LDS_BUDGET) {
+    int b = 16;
+    while (b > 1 && ((size_t)(il ? lpad<true>(b * n) : lpad<false>(b * n)) + n + b) * elem +
+                            2 * n * sizeof(int) > budget)
+        --b;
+    return b;
+}
+template <typename R, int NC, bool IL>
+static constexpr size_t xy_lds_elems(int B) {
+    // in-place Rader: line buffer, twiddles, X_0 per line, two int tables of NC
+    return rader_ip<R, NC>() ? (size_t)lpad<IL>(B * NC) + NC + B +
+                                   (2 * NC * sizeof(int) + sizeof(cplx<R>) - 1) / sizeof(cplx<R>)
+                             : 2 * (size_t)lpad<IL>(B * NC) + NC;
+}
 template <typename R, int NC>
-static constexpr int ct_lines_x() { return lines_per_block(NC, sizeof(cplx<R>), 64); }
+static constexpr int ct_lines_x() {
+    return rader_ip<R, NC>() ? lines_per_block_ip(NC, sizeof(cplx<R>), false)
+                             : lines_per_block(NC, sizeof(cplx<R>), 64, PUFF_XY_LDS_BUDGET, false);
+}
+#ifndef PUFF_Y_LDS_BUDGET
+#define PUFF_Y_LDS_BUDGET PUFF_XY_LDS_BUDGET
+#endif
 template <typename R, int NC>
-static constexpr int ct_lines_y() { return lines_per_block(NC, sizeof(cplx<R>), 16); }
+static constexpr int ct_lines_y() {
+    return rader_ip<R, NC>() ? lines_per_block_ip(NC, sizeof(cplx<R>), true)
+                             : lines_per_block(NC, sizeof(cplx<R>), 16, PUFF_Y_LDS_BUDGET, true);
+}
 // Compile-time plans may give a thread several butterflies of an in-place
 // pass (inplace_pass<KB>), up to Z_MAX_HELD complex values per thread
 // (64 VGPRs), so their lines per workgroup follow the LDS budget alone.
@@ -830,37 +1166,75 @@ static constexpr ZPlan ct_zplan() {
     return ZPlan{B2, false};
 }
 
+template <typename R, int NC, int BC, int OP>
+static void launch_x_op(const DiffArgs<R>& a, int B, int sign, hipStream_t s) {
+    const int64_t nlines = (int64_t)a.ny * a.nzp;
+    const size_t sh = (NC ? xy_lds_elems<R, NC, false>(B) : 2 * (size_t)lpad<false>(B * a.nx) + a.nx) *
+                      sizeof(cplx<R>);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_x<R, NC, BC, OP>), sh);
+    const int thr = a.ax.rader ? PUFF_RADER_THREADS : 256;
+    hipLaunchKernelGGL((k_fft_x<R, NC, BC, OP>), dim3((unsigned)((nlines + B - 1) / B)), dim3(thr), sh, s, a, B,
+                       sign);
+}
+
 template <typename R, int NC, int BC>
 static void launch_x(const DiffArgs<R>& a, int B, int op, int sign, hipStream_t s) {
-    const int64_t nlines = (int64_t)a.ny * a.nzp;
-    const size_t sh = (2 * (size_t)B + 1) * a.nx * sizeof(cplx<R>);
-    lds_opt_in(reinterpret_cast<const void*>(&k_fft_x<R, NC, BC>), sh);
-    const int thr = a.ax.rader ? PUFF_RADER_THREADS : 256;
-    hipLaunchKernelGGL((k_fft_x<R, NC, BC>), dim3((unsigned)((nlines + B - 1) / B)), dim3(thr), sh, s, a, B,
-                       op, sign);
+    if (BC) B = BC;   // compile-time plans fix their lines per workgroup
+    switch (op) {
+        case XOP_FORWARD: return launch_x_op<R, NC, BC, XOP_FORWARD>(a, B, sign, s);
+        case XOP_INV_SPLIT_FWD: return launch_x_op<R, NC, BC, XOP_INV_SPLIT_FWD>(a, B, sign, s);
+        case XOP_INV_RECOMBINE: return launch_x_op<R, NC, BC, XOP_INV_RECOMBINE>(a, B, sign, s);
+        case XOP_ADJ_FOLD_SAVE_FWD: return launch_x_op<R, NC, BC, XOP_ADJ_FOLD_SAVE_FWD>(a, B, sign, s);
+        case XOP_ADJ_INV_MIX_FWD: return launch_x_op<R, NC, BC, XOP_ADJ_INV_MIX_FWD>(a, B, sign, s);
+        default: return launch_x_op<R, NC, BC, XOP_ADJ_INV_SCALE>(a, B, sign, s);
+    }
 }
 
 template <typename R>
 static void run_x(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
-    const int B = lines_per_block(a.nx, sizeof(cplx<R>), 64);
+    const int B = lines_per_block(a.nx, sizeof(cplx<R>), 64, PUFF_XY_LDS_BUDGET, false);
     if (a.nx == 129) return launch_x<R, 129, ct_lines_x<R, 129>()>(a, B, op, sign, s);
     if (a.nx == 257) return launch_x<R, 257, ct_lines_x<R, 257>()>(a, B, op, sign, s);
     if (a.nx == 33) return launch_x<R, 33, ct_lines_x<R, 33>()>(a, B, op, sign, s);
     launch_x<R, 0, 0>(a, B, op, sign, s);
 }
 
+// Workgroups of a persistent pass: as many as the device holds at once (the
+// kernel's occupancy at `sh` bytes of LDS times the CUs), a multiple of the
+// XCD count, and no more than the tiles.
+static int64_t persistent_grid(const void* kernel, int threads, size_t sh, int64_t ntile, int* cache) {
+    if (*cache <= 0) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, sh);
+        *cache = (per < 1 ? 1 : per) * (cus < 1 ? 1 : cus);
+    }
+    int64_t g = *cache;
+    const int64_t cap = (ntile + NXCD - 1) / NXCD * NXCD;
+    if (g > cap) g = cap;
+    g = g / NXCD * NXCD;
+    return g < NXCD ? NXCD : g;
+}
+
 template <typename R, int NC, int BC>
 static void launch_y(const DiffArgs<R>& a, int B, int sign, hipStream_t s) {
-    const size_t sh = (2 * (size_t)B + 1) * a.ny * sizeof(cplx<R>);
+    if (BC) B = BC;   // compile-time plans fix their lines per workgroup
+    const size_t sh = (NC ? xy_lds_elems<R, NC, true>(B) : 2 * (size_t)lpad<true>(B * a.ny) + a.ny) *
+                      sizeof(cplx<R>);
     lds_opt_in(reinterpret_cast<const void*>(&k_fft_y<R, NC, BC>), sh);
     const int thr = a.ay.rader ? PUFF_RADER_THREADS : 256;
-    hipLaunchKernelGGL((k_fft_y<R, NC, BC>), dim3((unsigned)((a.nx + B - 1) / B), (unsigned)a.nzp), dim3(thr),
-                       sh, s, a, B, sign);
+    const int64_t ntile = (int64_t)((a.nx + B - 1) / B) * a.nzp;
+    static int cache = 0;   // resident workgroups (one device)
+    const int64_t blocks = (NC > 0 && PUFF_PERSIST_Y) ? persistent_grid(reinterpret_cast<const void*>(&k_fft_y<R, NC, BC>), thr, sh,
+                                                    ntile, &cache)
+                                  : xcd_grid(ntile);
+    hipLaunchKernelGGL((k_fft_y<R, NC, BC>), dim3((unsigned)blocks), dim3(thr), sh, s, a, B, sign);
 }
 
 template <typename R>
 static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
-    const int B = lines_per_block(a.ny, sizeof(cplx<R>), 16);
+    const int B = lines_per_block(a.ny, sizeof(cplx<R>), 16, PUFF_Y_LDS_BUDGET, true);
     if (a.ny == 129) return launch_y<R, 129, ct_lines_y<R, 129>()>(a, B, sign, s);
     if (a.ny == 257) return launch_y<R, 257, ct_lines_y<R, 257>()>(a, B, sign, s);
     if (a.ny == 33) return launch_y<R, 33, ct_lines_y<R, 33>()>(a, B, sign, s);
@@ -869,7 +1243,7 @@ static void run_y(const DiffArgs<R>& a, int sign, hipStream_t s) {
 
 template <typename R, bool IP, int NC, int BC>
 static void launch_z(const DiffArgs<R>& a, int B, int op, int sign, hipStream_t s) {
-    const size_t sh = (IP ? 1 : 2) * (size_t)B * a.nzp * sizeof(cplx<R>);
+    const size_t sh = (IP ? 1 : 2) * (size_t)lpad<true>(B * a.nzp) * sizeof(cplx<R>);
     const int64_t blocks = xcd_grid((int64_t)((a.nx + B - 1) / B) * a.ny);
     lds_opt_in(reinterpret_cast<const void*>(&k_fft_z<R, IP, NC, BC>), sh);
     hipLaunchKernelGGL((k_fft_z<R, IP, NC, BC>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, B, op,
@@ -884,8 +1258,52 @@ static bool launch_z_ct(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
     return true;
 }
 
+// Four-step z2 transform of 4096-point lines (FP64 always: a whole line and
+// its twiddles do not fit one workgroup's LDS; FP32 unless PUFF_Z4 is 0).
+#ifndef PUFF_Z4
+#define PUFF_Z4 1
+#endif
+#ifndef PUFF_Z4_SETS
+#define PUFF_Z4_SETS 2
+#endif
+template <typename R>
+static bool z4_line(int n) {
+    return n == Z4_M * Z4_M && (sizeof(R) == 8 || PUFF_Z4);
+}
+
+template <typename R, int PASS>
+static void launch_z4(const DiffArgs<R>& a, int op, int sign, cplx<R>* out, hipStream_t s) {
+    constexpr int BX = 128 / (int)sizeof(cplx<R>);   // one 128-byte row per plane
+    constexpr int SETS = PUFF_Z4_SETS;
+    constexpr int L = BX * SETS;
+    static_assert((L * Z4_M) % PUFF_Z_THREADS == 0, "whole loads per thread");
+    const size_t sh = ((size_t)lpad<true>(L * Z4_M) + Z4_M) * sizeof(cplx<R>);
+    const int nxb = (a.nx + BX - 1) / BX;
+    const int64_t blocks = xcd_grid((int64_t)nxb * (Z4_M / SETS) * a.ny);
+    lds_opt_in(reinterpret_cast<const void*>(&k_fft_z4<R, BX, SETS, PASS>), sh);
+    hipLaunchKernelGGL((k_fft_z4<R, BX, SETS, PASS>), dim3((unsigned)blocks), dim3(PUFF_Z_THREADS), sh, s, a, op,
+                       sign, out);
+}
+
+// op >= 0: forward, spectral op, inverse, in place.  op < 0: plain transform
+// with `sign`, its middle pass writing to tmp (a field-sized buffer), then
+// copied back.
+template <typename R>
+static void run_z4(const DiffArgs<R>& a, int op, int sign, cplx<R>* tmp, hipStream_t s) {
+    if (op >= 0) {
+        launch_z4<R, Z4_FIRST>(a, op, -1, nullptr, s);
+        launch_z4<R, Z4_MID>(a, op, 0, nullptr, s);
+        launch_z4<R, Z4_LAST>(a, op, +1, nullptr, s);
+    } else {
+        launch_z4<R, Z4_FIRST>(a, op, sign, nullptr, s);
+        launch_z4<R, Z4_MID>(a, op, sign, tmp, s);
+        (void)hipMemcpyAsync(a.a, tmp, sizeof(cplx<R>) * (size_t)a.nx * a.ny * a.nzp, hipMemcpyDeviceToDevice, s);
+    }
+}
+
 template <typename R>
 static void run_z(const DiffArgs<R>& a, int op, int sign, hipStream_t s) {
+    if (z4_line<R>(a.nzp)) return run_z4(a, op, sign, a.anb, s);
     if (launch_z_ct<R, 1024>(a, op, sign, s) || launch_z_ct<R, 4096>(a, op, sign, s) ||
         launch_z_ct<R, 128>(a, op, sign, s) || launch_z_ct<R, 256>(a, op, sign, s))
         return;
@@ -932,10 +1350,11 @@ void launch_diffraction_adjoint(const DiffArgs<R>& a, hipStream_t s) {
 }
 
 template <typename R>
-void launch_fft3d(cplx<R>* data, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
+void launch_fft3d(cplx<R>* data, cplx<R>* tmp, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
                   const FftAxis& az, int sign, hipStream_t s) {
     DiffArgs<R> a{};
     a.a = data;
+    a.anb = tmp;
     a.nx = nx;
     a.ny = ny;
     a.nz2 = nz;
@@ -952,9 +1371,9 @@ template void launch_diffraction<double>(const DiffArgs<double>&, hipStream_t);
 template void launch_diffraction<float>(const DiffArgs<float>&, hipStream_t);
 template void launch_diffraction_adjoint<double>(const DiffArgs<double>&, hipStream_t);
 template void launch_diffraction_adjoint<float>(const DiffArgs<float>&, hipStream_t);
-template void launch_fft3d<double>(cplx<double>*, int, int, int, const FftAxis&, const FftAxis&,
-                                   const FftAxis&, int, hipStream_t);
-template void launch_fft3d<float>(cplx<float>*, int, int, int, const FftAxis&, const FftAxis&,
+template void launch_fft3d<double>(cplx<double>*, cplx<double>*, int, int, int, const FftAxis&,
+                                   const FftAxis&, const FftAxis&, int, hipStream_t);
+template void launch_fft3d<float>(cplx<float>*, cplx<float>*, int, int, int, const FftAxis&, const FftAxis&,
                                   const FftAxis&, int, hipStream_t);
 
 }  // namespace puff

@@ -307,8 +307,9 @@ template <typename R>
 void launch_diffraction_adjoint(const DiffArgs<R>& a, hipStream_t s);
 
 // Plain 3-D transform (tests): sign -1 forward, +1 backward, unnormalised.
+// tmp: a buffer of the field's size (the four-step z2 pass's middle output).
 template <typename R>
-void launch_fft3d(cplx<R>* a, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
+void launch_fft3d(cplx<R>* a, cplx<R>* tmp, int nx, int ny, int nz, const FftAxis& ax, const FftAxis& ay,
                   const FftAxis& az, int sign, hipStream_t s);
 
 // ---- diagnostics -----------------------------------------------------------

@@ -91,6 +91,29 @@ def test_diffraction_adjoint_matches_oracle(periodic, filt):
     assert _rel(gi, li) <= 1e-12
 
 
+def test_diffraction_adjoint_four_step_lines_matches_oracle():
+    """The adjoint event on 4096-point z2 lines (four-step z2 passes)."""
+    print("[gpu] diffraction adjoint, 4096-point z2 lines", flush=True)
+    gpu = _gpu()
+    from puffin_amd.params import ModuleParams
+    mp = ModuleParams(nx=9, ny=17, nz2=4096, dx=0.3, dy=0.3, dz2=0.002, rho=0.01, eta=1e-4,
+                      kappa=0.5, gamma_r=100.0, aw=1.0, nbz2=65, filt_cutoff=15.0, beta=1.0)
+    rng = np.random.default_rng(9)
+    y = rng.standard_normal(mp.nmesh) + 1j * rng.standard_normal(mp.nmesh)
+    lr, li = y.real.copy(), y.imag.copy()
+    orc.diffraction_adjoint(mp, lr, li, 0.0837)
+    p = dict(x=np.zeros(1), y=np.zeros(1), z2=np.full(1, 0.1), px=np.zeros(1), py=np.zeros(1),
+             gam=np.ones(1), chi=np.zeros(1))
+    with gpu.Integrator(mp, 1, FP64) as drv:
+        drv.upload_particles(p)
+        drv.adjoint_enable()
+        drv.adjoint_upload(None, y.real.copy(), y.imag.copy())
+        drv.diffraction_adjoint(0.0837)
+        _, gr, gi = drv.adjoint_download()
+    assert _rel(gr, lr) <= 1e-12
+    assert _rel(gi, li) <= 1e-12
+
+
 def test_state_snapshot_round_trip():
     print("[gpu] state snapshot round trip", flush=True)
     gpu = _gpu()

@@ -75,14 +75,14 @@ def test_f1main_golden_vectors_on_gpu():
 @pytest.mark.parametrize("shape", [(9, 7, 8), (33, 33, 128), (129, 5, 16), (5, 6, 257), (11, 4, 43),
                                    (17, 17, 17), (257, 3, 4), (3, 257, 4), (85, 7, 6),
                                    # the line lengths with compile-time plans (x, y and z2);
-                                   # FP64 lines of 4096 points exceed a workgroup's LDS
+                                   # 4096-point z2 lines run as 64 x 64 (four-step), in
+                                   # x-blocks of 8 (FP64) or 16 (FP32) columns: 17 leaves a
+                                   # partial block in both forms
                                    (129, 129, 4), (33, 33, 256), (3, 5, 1024), (2, 3, 4096),
-                                   (257, 257, 4)])
+                                   (17, 3, 4096), (257, 257, 4)])
 @pytest.mark.parametrize("precision", [FP64, MIXED])
 def test_transform_matches_numpy(shape, precision):
     nx, ny, nz = shape
-    if precision == FP64 and nz == 4096:
-        pytest.skip("FP64 lines of 4096 points do not fit a workgroup's LDS (refused by puff_create)")
     print(f"[gpu] transform shape={shape} precision={precision}", flush=True)
     mp = ModuleParams(nx=nx, ny=ny, nz2=nz, dx=0.1, dy=0.1, dz2=0.01, rho=0.01, eta=1e-4,
                       kappa=0.5, gamma_r=100.0, aw=1.0)
@@ -133,6 +133,36 @@ def test_diffraction_step_matches_oracle(periodic):
         gr, gi = g.download_field()
     assert relerr(gr, o.ar) < 1e-12
     assert relerr(gi, o.ai) < 1e-12
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_diffraction_step_four_step_lines_match_oracle(periodic, precision):
+    """4096-point z2 lines (C4's temporal mesh) take the four-step z2 passes:
+    the propagator and the damping at q = k1 + 64 k2 between them."""
+    print(f"[gpu] diffraction 4096-point z2 lines periodic={periodic} precision={precision}", flush=True)
+    nz2 = 4097 if periodic else 4096
+    mp = ModuleParams(nx=17, ny=9, nz2=nz2, dx=0.3, dy=0.3, dz2=0.002, rho=0.01, eta=1e-4,
+                      kappa=0.5, gamma_r=100.0, aw=1.0, periodic=periodic, nbz2=65,
+                      filt_cutoff=15.0, beta=1.0)
+    rng = np.random.default_rng(23 + periodic)
+    ar = rng.standard_normal(mp.nmesh)
+    ai = rng.standard_normal(mp.nmesh)
+    if periodic:
+        ntr = mp.ntr
+        ar[(nz2 - 1) * ntr:] = ar[:ntr]
+        ai[(nz2 - 1) * ntr:] = ai[:ntr]
+    dummy = dict(x=np.zeros(1), y=np.zeros(1), z2=np.zeros(1), px=np.zeros(1), py=np.zeros(1),
+                 gam=np.ones(1), chi=np.zeros(1))
+    o = orc.Oracle(mp, dummy, (ar, ai))
+    o.diffraction_step(0.0837)
+    with gpu.Integrator(mp, 1, precision) as g:
+        g.upload_field(ar, ai)
+        g.diffraction_step(0.0837)
+        gr, gi = g.download_field()
+    tol = 1e-12 if precision == FP64 else 2e-5
+    assert relerr(gr, o.ar) < tol
+    assert relerr(gi, o.ai) < tol
 
 
 # ---------------------------------------------------------------------------
