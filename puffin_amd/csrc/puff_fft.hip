@@ -59,12 +59,12 @@ __device__ __forceinline__ int lidx(int b, int j, int B, int N) {
 // thread issued back to back.  A plain element loop (U = 1) writes each
 // loaded value to LDS before it issues the next load, so every thread has
 // one load in flight and pays the memory latency once per element.  U = 8
-// took C4's x and y passes from 4.69 and 3.37 ms to 4.25 and 2.66 ms; the
-// z2 passes, whose elements lie NX NY apart, ran 4-5 % slower with it and
-// keep U = 1.
+// took C4's x and y passes from 4.69 and 3.37 ms to 4.25 and 2.66 ms, and
+// (with the XCD-grouped z2 workgroups) C1's FP64 event from 2.47 to 2.40 ms
+// through its z2 passes.
 constexpr int STAGE_UNROLL = 8;
 #ifndef PUFF_Z_UNROLL
-#define PUFF_Z_UNROLL 1
+#define PUFF_Z_UNROLL 8
 #endif
 
 // dst[lpad<IL>(w)] = get(w) for 0 <= w < n.
@@ -890,10 +890,38 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) 
 
 // The spectral factor between the forward and inverse z2 transforms, at
 // wavenumbers (kx, ky, kz): multiplyexp's propagator (with its filter) or
-// AbsorptionStep's damping, in FP64 and rounded once.
+// AbsorptionStep's damping.  FP64 form: in FP64, rounded once.  Mixed form
+// (an FP32 field): the propagator phase is formed in FP64 and reduced
+// modulo 2 pi there (Cody-Waite, with 2 pi split into a double and its
+// remainder), so that the FP32 sine and cosine see |r| <= pi; the damping
+// exponent is formed in FP64 and exponentiated in FP32.  FP64 sine, cosine
+// and exponential per node cost C4's event 1.2 ms of 22.7.
 template <typename R>
 __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, double kx, double ky, double kz,
                                                cplx<R> v) {
+    if constexpr (sizeof(R) == 4) {
+        if (op == SPEC_PROPAGATE) {
+            if ((kz > a.cutoff) || (kz < -a.cutoff)) {
+                if (kz != 0.0) {
+                    constexpr double two_pi_hi = 6.283185307179586232;      // 2 pi rounded to double
+                    constexpr double two_pi_lo = 2.4492935982947064e-16;    // 2 pi - two_pi_hi
+                    constexpr double inv_two_pi = 0.15915494309189533577;
+                    const double th = a.h * (kx * kx + ky * ky) / (2.0 * kz);
+                    const double n = rint(th * inv_two_pi);
+                    const double r = fma(-n, two_pi_lo, fma(-n, two_pi_hi, th));
+                    float sn, cs;
+                    sincosf((float)r, &sn, &cs);
+                    v = {cs * v.x - sn * v.y, sn * v.x + cs * v.y};
+                }
+            } else if (a.filter) {
+                v = {0.0f, 0.0f};
+            }
+        } else if (kz != 0.0) {
+            const float f = expf((float)(-a.h * a.beta * (fabs(kx) + fabs(ky)) / (sqrt(fabs(2.0 * kz)))));
+            v = {f * v.x, f * v.y};
+        }
+        return v;
+    }
     if (op == SPEC_PROPAGATE) {
         // multiplyexp (diffraction.f90:150-168)
         if ((kz > a.cutoff) || (kz < -a.cutoff)) {
@@ -1104,10 +1132,13 @@ static constexpr ZPlan z_plan(int n, size_t elem, bool rader, int npass, const i
 }
 
 // Lines per workgroup of the in-place Rader passes: one line buffer, the
-// twiddle table and X_0 within `budget`; at most 16 lines (one butterfly per
-// thread in the radix-16 passes).
+// twiddle table, X_0 and the permutation tables within `budget`; at most 16
+// lines (one butterfly per thread in the radix-16 passes).  22 KiB (8 of
+// C4's 257-point FP32 lines, 7 workgroups per CU) measured 20.7 ms per C4
+// event against 21.7 (18 KiB, 6 lines), 21.6 (26 KiB, 10 lines) and 22.1
+// (40 KiB, 16 lines).
 #ifndef PUFF_RADER_LDS_BUDGET
-#define PUFF_RADER_LDS_BUDGET (40 * 1024)
+#define PUFF_RADER_LDS_BUDGET (22 * 1024)
 #endif
 static constexpr int lines_per_block_ip(int n, size_t elem, bool il, size_t budget = PUFF_RADER_LDS_BUDGET) {
     int b = 16;
