@@ -152,7 +152,17 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
         __syncthreads();
     }
     R fr = 0, fi = 0;
-    if (ok) {
+    if (ok && lds_field && !m.one_d) {
+        // getFFelecs' corner order, the weights formed once per lane
+        const CornerSet<R> cs = corner_set<R>(c, tb, 0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const cplx<R> v = s_fld[cr.t0 + cs.oxy[k & 3] + cs.oz[k >> 2]];
+            const R wk = cs.xy[k & 3] * cs.z[k >> 2];
+            fr = fma(wk, v.x, fr);
+            fi = fma(wk, v.y, fi);
+        }
+    } else if (ok) {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             if (k < nnb) {
@@ -186,7 +196,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
                               reinterpret_cast<unsigned long long*>(s_dep));
         } else if constexpr (PACKED) {
             if (use_tile) {
-                scatter_cells_packed(m, tb, ok, c, cr, (float)sr, (float)si, packed_scale(tb.extra),
+                scatter_cells_packed(m, tb, ok, c, cr, (float)sr, (float)si, packed_scale_log2(tb.extra),
                                      reinterpret_cast<cplx<float>*>(a.dAout), s_dep);
             } else {
                 scatter_cells<R, float>(m, tb, false, ok, c, cr, sr, si, a.dAout, (float*)nullptr);

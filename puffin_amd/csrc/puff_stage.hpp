@@ -182,17 +182,19 @@ __device__ __forceinline__ TileBox workgroup_box(const MeshGeom& m, bool ok, con
 }
 
 // Node q of a box -> (lx, ly, lz), x fastest, without integer division: the
-// quotients come from float reciprocals of the box extents, exact while
-// q < 2^22 (the relative error of (q + 1/2) * RN(1/t) stays below the
-// distance 1/(2t) of (q + 1/2)/t from the next integer); boxes here hold at
-// most a few thousand nodes.
+// quotients come from the hardware reciprocal of the box extents (v_rcp_f32,
+// within 1 ulp), exact while q < 2^21 (the relative error of
+// (q + 1/2) * rcp(t), below 2^-22, keeps it within the distance 1/(2t) of
+// (q + 1/2)/t from the next integer); boxes here hold at most a few thousand
+// nodes.  (A correctly rounded 1/t is a 12-instruction sequence, twice per
+// box: 6 % of a mixed-form stage kernel's VALU instructions.)
 struct BoxDiv {
     int tx, ty;
     float inv_tx, inv_ty;
 };
 
 __device__ __forceinline__ BoxDiv box_div(const TileBox& t) {
-    return BoxDiv{t.tx, t.ty, 1.0f / (float)t.tx, 1.0f / (float)t.ty};
+    return BoxDiv{t.tx, t.ty, __builtin_amdgcn_rcpf((float)t.tx), __builtin_amdgcn_rcpf((float)t.ty)};
 }
 
 __device__ __forceinline__ void box_coords(const BoxDiv& d, int q, int& lx, int& ly, int& lz) {
@@ -265,6 +267,47 @@ __device__ __forceinline__ R corner_weight(const MeshGeom& m, const Cell<R>& c, 
     return wx * wy * wz;
 }
 
+// The eight corners of a lane's 3-D cell at once, for the LDS-tile paths:
+// weights w = (wx wy) wz as corner_weight forms them, and tile offsets.
+// Corner label q stands for corner k = q ^ rot (rot < 8): visiting the
+// labels 0..7 in order with rot = lane mod 8 sends the lanes of one cell to
+// different corners at each visit, so that an instruction's LDS atomics do
+// not pile onto one word.  rot = 0: getInterps_3D's order.  Formed once per
+// lane: the per-corner selects of corner_weight / corner_tile under a
+// lane-dependent order cost about 25 VALU instructions per corner.
+template <typename R>
+struct CornerSet {
+    R xy[4];     // wx wy of labels 0..3 (x, then y)
+    R z[2];      // wz of the labels' z plane (labels 0-3, 4-7)
+    int oxy[4];  // tile offset of labels 0..3 within the z plane
+    int oz[2];   // tile offset of the two z planes
+};
+
+template <typename R>
+__device__ __forceinline__ CornerSet<R> corner_set(const Cell<R>& c, const TileBox& t, int rot) {
+    const R ux = (R)1 - c.fx, uy = (R)1 - c.fy, uz = (R)1 - c.fz;
+    const bool rx = rot & 1, ry = rot & 2, rz = rot & 4;
+    const R ax0 = rx ? c.fx : ux, ax1 = rx ? ux : c.fx;
+    const R ay0 = ry ? c.fy : uy, ay1 = ry ? uy : c.fy;
+    CornerSet<R> s;
+    s.xy[0] = ax0 * ay0;
+    s.xy[1] = ax1 * ay0;
+    s.xy[2] = ax0 * ay1;
+    s.xy[3] = ax1 * ay1;
+    s.z[0] = rz ? c.fz : uz;
+    s.z[1] = rz ? uz : c.fz;
+    const int ox0 = rx ? 1 : 0, ox1 = 1 - ox0;
+    const int oy0 = ry ? t.tx : 0, oy1 = t.tx - oy0;
+    s.oxy[0] = ox0 + oy0;
+    s.oxy[1] = ox1 + oy0;
+    s.oxy[2] = ox0 + oy1;
+    s.oxy[3] = ox1 + oy1;
+    const int txy = t.tx * t.ty;
+    s.oz[0] = rz ? txy : 0;
+    s.oz[1] = txy - s.oz[0];
+    return s;
+}
+
 __device__ __forceinline__ int corner_node(const MeshGeom& m, const CellRef& r, int k) {
     if (m.one_d) return k ? r.b1 : r.b0;
     return ((k & 4) ? r.b1 : r.b0) + (k & 1) + ((k & 2) ? m.nx : 0);
@@ -298,12 +341,23 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
         __syncthreads();
         if (ok) {
             const D dsr = (D)sr, dsi = (D)si;
-            for (int q = 0; q < nnb; ++q) {
-                const int k = (q + lane) & (nnb - 1);
-                const D wk = (D)corner_weight<R>(m, c, k);
-                const int tk = corner_tile(m, t, cr, k);
-                atomicAdd(&s_tile[tk], wk * dsr);
-                atomicAdd(&s_tile[nt + tk], wk * dsi);
+            if (!m.one_d) {
+                const CornerSet<R> cs = corner_set<R>(c, t, lane & 7);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const D wk = (D)(cs.xy[q & 3] * cs.z[q >> 2]);
+                    const int tk = cr.t0 + cs.oxy[q & 3] + cs.oz[q >> 2];
+                    atomicAdd(&s_tile[tk], wk * dsr);
+                    atomicAdd(&s_tile[nt + tk], wk * dsi);
+                }
+            } else {
+                for (int q = 0; q < nnb; ++q) {
+                    const int k = (q + lane) & (nnb - 1);
+                    const D wk = (D)corner_weight<R>(m, c, k);
+                    const int tk = corner_tile(m, t, cr, k);
+                    atomicAdd(&s_tile[tk], wk * dsr);
+                    atomicAdd(&s_tile[nt + tk], wk * dsi);
+                }
             }
         }
         __syncthreads();
@@ -361,17 +415,27 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
 // read as signed.  Integer sums are exact (and independent of the atomics'
 // order); each contribution is rounded once, to 2^-31 of the largest node
 // sum the scale allows.  The flush divides by S and adds in FP32.
-__device__ __forceinline__ float packed_scale(int max_bits) {
+//
+// In 3-D each contribution (wx wy) (wz s S) is rounded to an integer by one
+// FMA with 1.5 2^23 added (|w s S| < 2^22 puts the sum in [2^23, 2^24), where
+// FP32 steps by 1, so the FMA's single rounding is rint), whose bit pattern
+// is 0x4B400000 + the integer: the two patterns of a contribution, as one
+// 64-bit word, minus 0x4B4000004B400000 are qr 2^32 + qi (modulo 2^64).
+constexpr float PACK_MAGIC = 12582912.0f;                      // 1.5 2^23
+constexpr unsigned long long PACK_MAGIC2 = 0x4B4000004B400000ull;   // its pattern, twice
+
+// log2 of S for the workgroup's largest |s| (its bit pattern, max_bits).
+__device__ __forceinline__ int packed_scale_log2(int max_bits) {
     const float mx = __int_as_float(max_bits);
-    if (!(mx > 0.0f)) return 1.0f;
+    if (!(mx > 0.0f)) return 0;
     int e = 0;
-    frexpf(mx, &e);                   // mx < 2^e
-    return ldexpf(1.0f, 30 - 8 - e);  // 256 mx S < 2^30 (STAGE_THREADS <= 256)
+    frexpf(mx, &e);      // mx < 2^e
+    return 30 - 8 - e;   // 256 mx S < 2^30 (STAGE_THREADS <= 256)
 }
 
 __device__ __forceinline__ void scatter_cells_packed(const MeshGeom& m, const TileBox& t, bool ok,
                                                      const Cell<float>& c, const CellRef& cr, float sr,
-                                                     float si, float S, cplx<float>* out,
+                                                     float si, int log2S, cplx<float>* out,
                                                      unsigned long long* s_tile) {
     const int lane = threadIdx.x & 63;
     const int nnb = m.one_d ? 2 : 8;
@@ -379,24 +443,38 @@ __device__ __forceinline__ void scatter_cells_packed(const MeshGeom& m, const Ti
     for (int q = threadIdx.x; q < nt; q += blockDim.x) s_tile[q] = 0ull;
     __syncthreads();
     if (ok) {
-        const float ssr = sr * S, ssi = si * S;
-        for (int q = 0; q < nnb; ++q) {
-            const int k = (q + lane) & (nnb - 1);
-            const float wk = corner_weight<float>(m, c, k);
-            const long long qr = (long long)(int)rintf(wk * ssr);
-            const long long qi = (long long)(int)rintf(wk * ssi);
-            atomicAdd(&s_tile[corner_tile(m, t, cr, k)], (unsigned long long)(qr * 4294967296LL + qi));
+        const float ssr = ldexpf(sr, log2S), ssi = ldexpf(si, log2S);
+        if (!m.one_d) {
+            const CornerSet<float> cs = corner_set<float>(c, t, lane & 7);
+            const float zr[2] = {cs.z[0] * ssr, cs.z[1] * ssr};
+            const float zi[2] = {cs.z[0] * ssi, cs.z[1] * ssi};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const float vr = fmaf(cs.xy[q & 3], zr[q >> 2], PACK_MAGIC);
+                const float vi = fmaf(cs.xy[q & 3], zi[q >> 2], PACK_MAGIC);
+                const unsigned long long wq =
+                    (((unsigned long long)__float_as_uint(vr) << 32) | __float_as_uint(vi)) - PACK_MAGIC2;
+                atomicAdd(&s_tile[cr.t0 + cs.oxy[q & 3] + cs.oz[q >> 2]], wq);
+            }
+        } else {
+            for (int q = 0; q < nnb; ++q) {
+                const int k = (q + lane) & (nnb - 1);
+                const float wk = corner_weight<float>(m, c, k);
+                const long long qr = (long long)(int)rintf(wk * ssr);
+                const long long qi = (long long)(int)rintf(wk * ssi);
+                atomicAdd(&s_tile[corner_tile(m, t, cr, k)], (unsigned long long)(qr * 4294967296LL + qi));
+            }
         }
     }
     __syncthreads();
     const int ntr = m.nx * m.ny;
     const BoxDiv bd = box_div(t);
-    const float inv = 1.0f / S;
+    const float inv = ldexpf(1.0f, -log2S);
     for (int q = threadIdx.x; q < nt; q += blockDim.x) {
         const long long w = (long long)s_tile[q];
         if (w != 0) {
             const long long lo = (long long)(int)(unsigned int)(w & 0xffffffffLL);
-            const long long hi = (w - lo) / 4294967296LL;
+            const long long hi = (w - lo) >> 32;   // exact: w - lo is a multiple of 2^32
             int lx, ly, lz;
             box_coords(bd, q, lx, ly, lz);
             const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
@@ -438,10 +516,7 @@ __device__ __forceinline__ void scatter_cells_det(const MeshGeom& m, const TileB
         for (int q = threadIdx.x; q < DET_WORDS * nt; q += blockDim.x) s_tile[q] = 0ull;
         __syncthreads();
         if (ok) {
-            for (int q = 0; q < nnb; ++q) {
-                const int k = (q + lane) & (nnb - 1);
-                const double wk = corner_weight<double>(m, c, k);
-                const int tk = corner_tile(m, t, cr, k);
+            auto add = [&](double wk, int tk) {
                 unsigned long long h, l;
                 det_split(wk * sr, scale, h, l);
                 atomicAdd(&s_tile[tk], h);
@@ -449,6 +524,16 @@ __device__ __forceinline__ void scatter_cells_det(const MeshGeom& m, const TileB
                 det_split(wk * si, scale, h, l);
                 atomicAdd(&s_tile[2 * nt + tk], h);
                 atomicAdd(&s_tile[3 * nt + tk], l);
+            };
+            if (!m.one_d) {
+                const CornerSet<double> cs = corner_set<double>(c, t, lane & 7);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) add(cs.xy[q & 3] * cs.z[q >> 2], cr.t0 + cs.oxy[q & 3] + cs.oz[q >> 2]);
+            } else {
+                for (int q = 0; q < nnb; ++q) {
+                    const int k = (q + lane) & (nnb - 1);
+                    add(corner_weight<double>(m, c, k), corner_tile(m, t, cr, k));
+                }
             }
         }
         __syncthreads();
