@@ -71,6 +71,9 @@ constexpr int BOX_COPIES = 64;
 #define PUFF_STAGE_THREADS 256
 #endif
 constexpr int STAGE_THREADS = PUFF_STAGE_THREADS;
+// (256 or 512: the packed tile's rounding needs STAGE_THREADS >= 256)
+constexpr int STAGE_THREADS_LOG2 = STAGE_THREADS == 512 ? 9 : STAGE_THREADS == 256 ? 8 : -1;
+static_assert(STAGE_THREADS_LOG2 > 0, "stage workgroups of 256 or 512 threads");
 #ifndef PUFF_TILE_NODES_FP64
 #define PUFF_TILE_NODES_FP64 1280
 #endif

@@ -361,16 +361,22 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
             }
         }
         __syncthreads();
+        // flush: lanes 2j and 2j + 1 take the real and imaginary parts of
+        // node j, so that one atomic instruction covers both parts of 32
+        // nodes (each device-scope atomic is a memory-side request per
+        // 32-byte sector and instruction: half the requests of one
+        // instruction per part)
         const int ntr = m.nx * m.ny;
         const BoxDiv bd = box_div(t);
-        for (int q = threadIdx.x; q < nt; q += blockDim.x) {
-            const D tr = s_tile[q], ti = s_tile[nt + q];
-            if (tr != (D)0 || ti != (D)0) {
+        R* outc = reinterpret_cast<R*>(out);
+        for (int e = threadIdx.x; e < 2 * nt; e += blockDim.x) {
+            const int q = e >> 1, part = e & 1;
+            const D v = s_tile[part * nt + q];
+            if (v != (D)0) {
                 int lx, ly, lz;
                 box_coords(bd, q, lx, ly, lz);
                 const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
-                atomic_add(&out[node].x, (R)tr);
-                atomic_add(&out[node].y, (R)ti);
+                atomic_add(outc + 2 * node + part, (R)v);
             }
         }
     } else {
@@ -408,7 +414,7 @@ __device__ __forceinline__ void scatter_cells(const MeshGeom& m, const TileBox& 
 // One LDS atomic per corner for both components: each contribution becomes
 // the integers qr = rint(w sr S), qi = rint(w si S) with the workgroup's
 // power-of-two scale S, and qr 2^32 + qi is added to one 64-bit word of the
-// tile.  S keeps every node's sums below 2^30 in magnitude (256 lanes, each
+// tile.  S keeps every node's sums below 2^30 in magnitude (STAGE_THREADS lanes, each
 // adding at most the workgroup's largest |s| to a node), so the low half
 // never carries into the high half beyond what its sign extension undoes:
 // the word is sum qr 2^32 + sum qi exactly, and sum qi is its low 32 bits
@@ -430,7 +436,7 @@ __device__ __forceinline__ int packed_scale_log2(int max_bits) {
     if (!(mx > 0.0f)) return 0;
     int e = 0;
     frexpf(mx, &e);      // mx < 2^e
-    return 30 - 8 - e;   // 256 mx S < 2^30 (STAGE_THREADS <= 256)
+    return 30 - STAGE_THREADS_LOG2 - e;   // STAGE_THREADS mx S < 2^30
 }
 
 __device__ __forceinline__ void scatter_cells_packed(const MeshGeom& m, const TileBox& t, bool ok,
@@ -470,16 +476,19 @@ __device__ __forceinline__ void scatter_cells_packed(const MeshGeom& m, const Ti
     const int ntr = m.nx * m.ny;
     const BoxDiv bd = box_div(t);
     const float inv = ldexpf(1.0f, -log2S);
-    for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+    float* outc = reinterpret_cast<float*>(out);
+    // lanes 2j and 2j + 1: real and imaginary part of node j (as in
+    // scatter_cells: one atomic instruction per 32 nodes' both parts)
+    for (int e = threadIdx.x; e < 2 * nt; e += blockDim.x) {
+        const int q = e >> 1, part = e & 1;
         const long long w = (long long)s_tile[q];
-        if (w != 0) {
-            const long long lo = (long long)(int)(unsigned int)(w & 0xffffffffLL);
-            const long long hi = (w - lo) >> 32;   // exact: w - lo is a multiple of 2^32
+        const long long lo = (long long)(int)(unsigned int)(w & 0xffffffffLL);
+        const long long v = part ? lo : (w - lo) >> 32;   // exact: w - lo is a multiple of 2^32
+        if (v != 0) {
             int lx, ly, lz;
             box_coords(bd, q, lx, ly, lz);
             const int node = (t.bz0 + lz) * ntr + (t.by0 + ly) * m.nx + (t.bx0 + lx);
-            if (hi) atomic_add(&out[node].x, (float)hi * inv);
-            if (lo) atomic_add(&out[node].y, (float)lo * inv);
+            atomic_add(outc + 2 * node + part, (float)v * inv);
         }
     }
 }
@@ -539,17 +548,16 @@ __device__ __forceinline__ void scatter_cells_det(const MeshGeom& m, const TileB
         __syncthreads();
         const int64_t ntr = (int64_t)m.nx * m.ny;
         const BoxDiv bd = box_div(t);
-        for (int q = threadIdx.x; q < nt; q += blockDim.x) {
-            const unsigned long long w[DET_WORDS] = {s_tile[q], s_tile[nt + q], s_tile[2 * nt + q],
-                                                     s_tile[3 * nt + q]};
-            if (w[0] | w[1] | w[2] | w[3]) {
+        // lanes DET_WORDS j .. DET_WORDS j + 3 take the words of node j (one
+        // 32-byte sector): one atomic instruction covers 16 nodes
+        for (int e = threadIdx.x; e < DET_WORDS * nt; e += blockDim.x) {
+            const int q = e / DET_WORDS, j = e % DET_WORDS;
+            const unsigned long long w = s_tile[j * nt + q];
+            if (w) {
                 int lx, ly, lz;
                 box_coords(bd, q, lx, ly, lz);
                 const int64_t node = (int64_t)(t.bz0 + lz) * ntr + (int64_t)(t.by0 + ly) * m.nx + (t.bx0 + lx);
-                unsigned long long* o = out + DET_WORDS * node;
-#pragma unroll
-                for (int j = 0; j < DET_WORDS; ++j)
-                    if (w[j]) atomicAdd(o + j, w[j]);
+                atomicAdd(out + DET_WORDS * node + j, w);
             }
         }
     } else {
