@@ -41,8 +41,9 @@ def stage_bytes(precision: int):
         part = [48 + 8 + 48, 48 + 48 + 8 + 48, 48 + 48 + 8 + 48, 4 * 48 + 8 + 48]
         node = [2 * 16, 3 * 16, 3 * 16, 3 * 16]
         comb, diff = 6 * 16, 14 * 16
-    else:                # mixed: six-vector 32 B (z2, Gamma FP64), chi 4 B, complex 8 B
-        part = [32 + 4 + 32, 32 + 32 + 4 + 32, 32 + 32 + 4 + 32, 4 * 32 + 4 + 32]
+    else:                # mixed: state 32 B (z2, Gamma FP64), derivative k 24 B (all FP32),
+        # chi 4 B, complex 8 B
+        part = [32 + 4 + 24, 32 + 24 + 4 + 24, 32 + 24 + 4 + 24, 32 + 3 * 24 + 4 + 32]
         node = [2 * 8, 3 * 8, 3 * 8, 3 * 8]
         comb, diff = 6 * 8, 14 * 8
     return part, node, comb, diff

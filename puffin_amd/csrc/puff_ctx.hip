@@ -31,8 +31,12 @@ struct DevBuf {
 template <typename R>
 struct Particles {
     PartPtrs<R> y;
-    PartPtrs<R> k[3];
+    KPtrs<R> k[3];
     R* chi = nullptr;
+    // the sort's spare z2 and Gamma arrays in the mixed form (its spare x, y,
+    // p_r, p_i are k[0]'s; the FP64 form's k[0] serves whole)
+    double* z2_spare = nullptr;
+    double* g_spare = nullptr;
 };
 
 constexpr int DN2_TABLE_CAP = 4096;   // taper-table slots of the deferred adjoint
@@ -508,8 +512,21 @@ static int alloc_particles(Ctx* c) {
         PUFF_TRY(dalloc(c, (void**)&q.g, sizeof(double) * c->n));
         return PUFF_OK;
     };
+    auto setk = [&](KPtrs<R>& q) -> int {
+        PUFF_TRY(dalloc(c, (void**)&q.x, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.y, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.z2, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.pr, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.pi, sizeof(R) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&q.g, sizeof(R) * c->n));
+        return PUFF_OK;
+    };
     PUFF_TRY(setp(P.y));
-    for (int k = 0; k < 3; ++k) PUFF_TRY(setp(P.k[k]));
+    for (int k = 0; k < 3; ++k) PUFF_TRY(setk(P.k[k]));
+    if constexpr (sizeof(R) != sizeof(double)) {
+        PUFF_TRY(dalloc(c, (void**)&P.z2_spare, sizeof(double) * c->n));
+        PUFF_TRY(dalloc(c, (void**)&P.g_spare, sizeof(double) * c->n));
+    }
     PUFF_TRY(dalloc(c, (void**)&P.chi, sizeof(R) * c->n));
     PUFF_TRY(dalloc(c, &c->chi_spare, sizeof(R) * c->n));
     PUFF_TRY(dalloc(c, (void**)&c->orig, sizeof(int) * c->n));
@@ -686,7 +703,7 @@ struct CaptureView {
     R* chi;
     int* orig;
     cplx<R>* A0;
-    PartPtrs<R> k[3];
+    KPtrs<R> k[3];
     cplx<R>* dA[3];
 };
 
@@ -710,11 +727,19 @@ static size_t capture_view(const Ctx* c, char* base, CaptureView<R>* v) {
         q.pi = reinterpret_cast<R*>(take(sizeof(R) * n));
         q.g = reinterpret_cast<double*>(take(sizeof(double) * n));
     };
+    auto take_k = [&](KPtrs<R>& q) {
+        q.x = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.y = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.z2 = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.pr = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.pi = reinterpret_cast<R*>(take(sizeof(R) * n));
+        q.g = reinterpret_cast<R*>(take(sizeof(R) * n));
+    };
     take_parts(v->y);
     v->chi = reinterpret_cast<R*>(take(sizeof(R) * n));
     v->orig = reinterpret_cast<int*>(take(sizeof(int) * n));
     v->A0 = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * nm));
-    for (int k = 0; k < 3; ++k) take_parts(v->k[k]);
+    for (int k = 0; k < 3; ++k) take_k(v->k[k]);
     for (int k = 0; k < 3; ++k) v->dA[k] = reinterpret_cast<cplx<R>*>(take(sizeof(cplx<R>) * nm));
     return o;
 }
@@ -730,7 +755,17 @@ static void sort_particles(Ctx* c) {
     a.ncell = c->ncell;
     a.mesh = c->mesh;
     a.y = &P.y;
-    a.spare = &P.k[0];
+    // spare state arrays: k[0]'s, with z2 and Gamma of the state's type
+    KPtrs<R>& k0 = P.k[0];
+    PartPtrs<R> spare{k0.x, k0.y, nullptr, k0.pr, k0.pi, nullptr};
+    if constexpr (sizeof(R) == sizeof(double)) {
+        spare.z2 = reinterpret_cast<double*>(k0.z2);
+        spare.g = reinterpret_cast<double*>(k0.g);
+    } else {
+        spare.z2 = P.z2_spare;
+        spare.g = P.g_spare;
+    }
+    a.spare = &spare;
     a.chi = &P.chi;
     R* spare_chi = reinterpret_cast<R*>(c->chi_spare);
     a.chi_spare = &spare_chi;
@@ -742,6 +777,18 @@ static void sort_particles(Ctx* c) {
     a.perm = c->sort_perm;
     a.tile_sums = c->sort_tiles;
     launch_sort<R>(a, c->stream);
+    // the sort swapped the state and spare pointers: the spares go back
+    k0.x = spare.x;
+    k0.y = spare.y;
+    k0.pr = spare.pr;
+    k0.pi = spare.pi;
+    if constexpr (sizeof(R) == sizeof(double)) {
+        k0.z2 = reinterpret_cast<R*>(spare.z2);
+        k0.g = reinterpret_cast<R*>(spare.g);
+    } else {
+        P.z2_spare = spare.z2;
+        P.g_spare = spare.g;
+    }
     prof_stop(c, slot);
     c->chi_spare = spare_chi;
     c->sorted = true;
@@ -801,7 +848,7 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
     const double cin[4] = {0.0, hh, hh, h};
     const double cnext[4] = {hh, hh, h, 0.0};
     // stage derivatives and sources: the context's buffers, or the capture's
-    PartPtrs<R> K[3] = {P.k[0], P.k[1], P.k[2]};
+    KPtrs<R> K[3] = {P.k[0], P.k[1], P.k[2]};
     cplx<R>* D[4];
     for (int k = 0; k < 4; ++k) D[k] = reinterpret_cast<cplx<R>*>(c->dA[k]);
     if (cap) {
@@ -814,8 +861,8 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
         a.s = sc[st - 1];
         a.cin = cin[st - 1];
         a.cnext = cnext[st - 1];
-        a.kin = (st > 1) ? K[st - 2] : PartPtrs<R>{};
-        a.kout = (st < 4) ? K[st - 1] : PartPtrs<R>{};
+        a.kin = (st > 1) ? K[st - 2] : KPtrs<R>{};
+        a.kout = (st < 4) ? K[st - 1] : KPtrs<R>{};
         if (st == 4) {
             a.k1 = K[0];
             a.k2 = K[1];
@@ -1273,7 +1320,7 @@ static int replay_stages(Ctx* c, const StageScalars* sc, const double* cin, cons
         a.s = sc[st - 1];
         a.cin = cin[st - 1];
         a.cnext = cnext[st - 1];
-        a.kin = (st > 1) ? P.k[st - 2] : PartPtrs<R>{};
+        a.kin = (st > 1) ? P.k[st - 2] : KPtrs<R>{};
         a.kout = P.k[st - 1];
         a.dAin = (st > 1) ? reinterpret_cast<const cplx<R>*>(c->dA[st - 2]) : nullptr;
         a.dAout = reinterpret_cast<cplx<R>*>(c->dA[st - 1]);
@@ -1344,7 +1391,7 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
     PartPtrs<R> Y0 = P.y;
     const R* CHI = P.chi;
     const cplx<R>* A0 = reinterpret_cast<const cplx<R>*>(c->A);
-    PartPtrs<R> K[3] = {P.k[0], P.k[1], P.k[2]};
+    KPtrs<R> K[3] = {P.k[0], P.k[1], P.k[2]};
     const cplx<R>* D[3];
     for (int k = 0; k < 3; ++k) D[k] = reinterpret_cast<const cplx<R>*>(c->dA[k]);
     if (cap) {
@@ -1376,7 +1423,7 @@ static int rk4par_adjoint(Ctx* c, double sz, double h, double* dn2col, int32_t* 
     for (int st = 4; st >= 1; --st) {
         b.s = sc[st - 1];
         b.cin = cin[st - 1];
-        b.kin = (st > 1) ? K[st - 2] : PartPtrs<R>{};
+        b.kin = (st > 1) ? K[st - 2] : KPtrs<R>{};
         b.dAin = (st > 1) ? D[st - 2] : nullptr;
         b.ws = wts[st - 1] * h6;
         b.cnext = (st < 4) ? cin[st] : 0.0;

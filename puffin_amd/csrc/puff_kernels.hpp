@@ -26,16 +26,24 @@ struct Physics {
     int coupling, electrons_evolve, field_evolve;
 };
 
-// Six particle coordinates: x, y, p_r, p_i in R; z2 and Gamma in FP64.
-template <typename R>
+// Six particle coordinates: x, y, p_r, p_i in R; z2 and Gamma in Z (FP64
+// for the state).
+template <typename R, typename Z = double>
 struct PartPtrs {
     R* x;
     R* y;
-    double* z2;
+    Z* z2;
     R* pr;
     R* pi;
-    double* g;
+    Z* g;
 };
+
+// RK4 stage derivatives k_s: all six components in R.  In the mixed form the
+// derivatives of z2 and Gamma are stored in FP32, while the trial states
+// y0 + c k and the step's accumulation y0 + h/6 (k1 + 2 k2 + 2 k3 + k4) of z2
+// and Gamma are formed in FP64 (24 instead of 32 bytes per derivative).
+template <typename R>
+using KPtrs = PartPtrs<R, R>;
 
 // Deterministic deposit of the FP64 form.  Every contribution v = w_k s of a
 // macroparticle to a node is split into two fixed-point integers,
@@ -63,9 +71,9 @@ template <typename R>
 struct StageArgs {
     int64_t n;
     PartPtrs<R> y0;     // state at the start of the step (written by stage 4)
-    PartPtrs<R> kin;    // derivative of the previous stage
-    PartPtrs<R> kout;   // derivative of this stage (stages 1-3)
-    PartPtrs<R> k1, k2; // stage 4: first two derivatives (k3 = kin)
+    KPtrs<R> kin;       // derivative of the previous stage
+    KPtrs<R> kout;      // derivative of this stage (stages 1-3)
+    KPtrs<R> k1, k2;    // stage 4: first two derivatives (k3 = kin)
     const R* chi;
     double cin;         // trial coefficient of this stage (0, h/2, h/2, h)
     double cnext;       // trial coefficient of the next stage
@@ -118,7 +126,7 @@ template <typename R>
 struct AdjArgs {
     int64_t n;
     PartPtrs<R> y0;
-    PartPtrs<R> kin;       // k_{s-1} (stages 2-4)
+    KPtrs<R> kin;          // k_{s-1} (stages 2-4)
     const R* chi;
     double cin;
     const cplx<R>* A0;

@@ -226,18 +226,20 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     }
 
     if (STAGE < 4) {
+        // stored in R (FP32 in the mixed form: see KPtrs)
+        const R kz2s = (R)kz2;
         a.kout.x[i] = kx_;
         a.kout.y[i] = ky_;
-        a.kout.z2[i] = kz2;
+        a.kout.z2[i] = kz2s;
         a.kout.pr[i] = kpr;
         a.kout.pi[i] = kpi;
-        a.kout.g[i] = kg;
+        a.kout.g[i] = (R)kg;
         // next stage's trial state must lie inside the mesh, else the next
-        // derivs call would raise the flag (checked here so that no later
-        // kernel writes state when it does)
+        // derivs call would raise the flag (checked here, on the stored
+        // derivative, so that no later kernel writes state when it does)
         const R cn = (R)a.cnext;
         Cell<R> cc;
-        const int nf = locate<R>(m, fma(cn, kx_, x0), fma(cn, ky_, y0), fma(a.cnext, kz2, z20), cc);
+        const int nf = locate<R>(m, fma(cn, kx_, x0), fma(cn, ky_, y0), fma(a.cnext, (double)kz2s, z20), cc);
         if (nf) atomicOr(a.flags, nf);
     } else {
         // y = (y0 + h/6 (k1 + 2 (k2 + k3))) + h/6 k4  (puffin_mpi_RK4.f90:274-279, 307-312)
