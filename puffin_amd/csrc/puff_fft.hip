@@ -192,6 +192,86 @@ PUFF_CT_PLAN(4096, 0, 16, 16, 16)
 PUFF_CT_PLAN(64, 0, 16, 4)   // the four-step sub-lines of 4096-point z2 lines
 #undef PUFF_CT_PLAN
 
+// Direct DFT pass for an odd prime r (a radix pass of lds_pass), folded:
+// with the pre-twiddled inputs y_j = x_j W^(j k twst), u_j = y_j + y_(r-j),
+// v_j = y_j - y_(r-j) (j = 1 .. (r-1)/2) and w^(jq) = c + i s
+// (w^((r-j)q) = c - i s),
+//   X_q     = y_0 + sum u_j c + i sum v_j s
+//   X_(r-q) = y_0 + sum u_j c - i sum v_j s,
+// four FMAs per input pair for both outputs.  Each work item forms the
+// outputs of NQ consecutive values of q (with their mirrors r - q), sharing
+// the input loads, the folding and the loop bookkeeping: per input pair it
+// reads 2 inputs and NQ rotations from LDS for 4 NQ FMAs.  The pass is bound
+// by LDS reads; q >= (r + 1) / 2 slots of the last item read a valid table
+// entry (q = 0) and store nothing.
+#ifndef PUFF_DFT_WIDE
+#define PUFF_DFT_WIDE 4
+#endif
+#ifndef PUFF_DFT_WIDE_MIN
+#define PUFF_DFT_WIDE_MIN 23
+#endif
+template <typename R, bool IL, int NQ>
+__device__ __forceinline__ void direct_pass(const cplx<R>* src, cplx<R>* dst, int B, int LD, int N, int r,
+                                            int p, const cplx<R>* tw, int sign) {
+    const int T = N / r;
+    const int twst = N / (p * r);
+    const int nr = N / r;
+    const int hq = (r + 1) / 2;
+    const int nqq = (hq + NQ - 1) / NQ;
+    for (int w = threadIdx.x; w < B * T * nqq; w += blockDim.x) {
+        int b, t;
+        if (IL) { b = w % B; t = w / B; } else { t = w % (T * nqq); b = w / (T * nqq); }
+        const int q0 = NQ * (t / T);
+        const int i = t % T;
+        const int k = i % p;
+        const int dk = k * twst;
+        int dq[NQ], f[NQ];
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            dq[m] = (q0 + m < hq) ? (q0 + m) * nr : 0;   // 0: a valid table index, result unused
+            f[m] = dq[m];
+        }
+        const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
+        int e1 = dk, e2 = (r - 1) * dk;   // e1, e2 < N throughout
+        R sr[NQ], si[NQ], tr[NQ], ti[NQ];
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) sr[m] = si[m] = tr[m] = ti[m] = (R)0;
+        constexpr int UJ = NQ >= 4 ? 2 : 4;
+#pragma unroll UJ
+        for (int j = 1; j < hq; ++j) {
+            cplx<R> xa = src[lidx<IL>(b, i + j * T, B, LD)];
+            cplx<R> xb = src[lidx<IL>(b, i + (r - j) * T, B, LD)];
+            if (dk) {
+                xa = cmul(xa, tw_get(tw, e1, sign));
+                xb = cmul(xb, tw_get(tw, e2, sign));
+            }
+            const R ux = xa.x + xb.x, uy = xa.y + xb.y;
+            const R vx = xa.x - xb.x, vy = xa.y - xb.y;
+#pragma unroll
+            for (int m = 0; m < NQ; ++m) {
+                const cplx<R> o = tw_get(tw, f[m], sign);
+                sr[m] = fma(ux, o.x, sr[m]);
+                si[m] = fma(uy, o.x, si[m]);
+                tr[m] = fma(vx, o.y, tr[m]);
+                ti[m] = fma(vy, o.y, ti[m]);
+                f[m] += dq[m];
+                if (f[m] >= N) f[m] -= N;
+            }
+            e1 += dk;
+            e2 -= dk;
+        }
+        const int ob = (i / p) * p * r + k;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            const int q = q0 + m;
+            if (q < hq) {
+                dst[lidx<IL>(b, ob + q * p, B, LD)] = {y0.x + sr[m] - ti[m], y0.y + si[m] + tr[m]};
+                if (q > 0) dst[lidx<IL>(b, ob + (r - q) * p, B, LD)] = {y0.x + sr[m] + ti[m], y0.y + si[m] - tr[m]};
+            }
+        }
+    }
+}
+
 // One radix-r pass (after passes whose radices multiply to p) of an in-LDS
 // mixed-radix transform of length N over elements 0..N-1 of B lines laid
 // out with line length LD (N <= LD; elements N..LD-1 are left alone): src
@@ -255,69 +335,10 @@ __device__ __forceinline__ void lds_pass(const cplx<R>* src, cplx<R>* dst, int B
                 dst[lidx<IL>(b, ob + 3 * p, B, LD)] = csub(a1, wb);
             }
         }
+    } else if (r >= PUFF_DFT_WIDE_MIN) {
+        direct_pass<R, IL, PUFF_DFT_WIDE>(src, dst, B, LD, N, r, p, tw, sign);
     } else {
-        // direct DFT pass for an odd prime r, folded: with the pre-twiddled
-        // inputs y_j = x_j W^(j k twst), u_j = y_j + y_(r-j),
-        // v_j = y_j - y_(r-j) (j = 1 .. (r-1)/2) and w^(jq) = c + i s
-        // (w^((r-j)q) = c - i s),
-        //   X_q     = y_0 + sum u_j c + i sum v_j s
-        //   X_(r-q) = y_0 + sum u_j c - i sum v_j s,
-        // four FMAs per input pair for both outputs.
-        const int nr = N / r;
-        const int hq = (r + 1) / 2;
-        // each work item forms the outputs of two values of q (qa = 2 qq
-        // and qb = qa + 1, with their mirrors r - q), sharing the input
-        // loads, the folding and the loop bookkeeping
-        const int nqq = (hq + 1) / 2;
-        for (int w = threadIdx.x; w < B * T * nqq; w += blockDim.x) {
-            int b, t;
-            if (IL) { b = w % B; t = w / B; } else { t = w % (T * nqq); b = w / (T * nqq); }
-            const int qa = 2 * (t / T);
-            const bool hasb = qa + 1 < hq;
-            const int qb = hasb ? qa + 1 : 0;   // qb = 0: a valid table index, result unused
-            const int i = t % T;
-            const int k = i % p;
-            const int dk = k * twst;
-            const int dqa = qa * nr, dqb = qb * nr;
-            const cplx<R> y0 = src[lidx<IL>(b, i, B, LD)];
-            int e1 = dk, e2 = (r - 1) * dk, fa = dqa, fb = dqb;   // e1, e2 < N throughout
-            R sra = 0, sia = 0, tra = 0, tia = 0;
-            R srb = 0, sib = 0, trb = 0, tib = 0;
-#pragma unroll 4
-            for (int j = 1; j < hq; ++j) {
-                cplx<R> xa = src[lidx<IL>(b, i + j * T, B, LD)];
-                cplx<R> xb = src[lidx<IL>(b, i + (r - j) * T, B, LD)];
-                if (dk) {
-                    xa = cmul(xa, tw_get(tw, e1, sign));
-                    xb = cmul(xb, tw_get(tw, e2, sign));
-                }
-                const R ux = xa.x + xb.x, uy = xa.y + xb.y;
-                const R vx = xa.x - xb.x, vy = xa.y - xb.y;
-                const cplx<R> oa = tw_get(tw, fa, sign);
-                const cplx<R> obw = tw_get(tw, fb, sign);
-                sra = fma(ux, oa.x, sra);
-                sia = fma(uy, oa.x, sia);
-                tra = fma(vx, oa.y, tra);
-                tia = fma(vy, oa.y, tia);
-                srb = fma(ux, obw.x, srb);
-                sib = fma(uy, obw.x, sib);
-                trb = fma(vx, obw.y, trb);
-                tib = fma(vy, obw.y, tib);
-                e1 += dk;
-                e2 -= dk;
-                fa += dqa;
-                if (fa >= N) fa -= N;
-                fb += dqb;
-                if (fb >= N) fb -= N;
-            }
-            const int ob = (i / p) * p * r + k;
-            dst[lidx<IL>(b, ob + qa * p, B, LD)] = {y0.x + sra - tia, y0.y + sia + tra};
-            if (qa > 0) dst[lidx<IL>(b, ob + (r - qa) * p, B, LD)] = {y0.x + sra + tia, y0.y + sia - tra};
-            if (hasb) {
-                dst[lidx<IL>(b, ob + qb * p, B, LD)] = {y0.x + srb - tib, y0.y + sib + trb};
-                dst[lidx<IL>(b, ob + (r - qb) * p, B, LD)] = {y0.x + srb + tib, y0.y + sib - trb};
-            }
-        }
+        direct_pass<R, IL, 2>(src, dst, B, LD, N, r, p, tw, sign);
     }
 }
 
