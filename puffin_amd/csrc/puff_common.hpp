@@ -80,14 +80,15 @@ static_assert(STAGE_THREADS_LOG2 > 0, "stage workgroups of 256 or 512 threads");
 #ifndef PUFF_TILE_NODES_MIXED
 #define PUFF_TILE_NODES_MIXED 3072
 #endif
-// The mixed form's LDS deposit tile: packed fixed point (default) or, with
-// PUFF_MIXED_TILE_FP64 = 1, the FP64 tile of the FP64 form.
-#ifndef PUFF_MIXED_TILE_FP64
-#define PUFF_MIXED_TILE_FP64 0
+// The mixed form's LDS deposit tile: PUFF_MIXED_TILE 0 packed fixed point
+// (default), 1 FP64 (the FP64 form's tile), 2 FP32 (ds_add_f32; kept to
+// measure it against the other two).
+#ifndef PUFF_MIXED_TILE
+#define PUFF_MIXED_TILE 0
 #endif
 template <typename R>
 constexpr int stage_tile_nodes() {
-    return sizeof(R) == 8 ? PUFF_TILE_NODES_FP64 : PUFF_TILE_NODES_MIXED;
+    return sizeof(R) == 8 ? PUFF_TILE_NODES_FP64 : (PUFF_MIXED_TILE == 1 ? 1536 : PUFF_TILE_NODES_MIXED);
 }
 // Node capacity of the stage kernels' LDS copy of the field over the
 // workgroup's node box (one complex per node; larger boxes gather from

@@ -97,7 +97,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // ---- node box of this workgroup's cells and the step's active box (and,
     // for the packed fixed-point tile, the workgroup's largest |s|)
     const int lane = threadIdx.x & 63;
-    constexpr bool PACKED = sizeof(R) == 4 && !PUFF_MIXED_TILE_FP64 && !DET;
+    constexpr bool PACKED = sizeof(R) == 4 && PUFF_MIXED_TILE == 0 && !DET;
     const TileBox tb = workgroup_box<R, PACKED ? 7 : 6>(
         m, ok, c, a.box, PACKED ? __float_as_int(fmaxf(fabsf((float)sr), fabsf((float)si))) : 0);
     const bool use_tile = ph.field_evolve && tile_fits<stage_tile_nodes<R>()>(m, tb);
@@ -125,7 +125,9 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
     // the LDS-wait cycles of ds_add_f64; the packed tile cut the LDS-wait
     // cycles per wave from 398 to 6 and the step by 3-4 %.
     constexpr int TILE = stage_tile_nodes<R>();
-    using D = typename std::conditional<PACKED, unsigned long long, double>::type;
+    using D = typename std::conditional<
+        PACKED, unsigned long long,
+        typename std::conditional<sizeof(R) == 4 && PUFF_MIXED_TILE == 2, float, double>::type>::type;
     constexpr int WORDS = DET ? DET_WORDS : (PACKED ? 1 : 2);
     static_assert(WORDS * FIELD_TILE_NODES * sizeof(D) + FIELD_TILE_NODES * sizeof(cplx<R>) <=
                       WORDS * TILE * sizeof(D), "field tile inside the deposit tile");
