@@ -16,8 +16,11 @@
  * private grids reduced in thread order, so results do not depend on thread
  * timing (they do depend on the thread count, at the rounding level).
  *
- * How each part is pinned (tests/test_oracle.py, test_lattice.py,
- * test_adjoint_oracle.py):
+ * How each part is pinned (tests/test_oracle.py, test_oracle_pins.py,
+ * test_oracle_ref.py, test_lattice.py, test_adjoint_oracle.py):
+ *   the reference's own compiled getP2 / getGamma and constants (oracle/_ref,
+ *   built from utilities/gamma_to_p2.f90 and common/puffin_constants.f90):
+ *     p2 and Gamma bit for bit, pi, c and q_e;
  *   the reference's own 1-D run (test/expected/f1main_expected_*.h5):
  *     p2, main-section b-fields, 1-D interpolation and deposit, equations,
  *     RK4, taper, matchIn/matchOut and correctTrans (elliptical), power,
@@ -29,11 +32,13 @@
  *   getGamma inverse, chicane slip and delay, quad inverse, modulation,
  *   corrector means, matching round trips and the plane-pole and curved
  *   entry offsets), slice bounds and the periodic wrap, the adjoint
- *   (central differences, dot-product identity).
- * Parity unpinned against reference outputs (none exist and the reference
- * cannot be built here): the 3-D integrator, diffraction and FFT numbers,
- * the undulator-end b-field branches and the curved-pole b-fields; these
- * rest on the closed forms and invariants above.
+ *   (central differences, dot-product identity);
+ *   physics: the 3-D main-section and ramp b-fields of every undulator type
+ *   (curl-free up to the reference's stated departures) and the bz coupling
+ *   of the equations (natural-focusing betatron closed form).
+ * Parity unpinned against reference outputs (none exist and the path's
+ * reference cannot be built here): the 3-D integrator, diffraction and FFT
+ * numbers; these rest on the closed forms and invariants above.
  */
 #include <math.h>
 #include <stdint.h>
@@ -52,9 +57,11 @@
 /* ------------------------------------------------------------------------ */
 /* p2 from Gamma and p_perp: getP2, gamma_to_p2.f90:84-85                    */
 /* ------------------------------------------------------------------------ */
+/* gamma0**2 * gamma**2 is (gamma0 gamma0)(gamma gamma), as Fortran forms
+   it; the compiled reference routine agrees bit for bit (test_oracle_ref). */
 static inline double orc_p2(double gam, double pr, double pi, double eta,
                             double gamma_r, double aw) {
-    return ((1.0 / sqrt(1.0 - 1.0 / (gamma_r * gamma_r * gam * gam) *
+    return ((1.0 / sqrt(1.0 - 1.0 / ((gamma_r * gamma_r) * (gam * gam)) *
                                   (1.0 + aw * aw * (pr * pr + pi * pi)))) -
             1.0) / eta;
 }
