@@ -369,6 +369,78 @@ def adjoint_bench(steps_check: bool = True):
         drv.close()
 
 
+def c1_gradient(precision: int = 0, fd_rel: float = 1e-4):
+    """d(integrated power)/d(per-period taper, seed amplitude) of the whole
+    C1 module (129 x 129 x 1024, 2^24 macroparticles, 1920 steps on
+    UndSection's schedule), the workload the brief quotes its <= 4x gradient
+    cost on.  The linear taper of -1% over 64 periods enters as a 64-value
+    per-period table (the ramp at period centres, as n2col per period).
+    Checkpoints every 30 steps; each segment is replayed with step captures
+    before it is reversed.  Times a plain forward, the gradient, and central
+    differences of the seed amplitude and of one table entry; the device
+    memory is read while every checkpoint, capture and adjoint buffer is
+    live."""
+    import torch
+    from puffin_amd import gpu
+    from puffin_amd.adjoint import ModuleGradient, module_program
+    from puffin_amd.configs import case_c1
+    from puffin_amd.params import PI_REF
+    c = case_c1()
+    mp = c.mp
+    seed = 1.0e-3                      # the Gaussian seed's amplitude (Im part)
+    shape = (c.field[0] / seed, c.field[1] / seed)
+    period = 4.0 * PI_REF * mp.rho
+    periods = c.info["periods"]
+    table = c.n2col0 + c.undgrad * period * (np.arange(periods) + 0.5)
+    ops = module_program(c.nsteps, c.h, c.steps4diff, diffraction=True, periodic=False)
+    name = "fp64" if precision == 0 else "mixed"
+    out = dict(workload=WORKLOADS["c1"] + f", whole 64-period module (1920 steps), {name}",
+               taper="-1% over 64 periods as a 64-value per-period table (ramp at period centres)")
+    drv = gpu.Integrator(mp, c.n, precision)
+    try:
+        drv.upload_particles(c.particles)
+        drv.upload_field(np.zeros(mp.nmesh), np.zeros(mp.nmesh))
+        nc = drv.capture_bytes()
+        eng = ModuleGradient(drv, ops, shape, every=30, taper_period=period, capture=True,
+                             capture_budget=30 * nc, keep_all=False)
+        eng.evaluate(table, seed)           # warm-up
+        drv.synchronize()
+        t0 = time.perf_counter()
+        J = eng.evaluate(table, seed)
+        t1 = time.perf_counter()
+        eng.forward(table, seed)
+        t2 = time.perf_counter()
+        g = eng.backward()
+        t3 = time.perf_counter()
+        out.update(J=J, forward_s=t1 - t0, forward_checkpointed_s=t2 - t1, backward_s=t3 - t2,
+                   gradient_cost_vs_forward=(t3 - t1) / (t1 - t0),
+                   memory_plan="checkpoints every 30 steps, each segment replayed with step captures",
+                   checkpoint_bytes=eng.memory_bytes(), capture_bytes_per_step=nc,
+                   peak_device_bytes=eng.peak_device_bytes, grad_seed=g["seed"],
+                   grad_table_first=float(g["table"][0]), grad_table_last=float(g["table"][-1]))
+        log(f"c1 gradient ({name}): forward {t1 - t0:.2f} s, gradient {t3 - t1:.2f} s "
+            f"({out['gradient_cost_vs_forward']:.2f}x), {eng.peak_device_bytes / 2**30:.1f} GiB in use")
+        fd_s = ((eng.evaluate(table, seed * (1 + fd_rel)) - eng.evaluate(table, seed * (1 - fd_rel)))
+                / (2 * fd_rel * seed))
+        k = int(np.argmax(np.abs(g["table"])))
+        hk = 1e-6
+        tp, tm = table.copy(), table.copy()
+        tp[k] += hk
+        tm[k] -= hk
+        fd_t = (eng.evaluate(tp, seed) - eng.evaluate(tm, seed)) / (2 * hk)
+        out["fd_check"] = dict(seed_rel_step=fd_rel, fd_seed=fd_s,
+                               seed_rel_err=abs(fd_s - g["seed"]) / abs(g["seed"]),
+                               table_index=k, table_h=hk, fd_table=fd_t,
+                               table_rel_err=abs(fd_t - g["table"][k]) / np.abs(g["table"]).max())
+        log(f"c1 gradient ({name}): dJ/dseed {g['seed']:.6e} (fd {fd_s:.6e}), "
+            f"dJ/dtable[{k}] {g['table'][k]:.6e} (fd {fd_t:.6e})")
+        del eng, g
+        torch.cuda.empty_cache()
+    finally:
+        drv.close()
+    return out
+
+
 def device_memory_used() -> int:
     import torch
     free, total = torch.cuda.mem_get_info()
@@ -675,6 +747,8 @@ def main():
                     help="only the large FP32-mixed workload (its JSON object on stdout)")
     ap.add_argument("--c4-module", action="store_true",
                     help="only the whole 3840-step module of the large workload (JSON on stdout)")
+    ap.add_argument("--c1-gradient", action="store_true",
+                    help="only the gradient of the whole C1 module (FP64; JSON on stdout)")
     ap.add_argument("--c4-gradient", action="store_true",
                     help="only the gradient of the whole large module (binomial checkpointing; JSON on stdout)")
     args = ap.parse_args()
@@ -692,6 +766,9 @@ def main():
         return
     if args.c4_gradient:
         print(json.dumps(c4_full_gradient()), flush=True)
+        return
+    if args.c1_gradient:
+        print(json.dumps(c1_gradient()), flush=True)
         return
     if args.diffraction_only:
         print(json.dumps(diffraction_bench()), flush=True)
@@ -784,6 +861,11 @@ def main():
         line["mixed"]["accuracy"] = precision_report(case, acc_steps)
     if not args.no_adjoint and args.case == "c1":
         line["adjoint"] = adjoint_bench()
+        try:
+            line["adjoint_c1"] = c1_gradient(0)
+        except Exception as e:   # reported in the line, the C1 measurement stands
+            log(f"c1 gradient failed: {e!r}")
+            line["adjoint_c1"] = dict(error=repr(e))
     if not args.no_c4 and args.case == "c1":
         try:
             line["c4"] = c4_bench()
