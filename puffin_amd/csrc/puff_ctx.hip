@@ -869,6 +869,13 @@ static int rk4par(Ctx* c, double sz, double h, char* cap = nullptr, void* cap_hd
         }
         a.dAin = (st > 1) ? D[st - 2] : nullptr;
         a.dAout = D[st - 1];
+        if (cap && st == 1) {
+            a.ycap = v.y;
+            a.chicap = v.chi;
+        } else {
+            a.ycap = PartPtrs<R>{};
+            a.chicap = nullptr;
+        }
         slot = prof_start(c, st - 1);
         launch_stage<R>(st, a, s);
         if constexpr (sizeof(R) == 8) {
@@ -1712,14 +1719,14 @@ static void capture_start(Ctx* c, char* cap, void* cap_hdr, CaptureView<R>& v, c
     hipStream_t s = c->stream;
     launch_store_words(reinterpret_cast<uint32_t*>(cap), words, (int)(sizeof(BlobHeader) / 4), s);
     const size_t n = (size_t)c->n;
+    // the start state and chi are written by stage 1 (StageArgs::ycap); the
+    // slot order and the field are copied here
     CopySections cs{};
-    const void* src[9] = {P.y.x, P.y.y, P.y.z2, P.y.pr, P.y.pi, P.y.g, P.chi, c->orig, c->A};
-    void* dst[9] = {v.y.x, v.y.y, v.y.z2, v.y.pr, v.y.pi, v.y.g, v.chi, v.orig, v.A0};
-    const size_t bytes[9] = {sizeof(R) * n, sizeof(R) * n, sizeof(double) * n, sizeof(R) * n, sizeof(R) * n,
-                             sizeof(double) * n, sizeof(R) * n, sizeof(int) * n,
-                             sizeof(cplx<R>) * (size_t)c->nmesh()};
-    cs.n = 9;
-    for (int k = 0; k < 9; ++k) {
+    const void* src[2] = {c->orig, c->A};
+    void* dst[2] = {v.orig, v.A0};
+    const size_t bytes[2] = {sizeof(int) * n, sizeof(cplx<R>) * (size_t)c->nmesh()};
+    cs.n = 2;
+    for (int k = 0; k < 2; ++k) {
         cs.src[k] = reinterpret_cast<const uint8_t*>(src[k]);
         cs.dst[k] = reinterpret_cast<uint8_t*>(dst[k]);
         cs.bytes[k] = (int64_t)bytes[k];

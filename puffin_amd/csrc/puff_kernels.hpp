@@ -87,6 +87,11 @@ struct StageArgs {
     int* flags;
     Box* box;
     DetState det;       // acc == nullptr: the atomic FP64 deposit
+    // stage 1 of a captured step: the start state and chi also go to the
+    // capture (x == nullptr: not captured), written from the registers the
+    // stage loads them into instead of by a separate copy
+    PartPtrs<R> ycap;
+    R* chicap;
 };
 
 template <typename R>

@@ -49,6 +49,15 @@ __global__ __launch_bounds__(STAGE_THREADS) void k_stage(StageArgs<R> a) {
         chi = a.chi[i];
         if (STAGE == 1) {
             x = x0; y = y0; z2 = z20; pr = pr0; pi = pi0; g = g0;
+            if (a.ycap.x) {
+                a.ycap.x[i] = x0;
+                a.ycap.y[i] = y0;
+                a.ycap.z2[i] = z20;
+                a.ycap.pr[i] = pr0;
+                a.ycap.pi[i] = pi0;
+                a.ycap.g[i] = g0;
+                a.chicap[i] = chi;
+            }
         } else {
             const R c = (R)a.cin;
             const R k3x = a.kin.x[i], k3y = a.kin.y[i], k3pr = a.kin.pr[i], k3pi = a.kin.pi[i];
