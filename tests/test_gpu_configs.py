@@ -13,6 +13,8 @@
   stated per output (power, bunching, on-axis spectrum, particles).
 * C3: the gradient of the full 2^20-macroparticle, 480-step module against
   central differences of the forward.
+* C1 whole module (1920 steps) in both forms: it completes, and the mixed
+  form's integrated power is within 1e-4 of the FP64 form's.
 * C4: the full 2^27-macroparticle beam on the 257 x 257 x 4096 mesh (mixed):
   charge conservation, a flag-free module start, finite power.
 """
@@ -216,6 +218,39 @@ def test_c1_full_beam_properties(precision):
     assert case.n == 1 << 24
     check_charge_conservation(case, precision, 1e-12 if precision == FP64 else 1e-5)
     check_module_start(case, precision, 60)
+
+
+def test_c1_whole_module_in_both_forms():
+    """The whole 64-period C1 module (2^24 macroparticles, 1920 steps, 65
+    diffraction steps on UndSection's schedule) completes in both forms (the
+    reference STOPs when a macroparticle leaves the temporal mesh), the
+    power grows by two orders of magnitude, and the FP32-mixed form's final
+    integrated power is within the brief's 1e-4 of the FP64 form's (measured:
+    8e-8); power(z2) and bunching within 1e-4 of the largest FP64 value."""
+    case = case_c1()
+    assert case.n == 1 << 24 and case.nsteps == 1920
+    res = {}
+    for prec in (FP64, MIXED):
+        out, ar, ai, power, bunch, nsl, lam = run_module_gpu(case, prec, case.nsteps)
+        assert np.isfinite(power).all() and np.isfinite(bunch).all()
+        res[prec] = (power, bunch)
+    mp = case.mp
+    # power(z2) of the seed: transverse trapezoid of |A|^2 per plane
+    f0 = (case.field[0] ** 2 + case.field[1] ** 2).reshape(mp.nz2, mp.ny, mp.nx)
+    wx = np.full(mp.nx, mp.dx)
+    wx[[0, -1]] *= 0.5
+    wy = np.full(mp.ny, mp.dy)
+    wy[[0, -1]] *= 0.5
+    p_seed = np.einsum("zyx,y,x->z", f0, wy, wx)
+    p64, b64 = res[FP64]
+    p32, b32 = res[MIXED]
+    ip64, ip32 = float(np.trapezoid(p64)), float(np.trapezoid(p32))
+    gain = ip64 / float(np.trapezoid(p_seed))
+    say(f"C1 whole module: integrated power FP64 {ip64:.6e} (gain {gain:.1f}), mixed {ip32:.6e} "
+        f"(rel {abs(ip32 - ip64) / ip64:.2e}); power(z) {relerr(p32, p64):.2e}, |b| {relerr(b32, b64):.2e}")
+    assert gain > 100.0
+    assert abs(ip32 - ip64) / ip64 < 1e-4
+    assert relerr(p32, p64) < 1e-4 and relerr(b32, b64) < 1e-4
 
 
 def test_c4_full_beam_properties():
