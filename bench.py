@@ -210,11 +210,13 @@ def cpu_baseline(case, nsample: int, steps: int):
                         f"FP64 C oracle with OpenMP atomics"))
 
 
-def accuracy_run(case, precision: int, steps: int):
+def accuracy_run(case, precision: int, steps: int, progress_every: int | None = None):
     """The module's schedule (UndSection: first half diffraction step, rk4par,
     merged diffraction every iSteps4Diff steps, last half step) for `steps`
     steps; returns the final power(z2), slice bunching |b| and the on-axis
-    spectrum."""
+    spectrum.  With progress_every, a write every that many steps logs the
+    step (UndSection then splits the diffraction step there, as it does
+    around its writes, in both forms alike)."""
     from puffin_amd import gpu
     from puffin_amd.undulator import und_section
     mp = case.mp
@@ -226,8 +228,13 @@ def accuracy_run(case, precision: int, steps: int):
         if case.taper_table is not None:
             g.set_taper_table(case.taper_table, 0.0, case.taper_period)
         t0 = time.perf_counter()
+        wr = None
+        if progress_every:
+            def wr(istep):
+                log(f"accuracy ({'fp64' if precision == 0 else 'mixed'}): step {istep}, "
+                    f"{time.perf_counter() - t0:.1f} s")
         res = und_section(g, steps, case.h, diffraction=True, steps4diff=case.steps4diff,
-                          periodic=bool(mp.periodic))
+                          periodic=bool(mp.periodic), write=wr, write_every=progress_every)
         g.synchronize()
         wall = time.perf_counter() - t0
         if res.status != 0:
@@ -249,12 +256,12 @@ def accuracy_run(case, precision: int, steps: int):
                 diffractions=res.n_diffractions)
 
 
-def precision_report(case, steps: int):
+def precision_report(case, steps: int, progress_every: int | None = None):
     """Error of the FP32-mixed form against FP64 per output after `steps`
     steps of the module's schedule."""
     log(f"accuracy: {steps} steps FP64 and mixed")
-    r64 = accuracy_run(case, 0, steps)
-    r32 = accuracy_run(case, 1, steps)
+    r64 = accuracy_run(case, 0, steps, progress_every)
+    r32 = accuracy_run(case, 1, steps, progress_every)
 
     def err(k):
         a, b = r32[k], r64[k]
@@ -747,6 +754,8 @@ def main():
                     help="only the large FP32-mixed workload (its JSON object on stdout)")
     ap.add_argument("--c4-module", action="store_true",
                     help="only the whole 3840-step module of the large workload (JSON on stdout)")
+    ap.add_argument("--c4-accuracy", action="store_true",
+                    help="only the FP32-mixed against FP64 error of the whole 3840-step large module (JSON on stdout)")
     ap.add_argument("--c1-gradient", action="store_true",
                     help="only the gradient of the whole C1 module (FP64; JSON on stdout)")
     ap.add_argument("--c4-gradient", action="store_true",
@@ -769,6 +778,13 @@ def main():
         return
     if args.c1_gradient:
         print(json.dumps(c1_gradient()), flush=True)
+        return
+    if args.c4_accuracy:
+        from puffin_amd.configs import case_c4
+        c4 = case_c4()
+        out = precision_report(c4, c4.nsteps, progress_every=480)
+        out["workload"] = "planar FEL, temporal mesh 257x257x4096, 2^27 macroparticles, whole 128-period module"
+        print(json.dumps(out), flush=True)
         return
     if args.diffraction_only:
         print(json.dumps(diffraction_bench()), flush=True)
