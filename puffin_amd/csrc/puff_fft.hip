@@ -272,6 +272,119 @@ __device__ __forceinline__ void direct_pass(const cplx<R>* src, cplx<R>* dst, in
     }
 }
 
+// LDS read of a complex value at a byte offset from base, as one 2-vector
+// whose alignment is its size: one ds_read_b128 (FP64) or ds_read_b64
+// (FP32) whose 16-bit offset field takes the compile-time part of the
+// address.  A cplx<double> read (8-byte aligned) becomes ds_read2_b64,
+// whose offsets reach only 2040 bytes.
+template <typename R>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    typedef double __attribute__((ext_vector_type(2))) T;
+};
+template <>
+struct Vec2<float> {
+    typedef float __attribute__((ext_vector_type(2))) T;
+};
+template <typename R>
+__device__ __forceinline__ cplx<R> lds_at(const cplx<R>* base, uint32_t byte_off) {
+    const typename Vec2<R>::T v =
+        *reinterpret_cast<const typename Vec2<R>::T*>(reinterpret_cast<const char*>(base) + byte_off);
+    return {v.x, v.y};
+}
+
+// direct_pass for a compile-time radix RR of a compile-time line length N,
+// after passes whose radices multiply to P (same arithmetic, bit for bit):
+// - the j loop has a compile-time trip count and unrolls in groups;
+// - the rotations w^(jq) are read from the forward table and the sign
+//   applied to the s sums at the end (X_q takes +-i sum v s; negation is
+//   exact and commutes with the FMA rounding), instead of a conditional
+//   negate per rotation;
+// - each rotation's table offset is kept in bytes and advanced by q nr with
+//   one unsigned min for the wrap at N (a + d and a + d - N: the one below N
+//   is the smaller as unsigned), instead of add, compare, select and an
+//   address computation.
+template <typename R, bool IL, int NQ, int N, int RR, int P>
+__device__ __forceinline__ void direct_pass_ct(const cplx<R>* src, cplx<R>* dst, int B, const cplx<R>* tw,
+                                               int sign) {
+    constexpr int T = N / RR;
+    constexpr int twst = N / (P * RR);
+    constexpr int nr = N / RR;
+    constexpr int hq = (RR + 1) / 2;
+    constexpr int nqq = (hq + NQ - 1) / NQ;
+    constexpr uint32_t ES = sizeof(cplx<R>);
+    constexpr uint32_t WRAP = (uint32_t)N * ES;
+    for (int w = threadIdx.x; w < B * T * nqq; w += blockDim.x) {
+        int b, t;
+        if (IL) { b = w % B; t = w / B; } else { t = w % (T * nqq); b = w / (T * nqq); }
+        const int q0 = NQ * (t / T);
+        const int i = t % T;
+        const int k = i % P;
+        const int dk = k * twst;
+        uint32_t ofs[NQ], stp[NQ], stw[NQ];
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            // q >= hq: offset 0 throughout (a valid table entry, result unused)
+            stp[m] = (q0 + m < hq) ? (uint32_t)((q0 + m) * nr) * ES : 0u;
+            stw[m] = stp[m] - WRAP;
+            ofs[m] = stp[m];
+        }
+        const uint32_t la = (uint32_t)lidx<IL>(b, i, B, N) * ES;
+        const cplx<R> y0 = lds_at(src, la);
+        R sr[NQ], si[NQ], tr[NQ], ti[NQ];
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) sr[m] = si[m] = tr[m] = ti[m] = (R)0;
+        // unrolled in groups (a full unroll hoists every LDS read and takes
+        // ~490 VGPRs); the groups' reads take their offsets from the
+        // instruction
+        constexpr int UJ = (hq - 1) <= 6 ? (hq - 1) : ((hq - 1) % 3 == 0 ? 3 : 2);
+#pragma unroll UJ
+        for (int j = 1; j < hq; ++j) {
+            cplx<R> o[NQ];
+#pragma unroll
+            for (int m = 0; m < NQ; ++m) {
+                o[m] = lds_at(tw, ofs[m]);
+                ofs[m] = min(ofs[m] + stp[m], ofs[m] + stw[m]);
+            }
+            // interleaved lines: element i + j T of line b sits j T B
+            // elements after element i
+            cplx<R> xa = IL ? lds_at(src, la + (uint32_t)(j * T * B) * ES)
+                            : lds_at(src, (uint32_t)lidx<IL>(b, i + j * T, B, N) * ES);
+            cplx<R> xb = IL ? lds_at(src, la + (uint32_t)((RR - j) * T * B) * ES)
+                            : lds_at(src, (uint32_t)lidx<IL>(b, i + (RR - j) * T, B, N) * ES);
+            if constexpr (P > 1) {
+                if (dk) {   // exponents j dk, (RR - j) dk < N
+                    xa = cmul(xa, tw_get(tw, j * dk, sign));
+                    xb = cmul(xb, tw_get(tw, (RR - j) * dk, sign));
+                }
+            }
+            const R ux = xa.x + xb.x, uy = xa.y + xb.y;
+            const R vx = xa.x - xb.x, vy = xa.y - xb.y;
+#pragma unroll
+            for (int m = 0; m < NQ; ++m) {
+                sr[m] = fma(ux, o[m].x, sr[m]);
+                si[m] = fma(uy, o[m].x, si[m]);
+                tr[m] = fma(vx, o[m].y, tr[m]);
+                ti[m] = fma(vy, o[m].y, ti[m]);
+            }
+        }
+        const int ob = (i / P) * P * RR + k;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            if (sign > 0) {
+                tr[m] = -tr[m];
+                ti[m] = -ti[m];
+            }
+            const int q = q0 + m;
+            if (q < hq) {
+                dst[lidx<IL>(b, ob + q * P, B, N)] = {y0.x + sr[m] - ti[m], y0.y + si[m] + tr[m]};
+                if (q > 0) dst[lidx<IL>(b, ob + (RR - q) * P, B, N)] = {y0.x + sr[m] + ti[m], y0.y + si[m] - tr[m]};
+            }
+        }
+    }
+}
+
 // One radix-r pass (after passes whose radices multiply to p) of an in-LDS
 // mixed-radix transform of length N over elements 0..N-1 of B lines laid
 // out with line length LD (N <= LD; elements N..LD-1 are left alone): src
@@ -369,7 +482,13 @@ __device__ __forceinline__ cplx<R>* ct_passes(cplx<R>* src, cplx<R>* dst, int B,
         return src;
     } else {
         constexpr int r = CtPlan<NC>::r[K];
-        lds_pass<R, IL>(src, dst, B, LD, N, r, P, tw, sign);
+        constexpr int NN = CtPlan<NC>::rader ? NC - 1 : NC;   // the transform's length
+        if constexpr (r != 2 && r != 3 && r != 4 && r != 16 && !CtPlan<NC>::rader) {
+            constexpr int nq = r >= PUFF_DFT_WIDE_MIN ? PUFF_DFT_WIDE : 2;
+            direct_pass_ct<R, IL, nq, NN, r, P>(src, dst, B, tw, sign);
+        } else {
+            lds_pass<R, IL>(src, dst, B, LD, N, r, P, tw, sign);
+        }
         __syncthreads();
         return ct_passes<R, IL, NC, K + 1, P * r>(dst, src, B, LD, N, tw, sign);
     }
