@@ -65,6 +65,7 @@ struct Ctx {
     double* kx = nullptr;
     double* ky = nullptr;
     double* kz = nullptr;
+    double* kzr = nullptr;   // 1/(2 kz), 1/sqrt|2 kz| per kz (the mixed form's spectral factor)
     double* mxy = nullptr;
     double* mz = nullptr;
     FftAxis ax{}, ay{}, az{};
@@ -635,6 +636,15 @@ static int create(Ctx* c) {
         PUFF_HIP(hipMemcpy(c->kx, kx.data(), sizeof(double) * kx.size(), hipMemcpyHostToDevice));
         PUFF_HIP(hipMemcpy(c->ky, ky.data(), sizeof(double) * ky.size(), hipMemcpyHostToDevice));
         PUFF_HIP(hipMemcpy(c->kz, kz.data(), sizeof(double) * kz.size(), hipMemcpyHostToDevice));
+        std::vector<double> kzr(2 * kz.size(), 0.0);
+        for (size_t i = 0; i < kz.size(); ++i) {
+            if (kz[i] != 0.0) {
+                kzr[2 * i] = 1.0 / (2.0 * kz[i]);
+                kzr[2 * i + 1] = 1.0 / std::sqrt(std::fabs(2.0 * kz[i]));
+            }
+        }
+        PUFF_TRY(dalloc(c, (void**)&c->kzr, sizeof(double) * kzr.size()));
+        PUFF_HIP(hipMemcpy(c->kzr, kzr.data(), sizeof(double) * kzr.size(), hipMemcpyHostToDevice));
         PUFF_HIP(hipMemcpy(c->mxy, mxy.data(), sizeof(double) * mxy.size(), hipMemcpyHostToDevice));
         PUFF_HIP(hipMemcpy(c->mz, mz.data(), sizeof(double) * mz.size(), hipMemcpyHostToDevice));
     }
@@ -945,6 +955,7 @@ static int diffraction(Ctx* c, double h) {
     a.kx = c->kx;
     a.ky = c->ky;
     a.kz = c->kz;
+    a.kzr = c->kzr;
     a.mxy = c->mxy;
     a.mz = c->mz;
     a.nx = c->p.nx;
@@ -1503,6 +1514,7 @@ static int diffraction_adjoint(Ctx* c, double h) {
     a.kx = c->kx;
     a.ky = c->ky;
     a.kz = c->kz;
+    a.kzr = c->kzr;
     a.mxy = c->mxy;
     a.mz = c->mz;
     a.nx = c->p.nx;

@@ -45,14 +45,24 @@ namespace puff {
 #ifndef PUFF_LDS_PAD_X
 #define PUFF_LDS_PAD_X 1
 #endif
-template <bool IL>
-__host__ __device__ constexpr int lpad(int e) { return (!IL && PUFF_LDS_PAD_X) ? e + (e >> 4) : e; }
+//
+// SW (interleaved buffers of the z2 passes, 0: none) swizzles instead of
+// padding: flat index e becomes e ^ ((e >> (SW & 255)) & (SW >> 8)).  With
+// B lines interleaved, the first radix-16 pass writes lane i's outputs 16 B
+// elements apart (512 bytes for two FP64 or four FP32 lines: one LDS bank);
+// XOR-ing the block number 16 B (shift log2(16 B)) into the low bits, within
+// the block, puts the lanes on different banks.  A padded buffer would need
+// 3 % more LDS, one workgroup per CU fewer at 32 KiB.
+template <bool IL, int SW = 0>
+__host__ __device__ constexpr int lpad(int e) {
+    return (!IL && PUFF_LDS_PAD_X) ? e + (e >> 4) : ((IL && SW) ? e ^ ((e >> (SW & 255)) & (SW >> 8)) : e);
+}
 
 // LDS index of element j of line b.  Interleaved layout (b fastest) keeps
 // the strided y / z2 loads conflict free; line-major suits contiguous x lines.
-template <bool IL>
+template <bool IL, int SW = 0>
 __device__ __forceinline__ int lidx(int b, int j, int B, int N) {
-    return lpad<IL>(IL ? j * B + b : b * N + j);
+    return lpad<IL, SW>(IL ? j * B + b : b * N + j);
 }
 
 // Line staging between global memory and LDS with U loads (or stores) per
@@ -68,7 +78,7 @@ constexpr int STAGE_UNROLL = 8;
 #endif
 
 // dst[lpad<IL>(w)] = get(w) for 0 <= w < n.
-template <bool IL, int U, typename R, typename F>
+template <bool IL, int U, int SW = 0, typename R, typename F>
 __device__ __forceinline__ void lds_fill(cplx<R>* dst, int n, F get) {
     for (int w0 = threadIdx.x; w0 < n; w0 += U * (int)blockDim.x) {
         cplx<R> v[U];
@@ -80,7 +90,7 @@ __device__ __forceinline__ void lds_fill(cplx<R>* dst, int n, F get) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int w = w0 + u * (int)blockDim.x;
-            if (w < n) dst[lpad<IL>(w)] = v[u];
+            if (w < n) dst[lpad<IL, SW>(w)] = v[u];
         }
     }
 }
@@ -89,14 +99,14 @@ __device__ __forceinline__ void lds_fill(cplx<R>* dst, int n, F get) {
 // (element 0 stands in past n), so that v stays in registers: with
 // conditional reads the compiler kept v in scratch memory (144 bytes per
 // lane in FP64).
-template <bool IL, int U, typename R, typename F>
+template <bool IL, int U, int SW = 0, typename R, typename F>
 __device__ __forceinline__ void lds_drain(const cplx<R>* src, int n, F put) {
     for (int w0 = threadIdx.x; w0 < n; w0 += U * (int)blockDim.x) {
         cplx<R> v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int w = w0 + u * (int)blockDim.x;
-            v[u] = src[lpad<IL>(w < n ? w : 0)];
+            v[u] = src[lpad<IL, SW>(w < n ? w : 0)];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -586,7 +596,7 @@ __device__ __forceinline__ void small_radix(cplx<R>* v, int r, int sign) {
 // into registers, the workgroup passes a barrier, and the outputs overwrite
 // the same buffer.  Half the LDS of the ping-pong form, hence more
 // workgroups per CU.
-template <typename R, bool IL, int KB = 1>
+template <typename R, bool IL, int KB = 1, int SW = 0>
 __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, int p, const cplx<R>* tw,
                                              int sign, int LD) {
     // KB butterflies per thread: all their inputs are read into registers
@@ -611,7 +621,7 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
             if (r == 16) {
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, LD)];
+                    v[it][j] = buf[lidx<IL, SW>(b[it], i + j * T, B, LD)];
                     if (j > 0 && k > 0) v[it][j] = cmul(v[it][j], tw_get(tw, j * k * twst, sign));
                 }
                 dft16(v[it], sign);
@@ -619,7 +629,7 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     if (j < r) {
-                        v[it][j] = buf[lidx<IL>(b[it], i + j * T, B, LD)];
+                        v[it][j] = buf[lidx<IL, SW>(b[it], i + j * T, B, LD)];
                         if (j > 0 && k > 0) v[it][j] = cmul(v[it][j], tw_get(tw, j * k * twst, sign));
                     }
                 }
@@ -636,11 +646,11 @@ __device__ __forceinline__ void inplace_pass(cplx<R>* buf, int B, int N, int r, 
                 for (int q0 = 0; q0 < 4; ++q0)
 #pragma unroll
                     for (int q1 = 0; q1 < 4; ++q1)
-                        buf[lidx<IL>(b[it], ob[it] + (q0 + 4 * q1) * p, B, LD)] = v[it][4 * q0 + q1];
+                        buf[lidx<IL, SW>(b[it], ob[it] + (q0 + 4 * q1) * p, B, LD)] = v[it][4 * q0 + q1];
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (q < r) buf[lidx<IL>(b[it], ob[it] + q * p, B, LD)] = v[it][q];
+                    if (q < r) buf[lidx<IL, SW>(b[it], ob[it] + q * p, B, LD)] = v[it][q];
             }
         }
     }
@@ -665,15 +675,33 @@ constexpr int inplace_kb(int B, int N, int r, int threads) {
 
 // The passes of CtPlan<NC> in place over BC lines stored LD apart, with THR
 // threads per workgroup.
-template <typename R, bool IL, int NC, int BC, int LD = NC, int THR = PUFF_Z_THREADS, int K = 0, int P = 1>
+template <typename R, bool IL, int NC, int BC, int LD = NC, int THR = PUFF_Z_THREADS, int SW = 0, int K = 0,
+          int P = 1>
 __device__ __forceinline__ void ct_passes_inplace(cplx<R>* buf, const cplx<R>* tw, int sign) {
     if constexpr (K < CtPlan<NC>::np) {
         constexpr int r = CtPlan<NC>::r[K];
         constexpr int KB = inplace_kb(BC, NC, r, THR);
-        inplace_pass<R, IL, KB>(buf, BC, NC, r, P, tw, sign, LD);
+        inplace_pass<R, IL, KB, SW>(buf, BC, NC, r, P, tw, sign, LD);
         __syncthreads();
-        ct_passes_inplace<R, IL, NC, BC, LD, THR, K + 1, P * r>(buf, tw, sign);
+        ct_passes_inplace<R, IL, NC, BC, LD, THR, SW, K + 1, P * r>(buf, tw, sign);
     }
+}
+
+// The z2 buffers' swizzle (lpad's SW) for B interleaved lines of N points
+// of R: B a power of two whose 16 B elements fill whole 256-byte rows of
+// LDS banks, so that the first radix-16 pass's writes would land on one
+// bank; the block number goes into the low bits.  N a multiple of 16, so
+// that the buffer is whole blocks and the swizzle permutes within it.
+#ifndef PUFF_Z_SWIZZLE
+#define PUFF_Z_SWIZZLE 1
+#endif
+template <typename R, int B, int N>
+constexpr int z_swizzle() {
+    if (!PUFF_Z_SWIZZLE || B <= 0 || (B & (B - 1)) != 0 || N % 16 != 0) return 0;
+    if ((16 * B * (int)sizeof(cplx<R>)) % 256 != 0) return 0;
+    int sh = 0;
+    while ((1 << sh) < 16 * B) ++sh;
+    return sh | ((16 * B - 1) << 8);
 }
 
 template <typename R, bool IL, int NC>
@@ -708,13 +736,35 @@ __device__ __forceinline__ void stage_rader_perm(int* rps, const FftAxis& ax) {
     }
 }
 
+// The middle of the in-place Rader transform, between the permutations:
+// buf holds a_j = x[g^j] at element j < M and x_0 at element M; on return
+// element q < M holds c_q (X[g^-q] = x_0 + c_q, x_0 still at element M) and
+// x0s[b] holds X_0.
+template <typename R, bool IL, int NC, int BC>
+__device__ __forceinline__ void rader_ip_core(cplx<R>* buf, cplx<R>* x0s, const FftAxis& ax, const cplx<R>* tw,
+                                              int sign) {
+    constexpr int N = NC, M = N - 1;
+    constexpr int THR = PUFF_RADER_THREADS;
+    const cplx<R>* rb = reinterpret_cast<const cplx<R>*>(ax.rb) + (sign > 0 ? M : 0);
+    ct_passes_inplace<R, IL, M, BC, N, THR>(buf, tw, -1);
+    // C = A . rb_s (rb carries 1/M); X_0 = x_0 + A_0
+    for (int w = threadIdx.x; w < BC * M; w += THR) {
+        int b, k;
+        if (IL) { b = w % BC; k = w / BC; } else { k = w % M; b = w / M; }
+        const cplx<R> v = buf[lidx<IL>(b, k, BC, N)];
+        if (k == 0) x0s[b] = cadd(buf[lidx<IL>(b, M, BC, N)], v);
+        buf[lidx<IL>(b, k, BC, N)] = cmul(v, rb[k]);
+    }
+    __syncthreads();
+    ct_passes_inplace<R, IL, M, BC, N, THR>(buf, tw, +1);
+}
+
 template <typename R, bool IL, int NC, int BC>
 __device__ __forceinline__ void lds_rader_ip(cplx<R>* buf, cplx<R>* x0s, const int* rps, const FftAxis& ax,
                                              const cplx<R>* tw, int sign) {
     constexpr int N = NC, M = N - 1;
     constexpr int THR = PUFF_RADER_THREADS;
     constexpr int KP = (BC * N + THR - 1) / THR;   // elements per thread in a permutation
-    const cplx<R>* rb = reinterpret_cast<const cplx<R>*>(ax.rb) + (sign > 0 ? M : 0);
     // a_j = x[g^j] at element j < M; x_0 to element M
     {
         cplx<R> v[KP];
@@ -738,17 +788,7 @@ __device__ __forceinline__ void lds_rader_ip(cplx<R>* buf, cplx<R>* x0s, const i
         }
         __syncthreads();
     }
-    ct_passes_inplace<R, IL, M, BC, N, THR>(buf, tw, -1);
-    // C = A . rb_s (rb carries 1/M); X_0 = x_0 + A_0
-    for (int w = threadIdx.x; w < BC * M; w += THR) {
-        int b, k;
-        if (IL) { b = w % BC; k = w / BC; } else { k = w % M; b = w / M; }
-        const cplx<R> v = buf[lidx<IL>(b, k, BC, N)];
-        if (k == 0) x0s[b] = cadd(buf[lidx<IL>(b, M, BC, N)], v);
-        buf[lidx<IL>(b, k, BC, N)] = cmul(v, rb[k]);
-    }
-    __syncthreads();
-    ct_passes_inplace<R, IL, M, BC, N, THR>(buf, tw, +1);
+    rader_ip_core<R, IL, NC, BC>(buf, x0s, ax, tw, sign);
     // X[g^-q] = x_0 + c_q; X_0 to element 0
     {
         cplx<R> v[KP];
@@ -796,7 +836,7 @@ template <typename R, bool INPLACE, int NC, int BC>
 __device__ __forceinline__ cplx<R>* z_fft(cplx<R>* src, cplx<R>* dst, int B, const FftAxis& ax,
                                           const cplx<R>* tw, int sign) {
     if constexpr (INPLACE) {
-        if constexpr (NC > 0) ct_passes_inplace<R, true, NC, BC>(src, tw, sign);
+        if constexpr (NC > 0) ct_passes_inplace<R, true, NC, BC, NC, PUFF_Z_THREADS, z_swizzle<R, BC, NC>()>(src, tw, sign);
         else lds_passes_inplace<R, true>(src, B, ax.n, ax.npass, ax.radix, tw, sign);
         return src;
     } else {
@@ -1016,6 +1056,43 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) 
         return;
     }
     cplx<R>* base = a.a + (int64_t)iz * a.nx * a.ny + x0;
+    if constexpr (rader_ip<R, NC>()) {
+        // Rader's permutations folded into the global reads and writes: the
+        // permutation moves whole rows of B x-columns, so the accesses stay
+        // as contiguous as the plain ones, and the transform saves two LDS
+        // round trips and their barriers
+        constexpr int M = NC - 1;
+        const int* rps = reinterpret_cast<const int*>(x0s + BC);
+        __syncthreads();   // rps staged
+        lds_fill<true, STAGE_UNROLL>(s0, BC * NC, [&](int w) {
+            const int b = w % BC, k = w / BC;
+            const int j = k < M ? rps[k] : 0;   // element k < M: x[g^k]; element M: x_0
+            return (b < nb) ? base[(int64_t)j * a.nx + b] : cplx<R>{0, 0};
+        });
+        __syncthreads();
+        rader_ip_core<R, true, NC, BC>(s0, x0s, ay, tw, sign);
+        for (int w0 = threadIdx.x; w0 < BC * NC; w0 += STAGE_UNROLL * (int)blockDim.x) {
+            cplx<R> v[STAGE_UNROLL];
+            int j[STAGE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < STAGE_UNROLL; ++u) {
+                const int w = w0 + u * (int)blockDim.x;
+                const int ww = w < BC * NC ? w : 0;
+                const int b = ww % BC, q = ww / BC;
+                const cplx<R> c = s0[lidx<true>(b, q < M ? q : 0, BC, NC)];
+                const cplx<R> xz = s0[lidx<true>(b, M, BC, NC)];
+                v[u] = q < M ? cadd(xz, c) : x0s[b];   // X[g^-q] = x_0 + c_q; X_0
+                j[u] = q < M ? rps[NC + q] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE_UNROLL; ++u) {
+                const int w = w0 + u * (int)blockDim.x;
+                const int b = w % BC;
+                if (w < BC * NC && b < nb) base[(int64_t)j[u] * a.nx + b] = v[u];
+            }
+        }
+        return;
+    }
     lds_fill<true, STAGE_UNROLL>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
         return (b < nb) ? base[(int64_t)j * a.nx + b] : cplx<R>{0, 0};
@@ -1035,10 +1112,12 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) 
 // modulo 2 pi there (Cody-Waite, with 2 pi split into a double and its
 // remainder), so that the FP32 sine and cosine see |r| <= pi; the damping
 // exponent is formed in FP64 and exponentiated in FP32.  FP64 sine, cosine
-// and exponential per node cost C4's event 1.2 ms of 22.7.
+// and exponential per node cost C4's event 1.2 ms of 22.7.  The mixed form
+// also takes 1/(2 kz) and 1/sqrt|2 kz| from a per-wavenumber table (kzr)
+// instead of an FP64 divide and square root per node.
 template <typename R>
 __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, double kx, double ky, double kz,
-                                               cplx<R> v) {
+                                               int iz, cplx<R> v) {
     if constexpr (sizeof(R) == 4) {
         if (op == SPEC_PROPAGATE) {
             if ((kz > a.cutoff) || (kz < -a.cutoff)) {
@@ -1046,7 +1125,7 @@ __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, dou
                     constexpr double two_pi_hi = 6.283185307179586232;      // 2 pi rounded to double
                     constexpr double two_pi_lo = 2.4492935982947064e-16;    // 2 pi - two_pi_hi
                     constexpr double inv_two_pi = 0.15915494309189533577;
-                    const double th = a.h * (kx * kx + ky * ky) / (2.0 * kz);
+                    const double th = a.h * (kx * kx + ky * ky) * a.kzr[2 * iz];
                     const double n = rint(th * inv_two_pi);
                     const double r = fma(-n, two_pi_lo, fma(-n, two_pi_hi, th));
                     float sn, cs;
@@ -1057,7 +1136,7 @@ __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, dou
                 v = {0.0f, 0.0f};
             }
         } else if (kz != 0.0) {
-            const float f = expf((float)(-a.h * a.beta * (fabs(kx) + fabs(ky)) / (sqrt(fabs(2.0 * kz)))));
+            const float f = expf((float)(-a.h * a.beta * (fabs(kx) + fabs(ky)) * a.kzr[2 * iz + 1]));
             v = {f * v.x, f * v.y};
         }
         return v;
@@ -1107,7 +1186,9 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
     // few lines per workgroup here: the twiddles stay in global memory (L2)
     const cplx<R>* tw = reinterpret_cast<const cplx<R>*>(az.tw);
     cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
-    lds_fill<true, PUFF_Z_UNROLL>(s0, B * N, [&](int w) {
+    // the in-place compile-time passes' swizzled buffer (identity otherwise)
+    constexpr int SW = (INPLACE && NC > 0) ? z_swizzle<R, BC, NC>() : 0;
+    lds_fill<true, PUFF_Z_UNROLL, SW>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
         return (b < nb) ? base[(int64_t)j * ntr + b] : cplx<R>{0, 0};
     });
@@ -1121,13 +1202,13 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w % B, q = w / B;
             if (b >= nb) continue;
-            res[lpad<true>(w)] = spectral_op(a, op, a.kx[x0 + b], ky, a.kz[q], res[lpad<true>(w)]);
+            res[lpad<true, SW>(w)] = spectral_op(a, op, a.kx[x0 + b], ky, a.kz[q], q, res[lpad<true, SW>(w)]);
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
         res = z_fft<R, INPLACE, NC, BC>(res, other, B, az, tw, +1);
     }
-    lds_drain<true, PUFF_Z_UNROLL>(res, B * N, [&](int w, const cplx<R>& v) {
+    lds_drain<true, PUFF_Z_UNROLL, SW>(res, B * N, [&](int w, const cplx<R>& v) {
         const int b = w % B, j = w / B;
         if (b < nb) base[(int64_t)j * ntr + b] = v;
     });
@@ -1179,26 +1260,28 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z4(DiffArgs<R> a, int op
     cplx<R>* base = a.a + row;
     // plane of element j of set s
     auto plane = [](int s, int j) { return PASS == Z4_MID ? M * s + j : M * j + s; };
-    lds_fill<true, (L * M) / PUFF_Z_THREADS>(buf, L * M, [&](int w) {
+    constexpr int SW = z_swizzle<R, L, M>();
+    lds_fill<true, (L * M) / PUFF_Z_THREADS, SW>(buf, L * M, [&](int w) {
         const int b = w % L, j = w / L;
         const int xi = b % BX, si = b / BX;
         return xi < nb ? base[(int64_t)plane(sg + si, j) * ntr + xi] : cplx<R>{0, 0};
     });
     __syncthreads();
     const bool fused = PASS == Z4_MID && op >= 0;
-    ct_passes_inplace<R, true, M, L>(buf, tw, fused ? -1 : sign);
+    ct_passes_inplace<R, true, M, L, M, PUFF_Z_THREADS, SW>(buf, tw, fused ? -1 : sign);
     if (fused) {
         const double ky = a.ky[iy];
         for (int w = threadIdx.x; w < L * M; w += blockDim.x) {
             const int b = w % L, k2 = w / L;
             const int xi = b % BX, si = b / BX;
             if (xi >= nb) continue;
-            buf[lpad<true>(w)] = spectral_op(a, op, a.kx[x0 + xi], ky, a.kz[(sg + si) + M * k2], buf[lpad<true>(w)]);
+            const int iz = (sg + si) + M * k2;
+            buf[lpad<true, SW>(w)] = spectral_op(a, op, a.kx[x0 + xi], ky, a.kz[iz], iz, buf[lpad<true, SW>(w)]);
         }
         __syncthreads();
-        ct_passes_inplace<R, true, M, L>(buf, tw, +1);
+        ct_passes_inplace<R, true, M, L, M, PUFF_Z_THREADS, SW>(buf, tw, +1);
     }
-    lds_drain<true, (L * M) / PUFF_Z_THREADS>(buf, L * M, [&](int w, const cplx<R>& v) {
+    lds_drain<true, (L * M) / PUFF_Z_THREADS, SW>(buf, L * M, [&](int w, const cplx<R>& v) {
         const int b = w % L, j = w / L;
         const int xi = b % BX, si = b / BX;
         if (xi >= nb) return;

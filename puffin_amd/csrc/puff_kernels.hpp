@@ -306,6 +306,7 @@ struct DiffArgs {
     const double* kx;    // GetKValues
     const double* ky;
     const double* kz;
+    const double* kzr;   // per kz: 1/(2 kz), 1/sqrt|2 kz| (0 at kz = 0)
     const double* mxy;   // transverse mask
     const double* mz;    // z2 mask
     int nx, ny, nz2, nzp, periodic, filter;
