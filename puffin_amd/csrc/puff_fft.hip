@@ -869,6 +869,92 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int sign) 
     // plane and row of the block's first line; line L0 + b is row iy0 + b
     // counted on from there (b < B)
     const int iy0 = (int)(L0 % a.ny), iz0 = (int)(L0 / a.ny);
+    const double rff = 1.0 / a.ffact;
+    if constexpr (rader_ip<R, NC>() && (OP == XOP_FORWARD || OP == XOP_INV_SPLIT_FWD || OP == XOP_INV_RECOMBINE)) {
+        // Rader's permutations folded into the global reads and writes (as
+        // in the y pass; here they scatter within each line's 2 KB): the
+        // input permutation into the loads, the output permutation into the
+        // stores, and for the split, the inverse transform's output
+        // permutation and the forward transform's input permutation into
+        // one move within LDS (natural index g^-q = g^(M - q): element q
+        // goes to element (M - q) mod M, X_0 to element M)
+        constexpr int M = NC - 1;
+        constexpr int THR = PUFF_RADER_THREADS;
+        constexpr int KP = (BC * NC + THR - 1) / THR;
+        const int* rps = reinterpret_cast<const int*>(x0s + BC);
+        __syncthreads();   // rps staged
+        lds_fill<false, STAGE_UNROLL>(s0, BC * NC, [&](int w) {
+            const int b = w / NC, k = w % NC;
+            const int j = k < M ? rps[k] : 0;   // element k < M: x[g^k]; element M: x_0
+            return b < nb ? base[b * NC + j] : cplx<R>{0, 0};
+        });
+        __syncthreads();
+        rader_ip_core<R, false, NC, BC>(s0, x0s, ax, tw, OP == XOP_FORWARD ? sign : +1);
+        if constexpr (OP == XOP_INV_SPLIT_FWD) {
+            // /ffact then AbsorptionStep's split (diffraction.f90:310, 454-470)
+            cplx<R>* anb = a.anb + L0 * N;
+            cplx<R> v[KP];
+            int e[KP];
+            const int tid = opaque_tid();
+#pragma unroll
+            for (int u = 0; u < KP; ++u) {
+                const int w = tid + u * THR;
+                const int ww = w < BC * NC ? w : 0;
+                const int b = ww / NC, q = ww % NC;
+                const cplx<R> c = s0[lidx<false>(b, q < M ? q : 0, BC, NC)];
+                const cplx<R> xz = s0[lidx<false>(b, M, BC, NC)];
+                const cplx<R> x = q < M ? cadd(xz, c) : x0s[b];   // X[g^-q] = x_0 + c_q; X_0
+                const int n = q < M ? rps[NC + q] : 0;
+                e[u] = lidx<false>(b, q < M ? (q ? M - q : 0) : M, BC, NC);
+                const int lb = iy0 + b;   // (iy, iz) of line L0 + b
+                const int iy = lb % a.ny, iz = iz0 + lb / a.ny;
+                const bool st = w < BC * NC && b < nb;
+                const double mxy = st ? a.mxy[(int64_t)iy * a.nx + n] : 0.0;
+                const double m = mxy + ((st ? a.mz[iz] : 0.0) * (1.0 - mxy));
+                const double vr = (double)x.x * rff, vi = (double)x.y * rff;
+                if (st) anb[b * NC + n] = {(R)((1.0 - m) * vr), (R)((1.0 - m) * vi)};
+                v[u] = {(R)(m * vr), (R)(m * vi)};
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < KP; ++u)
+                if (tid + u * THR < BC * NC) s0[e[u]] = v[u];
+            __syncthreads();
+            rader_ip_core<R, false, NC, BC>(s0, x0s, ax, tw, -1);
+        }
+        const cplx<R>* anb = a.anb + L0 * N;
+        for (int w0 = threadIdx.x; w0 < BC * NC; w0 += STAGE_UNROLL * THR) {
+            cplx<R> v[STAGE_UNROLL];
+            int n[STAGE_UNROLL];
+#pragma unroll
+            for (int u = 0; u < STAGE_UNROLL; ++u) {
+                const int w = w0 + u * THR;
+                const int ww = w < BC * NC ? w : 0;
+                const int b = ww / NC, q = ww % NC;
+                const cplx<R> c = s0[lidx<false>(b, q < M ? q : 0, BC, NC)];
+                const cplx<R> xz = s0[lidx<false>(b, M, BC, NC)];
+                v[u] = q < M ? cadd(xz, c) : x0s[b];   // X[g^-q] = x_0 + c_q; X_0
+                n[u] = q < M ? rps[NC + q] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < STAGE_UNROLL; ++u) {
+                const int w = w0 + u * THR;
+                const int b = w / NC;
+                if (w >= BC * NC || b >= nb) continue;
+                cplx<R> x = v[u];
+                if constexpr (OP == XOP_INV_RECOMBINE) {
+                    // /ffact + interior (diffraction.f90:513-517)
+                    const cplx<R> qn = anb[b * NC + n[u]];
+                    x = {(R)((double)x.x * rff) + qn.x, (R)((double)x.y * rff) + qn.y};
+                    // keep the aliased plane NZ2 equal to plane 1 (para_field.f90:3197-3244)
+                    const int64_t L = L0 + b;
+                    if (a.periodic && L < a.ny) a.a[(int64_t)(a.nz2 - 1) * ntr + L * N + n[u]] = x;
+                }
+                base[b * NC + n[u]] = x;
+            }
+        }
+        return;
+    }
     if (op == XOP_ADJ_FOLD_SAVE_FWD) {
         for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
             const int b = w / N;
@@ -894,8 +980,7 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int sign) 
     __syncthreads();
     const int first_sign = (op == XOP_FORWARD) ? sign : (op == XOP_ADJ_FOLD_SAVE_FWD ? -1 : +1);
     // the backward transform's 1/ffact: a divide in FP64 (the reference's
-    // arithmetic), a reciprocal multiply in the mixed form
-    const double rff = 1.0 / a.ffact;
+    // arithmetic), a reciprocal multiply (rff) in the mixed form
     auto unscale = [&](R v) -> double {
         if constexpr (sizeof(R) == 8) return (double)v / a.ffact;
         else return (double)v * rff;

@@ -5,6 +5,7 @@ FP64 parity with the oracle is held at 1e-10 relative to the largest value.
 """
 from __future__ import annotations
 
+import functools
 import math
 
 import numpy as np
@@ -163,6 +164,46 @@ def test_diffraction_step_four_step_lines_match_oracle(periodic, precision):
     tol = 1e-12 if precision == FP64 else 2e-5
     assert relerr(gr, o.ar) < tol
     assert relerr(gi, o.ai) < tol
+
+
+@functools.lru_cache(maxsize=None)
+def _prime_case(periodic: int):
+    """A 257 x 257 mesh with 8 z2 planes (and the aliased one) and the
+    oracle's diffraction step of it."""
+    nz2 = 9 if periodic else 8
+    mp = ModuleParams(nx=257, ny=257, nz2=nz2, dx=0.1, dy=0.1, dz2=0.01, rho=0.01, eta=1e-4,
+                      kappa=0.5, gamma_r=100.0, aw=1.0, periodic=periodic, nbx=40, nby=40, nbz2=3,
+                      filt_cutoff=15.0, beta=1.0)
+    rng = np.random.default_rng(31 + periodic)
+    ar = rng.standard_normal(mp.nmesh)
+    ai = rng.standard_normal(mp.nmesh)
+    if periodic:
+        ntr = mp.ntr
+        ar[(nz2 - 1) * ntr:] = ar[:ntr]
+        ai[(nz2 - 1) * ntr:] = ai[:ntr]
+    dummy = dict(x=np.zeros(1), y=np.zeros(1), z2=np.zeros(1), px=np.zeros(1), py=np.zeros(1),
+                 gam=np.ones(1), chi=np.zeros(1))
+    o = orc.Oracle(mp, dummy, (ar, ai), nthreads=8)
+    o.diffraction_step(0.0837)
+    return mp, ar, ai, o.ar, o.ai
+
+
+@pytest.mark.parametrize("periodic", [0, 1])
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_diffraction_step_prime_lines_match_oracle(periodic, precision):
+    """257-point transverse lines (C4's axes) take Rader's algorithm in the x
+    and y passes; in the mixed form in place, with the permutations folded
+    into the passes' global reads and writes, in the mask split between two
+    transforms and in the recombination."""
+    print(f"[gpu] diffraction 257 x 257 lines periodic={periodic} precision={precision}", flush=True)
+    mp, ar, ai, oar, oai = _prime_case(periodic)
+    with gpu.Integrator(mp, 1, precision) as g:
+        g.upload_field(ar, ai)
+        g.diffraction_step(0.0837)
+        gr, gi = g.download_field()
+    tol = 1e-12 if precision == FP64 else 2e-5
+    assert relerr(gr, oar) < tol
+    assert relerr(gi, oai) < tol
 
 
 # ---------------------------------------------------------------------------
