@@ -1198,11 +1198,12 @@ __global__ __launch_bounds__(256) void k_fft_y(DiffArgs<R> a, int Bp, int sign) 
 // remainder), so that the FP32 sine and cosine see |r| <= pi; the damping
 // exponent is formed in FP64 and exponentiated in FP32.  FP64 sine, cosine
 // and exponential per node cost C4's event 1.2 ms of 22.7.  The mixed form
-// also takes 1/(2 kz) and 1/sqrt|2 kz| from a per-wavenumber table (kzr)
-// instead of an FP64 divide and square root per node.
+// also takes rz, 1/(2 kz) (propagator) or 1/sqrt|2 kz| (damping), from a
+// per-wavenumber table (kzr) instead of an FP64 divide and square root per
+// node; the FP64 form ignores it.
 template <typename R>
 __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, double kx, double ky, double kz,
-                                               int iz, cplx<R> v) {
+                                               double rz, cplx<R> v) {
     if constexpr (sizeof(R) == 4) {
         if (op == SPEC_PROPAGATE) {
             if ((kz > a.cutoff) || (kz < -a.cutoff)) {
@@ -1210,7 +1211,7 @@ __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, dou
                     constexpr double two_pi_hi = 6.283185307179586232;      // 2 pi rounded to double
                     constexpr double two_pi_lo = 2.4492935982947064e-16;    // 2 pi - two_pi_hi
                     constexpr double inv_two_pi = 0.15915494309189533577;
-                    const double th = a.h * (kx * kx + ky * ky) * a.kzr[2 * iz];
+                    const double th = a.h * (kx * kx + ky * ky) * rz;
                     const double n = rint(th * inv_two_pi);
                     const double r = fma(-n, two_pi_lo, fma(-n, two_pi_hi, th));
                     float sn, cs;
@@ -1221,7 +1222,7 @@ __device__ __forceinline__ cplx<R> spectral_op(const DiffArgs<R>& a, int op, dou
                 v = {0.0f, 0.0f};
             }
         } else if (kz != 0.0) {
-            const float f = expf((float)(-a.h * a.beta * (fabs(kx) + fabs(ky)) * a.kzr[2 * iz + 1]));
+            const float f = expf((float)(-a.h * a.beta * (fabs(kx) + fabs(ky)) * rz));
             v = {f * v.x, f * v.y};
         }
         return v;
@@ -1273,6 +1274,24 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
     cplx<R>* base = a.a + (int64_t)iy * a.nx + x0;
     // the in-place compile-time passes' swizzled buffer (identity otherwise)
     constexpr int SW = (INPLACE && NC > 0) ? z_swizzle<R, BC, NC>() : 0;
+    // FP64 compile-time plans: kx and the kz of the thread's elements for the
+    // spectral op, loaded with the lines (a load per element in the spectral
+    // loop waits on the memory latency each time); the thread's elements
+    // share one x-column
+    constexpr int KPT = (sizeof(R) == 8 && NC > 0 && BC > 0 && (BC * NC) % PUFF_Z_THREADS == 0 &&
+                         PUFF_Z_THREADS % BC == 0)
+                            ? (BC * NC) / PUFF_Z_THREADS
+                            : 0;
+    double kzp[KPT > 0 ? KPT : 1];
+    double kxp = 0.0;
+    if constexpr (KPT > 0) {
+        if (op >= 0) {
+            const int b = (int)threadIdx.x % BC;
+            kxp = a.kx[x0 + (b < nb ? b : 0)];
+#pragma unroll
+            for (int it = 0; it < KPT; ++it) kzp[it] = a.kz[((int)threadIdx.x + it * PUFF_Z_THREADS) / BC];
+        }
+    }
     lds_fill<true, PUFF_Z_UNROLL, SW>(s0, B * N, [&](int w) {
         const int b = w % B, j = w / B;
         return (b < nb) ? base[(int64_t)j * ntr + b] : cplx<R>{0, 0};
@@ -1284,10 +1303,19 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z(DiffArgs<R> a, int Bp,
     } else {
         res = z_fft<R, INPLACE, NC, BC>(s0, s1, B, az, tw, -1);
         const double ky = a.ky[iy];
-        for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
-            const int b = w % B, q = w / B;
-            if (b >= nb) continue;
-            res[lpad<true, SW>(w)] = spectral_op(a, op, a.kx[x0 + b], ky, a.kz[q], q, res[lpad<true, SW>(w)]);
+        if constexpr (KPT > 0) {
+#pragma unroll
+            for (int it = 0; it < KPT; ++it) {
+                const int w = (int)threadIdx.x + it * PUFF_Z_THREADS;
+                if (w % BC < nb) res[lpad<true, SW>(w)] = spectral_op(a, op, kxp, ky, kzp[it], 0.0, res[lpad<true, SW>(w)]);
+            }
+        } else {
+            for (int w = threadIdx.x; w < B * N; w += blockDim.x) {
+                const int b = w % B, q = w / B;
+                if (b >= nb) continue;
+                const double rz = sizeof(R) == 4 ? a.kzr[2 * q + (op == SPEC_ABSORB ? 1 : 0)] : 0.0;
+                res[lpad<true, SW>(w)] = spectral_op(a, op, a.kx[x0 + b], ky, a.kz[q], rz, res[lpad<true, SW>(w)]);
+            }
         }
         __syncthreads();
         cplx<R>* other = (res == s0) ? s1 : s0;
@@ -1343,6 +1371,25 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z4(DiffArgs<R> a, int op
     const int64_t ntr = (int64_t)a.nx * a.ny;
     const int64_t row = (int64_t)iy * a.nx + x0;
     cplx<R>* base = a.a + row;
+    // the spectral op's kz and rz of the workgroup's z2 wavenumbers
+    // (sg + si) + 64 k2, entry si + SETS k2, staged with the line loads (a
+    // load per element in the spectral loop waits on the memory latency
+    // each time)
+    double* kzt = reinterpret_cast<double*>(tw + M);
+    const bool fused = PASS == Z4_MID && op >= 0;
+    // a thread's elements w = tid + 256 u share b = tid mod L (L divides
+    // 256), hence its x-column and kx
+    static_assert(PUFF_Z_THREADS % L == 0, "one x-column per thread");
+    const int xt = ((int)threadIdx.x % L) % BX;
+    double kxt = 0.0;
+    if (fused) {
+        kxt = a.kx[x0 + (xt < nb ? xt : 0)];
+        for (int w = threadIdx.x; w < SETS * M; w += blockDim.x) {
+            const int iz = (sg + w % SETS) + M * (w / SETS);
+            kzt[w] = a.kz[iz];
+            kzt[SETS * M + w] = sizeof(R) == 4 ? a.kzr[2 * iz + (op == SPEC_ABSORB ? 1 : 0)] : 0.0;
+        }
+    }
     // plane of element j of set s
     auto plane = [](int s, int j) { return PASS == Z4_MID ? M * s + j : M * j + s; };
     constexpr int SW = z_swizzle<R, L, M>();
@@ -1352,7 +1399,6 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z4(DiffArgs<R> a, int op
         return xi < nb ? base[(int64_t)plane(sg + si, j) * ntr + xi] : cplx<R>{0, 0};
     });
     __syncthreads();
-    const bool fused = PASS == Z4_MID && op >= 0;
     ct_passes_inplace<R, true, M, L, M, PUFF_Z_THREADS, SW>(buf, tw, fused ? -1 : sign);
     if (fused) {
         const double ky = a.ky[iy];
@@ -1360,8 +1406,8 @@ __global__ __launch_bounds__(PUFF_Z_THREADS) void k_fft_z4(DiffArgs<R> a, int op
             const int b = w % L, k2 = w / L;
             const int xi = b % BX, si = b / BX;
             if (xi >= nb) continue;
-            const int iz = (sg + si) + M * k2;
-            buf[lpad<true, SW>(w)] = spectral_op(a, op, a.kx[x0 + xi], ky, a.kz[iz], iz, buf[lpad<true, SW>(w)]);
+            const int e = si + SETS * k2;
+            buf[lpad<true, SW>(w)] = spectral_op(a, op, kxt, ky, kzt[e], kzt[SETS * M + e], buf[lpad<true, SW>(w)]);
         }
         __syncthreads();
         ct_passes_inplace<R, true, M, L, M, PUFF_Z_THREADS, SW>(buf, tw, +1);
@@ -1616,7 +1662,8 @@ static void launch_z4(const DiffArgs<R>& a, int op, int sign, cplx<R>* out, hipS
     constexpr int SETS = PUFF_Z4_SETS;
     constexpr int L = BX * SETS;
     static_assert((L * Z4_M) % PUFF_Z_THREADS == 0, "whole loads per thread");
-    const size_t sh = ((size_t)lpad<true>(L * Z4_M) + Z4_M) * sizeof(cplx<R>);
+    // line buffer, twiddles, the middle pass's kz and rz
+    const size_t sh = ((size_t)lpad<true>(L * Z4_M) + Z4_M) * sizeof(cplx<R>) + 2 * SETS * Z4_M * sizeof(double);
     const int nxb = (a.nx + BX - 1) / BX;
     const int64_t blocks = xcd_grid((int64_t)nxb * (Z4_M / SETS) * a.ny);
     lds_opt_in(reinterpret_cast<const void*>(&k_fft_z4<R, BX, SETS, PASS>), sh);
