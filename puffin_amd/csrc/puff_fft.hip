@@ -883,6 +883,20 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int sign) 
         constexpr int KP = (BC * NC + THR - 1) / THR;
         const int* rps = reinterpret_cast<const int*>(x0s + BC);
         __syncthreads();   // rps staged
+        // the recombination's interior part, read with the lines (read in
+        // the final loop, each load waits on the memory latency there)
+        cplx<R> anbp[OP == XOP_INV_RECOMBINE ? KP : 1];
+        if constexpr (OP == XOP_INV_RECOMBINE) {
+            const int tid = opaque_tid();
+#pragma unroll
+            for (int u = 0; u < KP; ++u) {
+                const int w = tid + u * THR;
+                const int ww = w < BC * NC ? w : 0;
+                const int b = ww / NC, q = ww % NC;
+                const int n = q < M ? rps[NC + q] : 0;
+                anbp[u] = (w < BC * NC && b < nb) ? a.anb[(L0 + b) * N + n] : cplx<R>{0, 0};
+            }
+        }
         lds_fill<false, STAGE_UNROLL>(s0, BC * NC, [&](int w) {
             const int b = w / NC, k = w % NC;
             const int j = k < M ? rps[k] : 0;   // element k < M: x[g^k]; element M: x_0
@@ -922,13 +936,13 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int sign) 
             __syncthreads();
             rader_ip_core<R, false, NC, BC>(s0, x0s, ax, tw, -1);
         }
-        const cplx<R>* anb = a.anb + L0 * N;
-        for (int w0 = threadIdx.x; w0 < BC * NC; w0 += STAGE_UNROLL * THR) {
-            cplx<R> v[STAGE_UNROLL];
-            int n[STAGE_UNROLL];
+        {
+            cplx<R> v[KP];
+            int n[KP];
+            const int tid = opaque_tid();
 #pragma unroll
-            for (int u = 0; u < STAGE_UNROLL; ++u) {
-                const int w = w0 + u * THR;
+            for (int u = 0; u < KP; ++u) {
+                const int w = tid + u * THR;
                 const int ww = w < BC * NC ? w : 0;
                 const int b = ww / NC, q = ww % NC;
                 const cplx<R> c = s0[lidx<false>(b, q < M ? q : 0, BC, NC)];
@@ -937,14 +951,14 @@ __global__ __launch_bounds__(256) void k_fft_x(DiffArgs<R> a, int Bp, int sign) 
                 n[u] = q < M ? rps[NC + q] : 0;
             }
 #pragma unroll
-            for (int u = 0; u < STAGE_UNROLL; ++u) {
-                const int w = w0 + u * THR;
+            for (int u = 0; u < KP; ++u) {
+                const int w = tid + u * THR;
                 const int b = w / NC;
                 if (w >= BC * NC || b >= nb) continue;
                 cplx<R> x = v[u];
                 if constexpr (OP == XOP_INV_RECOMBINE) {
                     // /ffact + interior (diffraction.f90:513-517)
-                    const cplx<R> qn = anb[b * NC + n[u]];
+                    const cplx<R> qn = anbp[u];
                     x = {(R)((double)x.x * rff) + qn.x, (R)((double)x.y * rff) + qn.y};
                     // keep the aliased plane NZ2 equal to plane 1 (para_field.f90:3197-3244)
                     const int64_t L = L0 + b;
