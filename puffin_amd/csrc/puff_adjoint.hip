@@ -65,42 +65,72 @@ __global__ __launch_bounds__(ADJ_THREADS) void k_adj_stage(AdjArgs<R> a) {
     const MeshGeom& m = a.mesh;
     const Physics& ph = a.phys;
 
-    // ---- trial state, formed exactly as the forward stage forms it
+    // ---- trial state, formed exactly as the forward stage forms it, and
+    // the cotangent of this stage's particle derivative.  All of the
+    // particle's loads are issued before any of them is used: split into a
+    // block per quantity, the cotangent loads waited for the trial state's.
     R x = 0, y = 0, pr = 0, pi = 0, chi = 0;
     double z2 = 0.0, g = 1.0;
+    double ax = 0, ay = 0, az = 0, apr = 0, api = 0, ag = 0;
     if (in) {
         const R x0 = a.y0.x[i], y0 = a.y0.y[i], pr0 = a.y0.pr[i], pi0 = a.y0.pi[i];
         const double z20 = a.y0.z2[i], g0 = a.y0.g[i];
+        R kx = 0, ky = 0, kpr = 0, kpi = 0;
+        double kz2 = 0.0, kg = 0.0;
+        if (STAGE > 1) {
+            kx = a.kin.x[i];
+            ky = a.kin.y[i];
+            kz2 = a.kin.z2[i];
+            kpr = a.kin.pr[i];
+            kpi = a.kin.pi[i];
+            kg = a.kin.g[i];
+        }
         chi = a.chi[i];
+        const bool ev = ph.electrons_evolve;
+        double Lx = 0, Ly = 0, Lz2 = 0, Lpr = 0, Lpi = 0, Lg = 0;
+        double nx = 0, ny = 0, nz2 = 0, npr = 0, npi = 0, ng = 0;
+        if (ev) {
+            Lx = (double)a.L.x[i];
+            Ly = (double)a.L.y[i];
+            Lz2 = a.L.z2[i];
+            Lpr = (double)a.L.pr[i];
+            Lpi = (double)a.L.pi[i];
+            Lg = a.L.g[i];
+            if (STAGE < 4) {
+                nx = (double)a.mnext.x[i];
+                ny = (double)a.mnext.y[i];
+                nz2 = a.mnext.z2[i];
+                npr = (double)a.mnext.pr[i];
+                npi = (double)a.mnext.pi[i];
+                ng = a.mnext.g[i];
+            }
+        }
         if (STAGE == 1) {
             x = x0; y = y0; z2 = z20; pr = pr0; pi = pi0; g = g0;
         } else {
             const R c = (R)a.cin;
-            x = fma(c, a.kin.x[i], x0);
-            y = fma(c, a.kin.y[i], y0);
-            z2 = fma(a.cin, a.kin.z2[i], z20);
-            pr = fma(c, a.kin.pr[i], pr0);
-            pi = fma(c, a.kin.pi[i], pi0);
-            g = fma(a.cin, a.kin.g[i], g0);
+            x = fma(c, kx, x0);
+            y = fma(c, ky, y0);
+            z2 = fma(a.cin, kz2, z20);
+            pr = fma(c, kpr, pr0);
+            pi = fma(c, kpi, pi0);
+            g = fma(a.cin, kg, g0);
         }
-    }
-
-    // ---- cotangent of this stage's particle derivative
-    double ax = 0, ay = 0, az = 0, apr = 0, api = 0, ag = 0;
-    if (in && ph.electrons_evolve) {
-        ax = a.ws * (double)a.L.x[i];
-        ay = a.ws * (double)a.L.y[i];
-        az = a.ws * a.L.z2[i];
-        apr = a.ws * (double)a.L.pr[i];
-        api = a.ws * (double)a.L.pi[i];
-        ag = a.ws * a.L.g[i];
-        if (STAGE < 4) {
-            ax += a.cnext * (double)a.mnext.x[i];
-            ay += a.cnext * (double)a.mnext.y[i];
-            az += a.cnext * a.mnext.z2[i];
-            apr += a.cnext * (double)a.mnext.pr[i];
-            api += a.cnext * (double)a.mnext.pi[i];
-            ag += a.cnext * a.mnext.g[i];
+        if (ev) {
+            ax = a.ws * Lx;
+            ay = a.ws * Ly;
+            az = a.ws * Lz2;
+            apr = a.ws * Lpr;
+            api = a.ws * Lpi;
+            ag = a.ws * Lg;
+            if (STAGE < 4) {
+                ax += a.cnext * nx;
+                ay += a.cnext * ny;
+                az += a.cnext * nz2;
+                apr += a.cnext * npr;
+                api += a.cnext * npi;
+                ag += a.cnext * ng;
+            }
         }
     }
 
