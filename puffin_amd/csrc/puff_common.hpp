@@ -102,7 +102,12 @@ constexpr int FIELD_TILE_NODES = PUFF_FIELD_TILE_NODES;
 // Workgroup and LDS tile of the adjoint stage kernels: more live values per
 // particle than the forward, so smaller workgroups with a larger VGPR budget.
 constexpr int ADJ_THREADS = 256;
-constexpr int ADJ_TILE_NODES = 1024;
+// Deposit tile (two doubles per node): 2000 nodes keep a workgroup's LDS
+// (with the field tile) under 40 KiB, four workgroups per CU, as many as the
+// kernel's 116 VGPRs allow.  1024 nodes sent more of C4's sparse-beam
+// workgroups to global atomics: its adjoint stages took 969 ms per 30 steps
+// against 888 ms.
+constexpr int ADJ_TILE_NODES = 2000;
 constexpr int ADJ_FIELD_TILE_NODES = 256;   // LDS copy of the trial field and source cotangent
 constexpr int MAX_BOX_WAVES = (STAGE_THREADS > ADJ_THREADS ? STAGE_THREADS : ADJ_THREADS) / 64;
 
